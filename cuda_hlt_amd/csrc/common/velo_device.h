@@ -1,0 +1,242 @@
+// Device-side definitions shared by the gfx950 kernels: constants, bit
+// layouts, the phi function, the binary searches and wave64 primitives.
+//
+// Float expressions are written one operation per statement where the order
+// matters; the kernels are compiled with -ffp-contract=off so that each
+// operation rounds once, as the CPU oracle does.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "velo_hlt.h"
+
+namespace velo {
+
+// VeloDefinitions.cuh:45-77
+constexpr float kPhiExtrapolation = 0.062f;
+constexpr float kForwardPhiTolerance = 0.052f;
+constexpr float kMaxSlope = 0.4f;
+constexpr float kTolerance = 0.6f;
+constexpr float kMaxScatterSeeding = 0.004f;
+constexpr float kMaxScatterForwarding = 0.004f;
+constexpr uint32_t kMaxSkippedModules = 1u;
+constexpr float kParamW = 3966.94f;
+constexpr float kMaxChi2 = 20.0f;
+constexpr float kPixelSize = 0.055f;
+constexpr float kFltMax = 3.402823466e+38f;
+
+// ClusteringDefinitions.cuh:14-21
+constexpr uint32_t kMaskBottom = 0xFFFEFFFFu;
+constexpr uint32_t kMaskTop = 0xFFFF7FFFu;
+constexpr uint32_t kMaskTopLeft = 0x7FFF7FFFu;
+constexpr uint32_t kMaskBottomRight = 0xFFFEFFFEu;
+constexpr uint32_t kMaskLtrTopRight = 0x7FFF0000u;
+constexpr uint32_t kMaskRtlBottomLeft = 0x0000FFFEu;
+constexpr int kMaxClusteringIterations = 12;
+
+constexpr uint32_t kWave = 64;
+
+// ---------------------------------------------------------------------------
+// phi: atan2 from a minimax polynomial, atan(t) = t * P(t^2) on [0, 1].
+// The same coefficients and operation order are used by the CPU oracle.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline float atan2_poly(float y, float x)
+{
+  constexpr float c0 = 1.0f, c1 = -0.3333328366279602f, c2 = 0.1999826729297638f,
+                  c3 = -0.14261946082115173f, c4 = 0.10942086577415466f, c5 = -0.0837746188044548f,
+                  c6 = 0.05755440890789032f, c7 = -0.03080674819648266f, c8 = 0.010728039778769016f,
+                  c9 = -0.001754164695739746f;
+  constexpr float kPi = 3.14159274101257324f;
+  constexpr float kPi2 = 1.57079637050628662f;
+  const float ax = fabsf(x);
+  const float ay = fabsf(y);
+  const float mx = ax > ay ? ax : ay;
+  const float mn = ax > ay ? ay : ax;
+  float a;
+  if (mx == 0.0f) {
+    a = 0.0f;
+  }
+  else {
+    const float t = mn / mx;
+    const float s = t * t;
+    float p = c9;
+    p = p * s; p = p + c8;
+    p = p * s; p = p + c7;
+    p = p * s; p = p + c6;
+    p = p * s; p = p + c5;
+    p = p * s; p = p + c4;
+    p = p * s; p = p + c3;
+    p = p * s; p = p + c2;
+    p = p * s; p = p + c1;
+    p = p * s; p = p + c0;
+    a = t * p;
+  }
+  if (ay > ax) a = kPi2 - a;
+  if (signbit(x)) a = kPi - a;
+  if (signbit(y)) a = -a;
+  return a;
+}
+
+// VeloTools.cu:3-10: odd modules in (-pi, pi], even modules in [0, 2pi).
+__host__ __device__ inline float phi_odd(float x, float y) { return atan2_poly(y, x); }
+__host__ __device__ inline float phi_even(float x, float y)
+{
+  const float phi = atan2_poly(y, x);
+  const float shift = phi < 0.0f ? 6.28318548202514648f : 0.0f;
+  return phi + shift;
+}
+
+// VeloTools.cuh:6-26 (array_size 0 still inspects array[0]).
+__device__ inline int bs_first_candidate(const float* a, uint32_t size, float value, float margin)
+{
+  bool found = false;
+  int l = 0;
+  int r = (int) (size - 1u);
+  while (l < r) {
+    const int m = (l + r) / 2;
+    const float el = a[m];
+    const float d = value - el;
+    found |= fabsf(d) < margin;
+    const float lo = value - margin;
+    if (lo > el) l = m + 1;
+    else r = m;
+  }
+  const float d = value - a[l];
+  found |= fabsf(d) < margin;
+  return found ? l : -1;
+}
+
+// VeloTools.cuh:28-45
+__device__ inline int bs_second_candidate(const float* a, uint32_t size, float value, float margin)
+{
+  int l = 0;
+  int r = (int) (size - 1u);
+  while (l < r) {
+    const int m = (l + r) / 2;
+    const float hi = value + margin;
+    if (hi > a[m]) l = m + 1;
+    else r = m;
+  }
+  const float d = value - a[l];
+  return fabsf(d) < margin ? l + 1 : l;
+}
+
+// ---------------------------------------------------------------------------
+// raw format (ClusteringDefinitions.cu:3-21)
+// ---------------------------------------------------------------------------
+struct RawEvent {
+  uint32_t n_banks;
+  const uint32_t* offsets;
+  const char* payload;
+  __device__ explicit RawEvent(const char* ev)
+  {
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(ev);
+    n_banks = w[0];
+    offsets = w + 1;
+    payload = ev + 4 * (n_banks + 2);
+  }
+};
+
+struct RawBank {
+  uint32_t sensor_index;
+  uint32_t sp_count;
+  const uint32_t* sp_word;
+  __device__ RawBank(const RawEvent& e, uint32_t b)
+  {
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(e.payload + e.offsets[b]);
+    sensor_index = w[0];
+    sp_count = w[1];
+    sp_word = w + 2;
+  }
+};
+
+__host__ __device__ inline uint32_t lhcb_id(uint32_t sensor, uint32_t cx, uint32_t cy)
+{
+  return 0x80000000u + ((sensor << 18) | ((cx / 256u) << 16) | ((cx % 256u) << 8) | cy);
+}
+
+// EstimateInputSize.cu:71-79
+__device__ inline uint32_t isolated_cluster_count(uint32_t sp)
+{
+  const bool p0 = (sp & 0x88u) && !(sp & 0x44u) && (sp & 0x33u);
+  const bool p1 = (sp & 0xCCu) && !(sp & 0x22u) && (sp & 0x11u);
+  return 1u + (uint32_t) (p0 | p1);
+}
+
+// MaskedVeloClustering.cu:4-46
+__device__ inline uint32_t current_mask(uint32_t p)
+{
+  return ((p & kMaskTop) << 1) | ((p & kMaskBottom) >> 1) | ((p & kMaskBottomRight) << 15) |
+         ((p & kMaskTopLeft) >> 15) | (p >> 16) | (p >> 17) | (p << 16) | (p << 17);
+}
+__device__ inline uint32_t mask_ltr(uint32_t p) { return ((p & kMaskLtrTopRight) >> 15) | (p >> 16) | (p >> 17); }
+__device__ inline uint32_t mask_rtl(uint32_t p) { return ((p & kMaskRtlBottomLeft) << 15) | (p << 16) | (p << 17); }
+__device__ inline uint32_t cluster_current_mask(uint32_t p)
+{
+  return ((p & kMaskTop) << 1) | ((p & kMaskBottomRight) << 15) | (p << 16) | (p << 17);
+}
+__device__ inline uint32_t cand_mask_rc(uint32_t p, uint32_t rp)
+{
+  return ((p & kMaskBottom) >> 1) & (((p & kMaskTopLeft) >> 15) | (rp << 17)) & ((p >> 16) | (rp << 16)) &
+         ((p >> 17) | ((rp & kMaskRtlBottomLeft) << 15));
+}
+
+// ---------------------------------------------------------------------------
+// wave64 primitives
+// ---------------------------------------------------------------------------
+__device__ inline uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+__device__ inline uint64_t ballot(bool p) { return __ballot(p); }
+
+// number of set bits of m in lanes below the current one
+__device__ inline uint32_t count_below(uint64_t m)
+{
+  return __builtin_amdgcn_mbcnt_hi((uint32_t) (m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t) m, 0u));
+}
+
+// Exclusive prefix sum across the wave of small non-negative values (< 4).
+__device__ inline uint32_t wave_prefix_small(uint32_t v, uint32_t* total)
+{
+  const uint64_t b0 = ballot(v & 1u);
+  const uint64_t b1 = ballot(v & 2u);
+  *total = (uint32_t) __popcll(b0) + 2u * (uint32_t) __popcll(b1);
+  return count_below(b0) + 2u * count_below(b1);
+}
+
+__device__ inline uint32_t wave_sum(uint32_t v)
+{
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Inclusive prefix sum across the wave (Hillis-Steele on 64 lanes).
+__device__ inline uint32_t wave_inclusive_scan(uint32_t v)
+{
+  const uint32_t l = lane_id();
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t u = __shfl_up(v, o, 64);
+    if (l >= (uint32_t) o) v += u;
+  }
+  return v;
+}
+
+// Orders this wave's LDS and global accesses (a wave-local barrier).
+__device__ inline void wave_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline uint32_t event_index(const uint32_t* event_list, uint32_t e)
+{
+  return event_list ? event_list[e] : e;
+}
+
+__device__ inline void set_status(uint32_t* status, uint32_t e, uint32_t bits)
+{
+  if (status && bits) atomicOr(status + e, bits);
+}
+
+} // namespace velo

@@ -1,0 +1,584 @@
+// Raw SuperPixel decoding: init_event_list, estimate_input_size and masked
+// clustering for gfx950, plus the device constants.
+//
+// Work layout: one workgroup per event; wave w of the workgroup owns modules
+// w, w+W, w+2W, ... (W = waves per workgroup).  A wave walks the four banks
+// of its module, stages each bank's SP words in its own LDS slice (coalesced
+// load), and gives one SP to each lane.  Each lane resolves its SP's
+// neighbourhood by scanning the staged bank; output slots come from a
+// ballot-based wave prefix sum, so clusters are written in bank, SP,
+// cluster order with no atomics and no cross-wave synchronisation.
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "velo_device.h"
+
+namespace velo {
+
+constexpr uint32_t kDefaultBlock = 256;
+constexpr uint32_t kDefaultBankTile = 512; // SP words staged per wave
+
+// Constants.cu:36
+__constant__ uint8_t c_candidate_ks[9] = {0, 0, 1, 4, 4, 5, 5, 5, 5};
+
+// Candidate byte of a non-isolated SP (EstimateInputSize.cu:118-183).  Every
+// SP of the bank enters the neighbour test, isolated or not (:126, Q1).
+__device__ inline uint32_t neighbour_pixels(uint32_t w, const uint32_t* __restrict__ words, uint32_t n)
+{
+  const uint32_t sp_addr = (w & 0x007FFF00u) >> 8;
+  const uint32_t sp = w & 0xFFu;
+  uint32_t pixels = (sp & 0x0Fu) | ((sp & 0xF0u) << 2);
+  const uint32_t sp_row = sp_addr & 0x3Fu;
+  const uint32_t sp_col = sp_addr >> 6;
+  for (uint32_t k = 0; k < n; ++k) {
+    const uint32_t o = words[k];
+    const uint32_t o_addr = (o & 0x007FFF00u) >> 8;
+    const uint32_t o_row = o_addr & 0x3Fu;
+    const uint32_t o_col = o_addr >> 6;
+    const uint32_t osp = o & 0xFFu;
+    const uint32_t dcol = o_col - sp_col; // 0 or 1 for the columns of interest
+    const uint32_t drow = o_row - sp_row; // 1, 0 or -1
+    if (dcol == 0u && drow == 1u) pixels |= ((osp & 0x01u) | ((osp & 0x10u) << 2)) << 4;
+    if (dcol == 1u && drow == 1u) pixels |= (osp & 0x01u) << 16;
+    if (dcol == 1u && drow == 0u) pixels |= (osp & 0x0Fu) << 12;
+    if (dcol == 1u && drow == 0xFFFFFFFFu) pixels |= (osp & 0x08u) << 8;
+    if (dcol == 0u && drow == 0xFFFFFFFFu) pixels |= (osp & 0x80u) >> 2;
+  }
+  return pixels;
+}
+
+__device__ inline uint32_t candidates_from_pixels(uint32_t pixels)
+{
+  const uint32_t inside = pixels & 0x3CFu;
+  const uint32_t mask = (inside << 1) | (inside << 5) | (inside << 6) | (inside << 7);
+  const uint32_t working = mask & ~pixels;
+  const uint32_t cand = (working >> 1) & (working >> 5) & (working >> 6) & (working >> 7) & pixels;
+  return (cand & 0x03u) | ((cand & 0xC0u) >> 4) | ((cand & 0x0Cu) << 2) | ((cand & 0x0300u) >> 2);
+}
+
+// The 3x4 SP window around (sp_row, sp_col) built from non-isolated SPs
+// (MaskedVeloClustering.cu:221-272).
+__device__ inline void load_window(
+  int32_t sp_row, int32_t sp_col, const uint32_t* __restrict__ words, uint32_t n, uint32_t pa[3])
+{
+  pa[0] = pa[1] = pa[2] = 0;
+  const int32_t row_lo = sp_row - 2;
+  const int32_t col_lo = sp_col - 1;
+  for (uint32_t k = 0; k < n; ++k) {
+    const uint32_t o = words[k];
+    if (o & 0x80000000u) continue;
+    const uint32_t o_addr = (o & 0x007FFF00u) >> 8;
+    const int32_t rr = (int32_t) (o_addr & 0x3Fu) - row_lo;
+    const int32_t rc = (int32_t) (o_addr >> 6) - col_lo;
+    if ((uint32_t) rr < 4u && (uint32_t) rc < 3u) {
+      const uint32_t osp = o & 0xFFu;
+      const uint32_t bits = ((osp & 0x0Fu) << (4 * rr)) | ((osp & 0xF0u) << (12 + 4 * rr));
+      if (rc == 0) pa[0] |= bits;
+      else if (rc == 1) pa[1] |= bits;
+      else pa[2] |= bits;
+    }
+  }
+}
+
+struct Hit {
+  float x, y, z;
+  uint32_t id;
+};
+
+// Local -> global, MaskedVeloClustering.cu:122-138 and :431-449.
+__device__ inline Hit make_hit(const char* geometry, uint32_t sensor, uint32_t cx, uint32_t cy, float fx, float fy)
+{
+  const float* local_x = reinterpret_cast<const float*>(geometry);
+  const float* x_pitch = local_x + 768;
+  const float* ltg = x_pitch + 768 + 16 * sensor;
+  const float t = fx * x_pitch[cx];
+  const float lx = local_x[cx] + t;
+  double dly = (double) cy + 0.5;
+  dly = dly + (double) fy;
+  dly = dly * (double) kPixelSize;
+  const float ly = (float) dly;
+  Hit h;
+  float a = ltg[0] * lx;
+  float b = ltg[1] * ly;
+  h.x = a + b;
+  h.x = h.x + ltg[9];
+  a = ltg[3] * lx;
+  b = ltg[4] * ly;
+  h.y = a + b;
+  h.y = h.y + ltg[10];
+  a = ltg[6] * lx;
+  b = ltg[7] * ly;
+  h.z = a + b;
+  h.z = h.z + ltg[11];
+  h.id = lhcb_id(sensor, cx, cy);
+  return h;
+}
+
+// Cluster grown from candidate pixel k (MaskedVeloClustering.cu:274-461).
+__device__ inline bool cluster_candidate(
+  uint32_t pa0, uint32_t pa1, uint32_t pa2, int32_t sp_row, int32_t sp_col, uint32_t k, const char* geometry,
+  uint32_t sensor, Hit* out)
+{
+  const int32_t row_lower_limit = (sp_row - 2) * 4;
+  const int32_t col_lower_limit = (sp_col - 1) * 2;
+  const uint32_t row = (uint32_t) (sp_row * 4) + (k % 4u);
+  const uint32_t col = (uint32_t) (sp_col * 2) + (k >= 4u ? 1u : 0u);
+  const uint32_t start = (1u << (row - (uint32_t) row_lower_limit)) << (16u * (col % 2u));
+  uint32_t c0 = 0, c1 = start, c2 = 0;
+  uint32_t w0 = 0, w1 = start, w2 = 0;
+  pa1 &= ~c1;
+  for (int it = 0; it < kMaxClusteringIterations; ++it) {
+    const uint32_t m0 = current_mask(w0) | mask_rtl(w1);
+    const uint32_t m1 = current_mask(w1) | mask_rtl(w2) | mask_ltr(w0);
+    const uint32_t m2 = current_mask(w2) | mask_ltr(w1);
+    w0 = pa0 & m0;
+    w1 = pa1 & m1;
+    w2 = pa2 & m2;
+    if (!(w0 | w1 | w2)) break;
+    c0 |= w0;
+    c1 |= w1;
+    c2 |= w2;
+    pa0 &= ~c0;
+    pa1 &= ~c1;
+    pa2 &= ~c2;
+  }
+  if (c2) return false;
+  const int n = __popc(c0) + __popc(c1);
+  if (n >= 3) {
+    const uint32_t x0 = cluster_current_mask(c0) & ~c0;
+    const uint32_t x1 = (cluster_current_mask(c1) | mask_ltr(c0)) & ~c1;
+    const uint32_t x2 = mask_ltr(c1);
+    const uint32_t k0 = cand_mask_rc(x0, x1) & c0;
+    const uint32_t k1 = (cand_mask_rc(x1, x2) & c1) ^ start;
+    if (k0 | k1) {
+      const int32_t neg = (int32_t) ~(0u - start);
+      const bool under = start < 4096u;
+      uint32_t smaller = under ? (0xFFFu & (uint32_t) neg) : (0xFFFu & (uint32_t) (neg >> 16));
+      smaller |= smaller << 16;
+      const uint32_t cur = smaller | (under ? 0xFFFF0000u : 0u);
+      if (((k0 & smaller) | (k1 & cur)) != 0u) return false;
+    }
+  }
+  // centroid: sums of absolute pixel columns and rows
+  const int x = __popc(c0 & 0xFFFFu) * col_lower_limit + __popc(c0 >> 16) * (col_lower_limit + 1) +
+                __popc(c1 & 0xFFFFu) * (col_lower_limit + 2) + __popc(c1 >> 16) * (col_lower_limit + 3);
+  int y = 0;
+  for (int r = 0; r < 16; ++r) {
+    const uint32_t m = (1u << r) | (1u << (r + 16));
+    y += (__popc(c0 & m) + __popc(c1 & m)) * (row_lower_limit + r);
+  }
+  const uint32_t cx = (uint32_t) (x / n);
+  const uint32_t cy = (uint32_t) (y / n);
+  const float fxq = (float) x / (float) n;
+  const float fyq = (float) y / (float) n;
+  const float fx = fxq - (float) cx;
+  const float fy = fyq - (float) cy;
+  *out = make_hit(geometry, sensor, cx, cy, fx, fy);
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// K0
+// ---------------------------------------------------------------------------
+__global__ void init_event_list_kernel(uint32_t* event_list, uint32_t n)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) event_list[i] = i;
+}
+
+// ---------------------------------------------------------------------------
+// K1: estimate_input_size
+// ---------------------------------------------------------------------------
+// Stages bank b's words in this wave's LDS slice when they fit; returns the
+// pointer the lanes should scan.
+__device__ inline const uint32_t* stage_bank(const RawBank& bank, uint32_t* slice, uint32_t tile)
+{
+  if (bank.sp_count > tile) return bank.sp_word;
+  for (uint32_t i = lane_id(); i < bank.sp_count; i += kWave) slice[i] = bank.sp_word[i];
+  wave_sync();
+  return slice;
+}
+
+__global__ void estimate_input_size_kernel(velo_estimate_input_size_args a)
+{
+  extern __shared__ uint32_t lds[];
+  __shared__ uint32_t module_cands[VELO_N_MODULES];
+  __shared__ uint32_t module_cand_offset[VELO_N_MODULES + 1];
+  const uint32_t e = blockIdx.x;
+  const uint32_t waves = blockDim.x / kWave;
+  const uint32_t wave = threadIdx.x / kWave;
+  const uint32_t lane = lane_id();
+  const uint32_t tile = a.tuning.lds_tile;
+  uint32_t* slice = lds + wave * tile;
+  const RawEvent ev(a.dev_raw_input + a.dev_raw_input_offsets[event_index(a.dev_event_list, e)]);
+  const uint32_t n_banks = ev.n_banks < VELO_N_SENSORS ? ev.n_banks : VELO_N_SENSORS;
+  uint32_t status = 0;
+
+  // pass 1: estimates and candidate counts per module
+  for (uint32_t m = wave; m < VELO_N_MODULES; m += waves) {
+    uint32_t est = 0, ncand = 0;
+    for (uint32_t b = 4 * m; b < 4 * m + 4 && b < n_banks; ++b) {
+      const RawBank bank(ev, b);
+      if (bank.sensor_index != b) {
+        status |= VELO_STATUS_BAD_BANK;
+        continue;
+      }
+      if (bank.sp_count > 256u) status |= VELO_STATUS_SP_INDEX_TRUNCATED;
+      const uint32_t* words = stage_bank(bank, slice, tile);
+      for (uint32_t i = lane; i < bank.sp_count; i += kWave) {
+        const uint32_t w = words[i];
+        if (w & 0x80000000u) {
+          est += isolated_cluster_count(w & 0xFFu);
+        }
+        else {
+          const uint32_t c = candidates_from_pixels(neighbour_pixels(w, words, bank.sp_count));
+          const uint32_t f = ((c & 0x0Fu) ? 1u : 0u) + ((c & 0xF0u) ? 1u : 0u);
+          est += f;
+          ncand += f;
+        }
+      }
+      wave_sync();
+    }
+    est = wave_sum(est);
+    ncand = wave_sum(ncand);
+    if (lane == 0) {
+      a.dev_estimated_input_size[e * VELO_N_MODULES + m] = est;
+      a.dev_module_cluster_num[e * VELO_N_MODULES + m] = 0;
+      module_cands[m] = ncand;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < kWave) {
+    // exclusive scan of the 52 module candidate counts
+    const uint32_t v = lane < VELO_N_MODULES ? module_cands[lane] : 0u;
+    const uint32_t inc = wave_inclusive_scan(v);
+    if (lane < VELO_N_MODULES) module_cand_offset[lane] = inc - v;
+    if (lane == VELO_N_MODULES - 1) module_cand_offset[VELO_N_MODULES] = inc;
+  }
+  __syncthreads();
+
+  // pass 2: candidates in bank, SP, low-then-high-nibble order
+  uint32_t* cands = a.dev_cluster_candidates + (size_t) e * VELO_MAX_CANDIDATES_EVENT;
+  for (uint32_t m = wave; m < VELO_N_MODULES; m += waves) {
+    uint32_t pos = module_cand_offset[m];
+    if (module_cands[m] == 0) continue;
+    for (uint32_t b = 4 * m; b < 4 * m + 4 && b < n_banks; ++b) {
+      const RawBank bank(ev, b);
+      if (bank.sensor_index != b) continue;
+      const uint32_t* words = stage_bank(bank, slice, tile);
+      for (uint32_t base = 0; base < bank.sp_count; base += kWave) {
+        const uint32_t i = base + lane;
+        uint32_t c = 0;
+        if (i < bank.sp_count) {
+          const uint32_t w = words[i];
+          if (!(w & 0x80000000u)) c = candidates_from_pixels(neighbour_pixels(w, words, bank.sp_count));
+        }
+        const uint32_t cnt = ((c & 0x0Fu) ? 1u : 0u) + ((c & 0xF0u) ? 1u : 0u);
+        uint32_t total;
+        const uint32_t pre = wave_prefix_small(cnt, &total);
+        uint32_t slot = pos + pre;
+        if (c & 0x0Fu) {
+          if (slot < VELO_MAX_CANDIDATES_EVENT) cands[slot] = (i << 11) | (b << 3) | c_candidate_ks[c & 0x0Fu];
+          ++slot;
+        }
+        if (c & 0xF0u) {
+          if (slot < VELO_MAX_CANDIDATES_EVENT) cands[slot] = (i << 11) | (b << 3) | (c_candidate_ks[c >> 4] + 2u);
+        }
+        pos += total;
+      }
+      wave_sync();
+    }
+  }
+  const uint32_t total_cands = module_cand_offset[VELO_N_MODULES];
+  if (threadIdx.x == 0) {
+    a.dev_event_candidate_num[e] = total_cands < VELO_MAX_CANDIDATES_EVENT ? total_cands : VELO_MAX_CANDIDATES_EVENT;
+    if (total_cands > VELO_MAX_CANDIDATES_EVENT) status |= VELO_STATUS_CANDIDATES_OVERFLOW;
+  }
+  if (a.dev_velo_event_status) {
+    // OR the per-lane status bits of the whole workgroup into the event word
+    const uint64_t any = ballot(status != 0);
+    if (any) {
+      uint32_t s = status;
+      for (int o = 32; o > 0; o >>= 1) s |= __shfl_xor(s, o, 64);
+      if (lane == 0 && s) atomicOr(a.dev_velo_event_status + e, s);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K5: masked clustering
+// ---------------------------------------------------------------------------
+__global__ void masked_clustering_kernel(velo_masked_clustering_args a)
+{
+  extern __shared__ uint32_t lds[];
+  const uint32_t e = blockIdx.x;
+  const uint32_t waves = blockDim.x / kWave;
+  const uint32_t wave = threadIdx.x / kWave;
+  const uint32_t lane = lane_id();
+  const uint32_t tile = a.tuning.lds_tile;
+  uint32_t* slice = lds + wave * tile;
+  const RawEvent ev(a.dev_raw_input + a.dev_raw_input_offsets[event_index(a.dev_event_list, e)]);
+  const uint32_t n_banks = ev.n_banks < VELO_N_SENSORS ? ev.n_banks : VELO_N_SENSORS;
+  const uint32_t total = a.dev_module_cluster_start[VELO_N_MODULES * a.n_events];
+  float* xs = reinterpret_cast<float*>(a.dev_velo_cluster_container);
+  float* ys = xs + total;
+  float* zs = ys + total;
+  uint32_t* ids = a.dev_velo_cluster_container + 3 * (size_t) total;
+  const uint32_t* starts = a.dev_module_cluster_start + e * VELO_N_MODULES;
+  uint32_t status = 0;
+  uint32_t event_hits = 0;
+
+  for (uint32_t m = wave; m < VELO_N_MODULES; m += waves) {
+    const uint32_t mstart = starts[m];
+    uint32_t pos = 0;
+    for (uint32_t b = 4 * m; b < 4 * m + 4 && b < n_banks; ++b) {
+      const RawBank bank(ev, b);
+      if (bank.sensor_index != b) continue;
+      const uint32_t* words = stage_bank(bank, slice, tile);
+      const uint32_t n = bank.sp_count;
+      for (uint32_t base = 0; base < n; base += kWave) {
+        const uint32_t i = base + lane;
+        Hit h0 {}, h1 {};
+        uint32_t cnt = 0;
+        if (i < n) {
+          const uint32_t w = words[i];
+          const uint32_t sp_addr = (w & 0x007FFF00u) >> 8;
+          const uint32_t sp = w & 0xFFu;
+          if (w & 0x80000000u) {
+            // isolated SP: pre-computed pattern (MaskedVeloClustering.cu:105-167)
+            const uint32_t idx = a.dev_velo_sp_patterns[sp];
+            const uint32_t sp_row = sp_addr & 0x3Fu, sp_col = sp_addr >> 6;
+            h0 = make_hit(a.dev_velo_geometry, bank.sensor_index, sp_col * 2u + ((idx >> 2) & 1u),
+                          sp_row * 4u + (idx & 3u), a.dev_velo_sp_fx[2 * sp], a.dev_velo_sp_fy[2 * sp]);
+            cnt = 1;
+            if (idx & 8u) {
+              h1 = make_hit(a.dev_velo_geometry, bank.sensor_index, sp_col * 2u + ((idx >> 6) & 1u),
+                            sp_row * 4u + ((idx >> 4) & 3u), a.dev_velo_sp_fx[2 * sp + 1],
+                            a.dev_velo_sp_fy[2 * sp + 1]);
+              cnt = 2;
+            }
+          }
+          else {
+            const uint32_t c = candidates_from_pixels(neighbour_pixels(w, words, n));
+            if (c) {
+              // the reference decodes the candidate's SP index as uint8_t (:179)
+              const uint32_t ci = i & 0xFFu;
+              const uint32_t cw = ci == i ? w : words[ci];
+              const uint32_t caddr = (cw & 0x007FFF00u) >> 8;
+              const int32_t sp_row = (int32_t) (caddr & 0x3Fu), sp_col = (int32_t) (caddr >> 6);
+              uint32_t pa[3];
+              load_window(sp_row, sp_col, words, n, pa);
+              Hit t;
+              if ((c & 0x0Fu) && cluster_candidate(pa[0], pa[1], pa[2], sp_row, sp_col, c_candidate_ks[c & 0x0Fu],
+                                                   a.dev_velo_geometry, bank.sensor_index, &t)) {
+                h0 = t;
+                cnt = 1;
+              }
+              if ((c & 0xF0u) && cluster_candidate(pa[0], pa[1], pa[2], sp_row, sp_col,
+                                                   c_candidate_ks[c >> 4] + 2u, a.dev_velo_geometry,
+                                                   bank.sensor_index, &t)) {
+                if (cnt == 0) h0 = t;
+                else h1 = t;
+                ++cnt;
+              }
+            }
+          }
+        }
+        uint32_t tot;
+        const uint32_t pre = wave_prefix_small(cnt, &tot);
+        if (cnt > 0) {
+          const uint32_t slot = mstart + pos + pre;
+          xs[slot] = h0.x;
+          ys[slot] = h0.y;
+          zs[slot] = h0.z;
+          ids[slot] = h0.id;
+        }
+        if (cnt > 1) {
+          const uint32_t slot = mstart + pos + pre + 1;
+          xs[slot] = h1.x;
+          ys[slot] = h1.y;
+          zs[slot] = h1.z;
+          ids[slot] = h1.id;
+        }
+        pos += tot;
+      }
+      wave_sync();
+    }
+    if (lane == 0) a.dev_module_cluster_num[e * VELO_N_MODULES + m] = pos;
+    if (pos > VELO_MAX_NUMHITS_IN_MODULE) status |= VELO_STATUS_MODULE_HITS_OVERFLOW;
+    event_hits += pos;
+  }
+  if (a.dev_velo_event_status) {
+    __shared__ uint32_t hits_sum;
+    if (threadIdx.x == 0) hits_sum = 0;
+    __syncthreads();
+    if (lane == 0) atomicAdd(&hits_sum, event_hits);
+    __syncthreads();
+    if (threadIdx.x == 0 && hits_sum >= VELO_MAX_HITS_PER_EVENT) status |= VELO_STATUS_EVENT_HITS_OVERFLOW;
+    if (lane == 0 && status) atomicOr(a.dev_velo_event_status + e, status);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// SP pattern tables (ClusteringCommon.cpp:3-101), built once on the host
+// ---------------------------------------------------------------------------
+static void build_sp_tables(uint8_t* patterns, uint8_t* sizes, float* fx, float* fy)
+{
+  std::memset(patterns, 0, 256);
+  std::memset(sizes, 0, 256);
+  for (int i = 0; i < 512; ++i) fx[i] = fy[i] = 0.f;
+  for (uint32_t sp = 0; sp < 256; ++sp) {
+    // label the 8 pixels (bit b: col b/4, row b%4) by 8-connected components,
+    // components ordered by their lowest bit
+    int label[8];
+    for (int b = 0; b < 8; ++b) label[b] = -1;
+    int n_labels = 0;
+    for (int b = 0; b < 8; ++b) {
+      if (!(sp & (1u << b)) || label[b] >= 0) continue;
+      int stack[8], top = 0;
+      stack[top++] = b;
+      label[b] = n_labels;
+      while (top) {
+        const int p = stack[--top];
+        const int pr = p % 4, pc = p / 4;
+        for (int q = 0; q < 8; ++q) {
+          if (!(sp & (1u << q)) || label[q] >= 0) continue;
+          const int qr = q % 4, qc = q / 4;
+          if (abs(qr - pr) <= 1 && abs(qc - pc) <= 1) {
+            label[q] = n_labels;
+            stack[top++] = q;
+          }
+        }
+      }
+      ++n_labels;
+    }
+    for (int l = 0; l < n_labels && l < 2; ++l) {
+      uint32_t x = 0, y = 0, n = 0;
+      for (int b = 0; b < 8; ++b)
+        if (label[b] == l) {
+          x += (uint32_t) (b / 4);
+          y += (uint32_t) (b % 4);
+          ++n;
+        }
+      const uint32_t cx = x / n, cy = y / n;
+      sizes[sp] |= (uint8_t) (n << (4 * l));
+      patterns[sp] |= (uint8_t) (((cx << 2) | cy) << (4 * l));
+      patterns[sp] |= (uint8_t) (l << 3);
+      fx[2 * sp + l] = (float) x / (float) n - (float) cx;
+      fy[2 * sp + l] = (float) y / (float) n - (float) cy;
+    }
+  }
+}
+
+static uint32_t block_of(const velo_tuning& t)
+{
+  uint32_t b = t.block_dim ? t.block_dim : kDefaultBlock;
+  if (b % kWave || b > 1024) return 0;
+  return b;
+}
+
+} // namespace velo
+
+using namespace velo;
+
+extern "C" {
+
+void velo_sp_tables(uint8_t* patterns, uint8_t* sizes, float* fx, float* fy)
+{
+  build_sp_tables(patterns, sizes, fx, fy);
+}
+
+int velo_constants_init(const void* geometry, size_t geometry_bytes, const float* module_zs, velo_constants* out)
+{
+  // Constants.cu:27-31
+  static const float default_zs[VELO_N_MODULES] = {
+    -287.5f, -275.f, -262.5f, -250.f, -237.5f, -225.f, -212.5f, -200.f, -137.5f, -125.f, -62.5f, -50.f, -37.5f,
+    -25.f,   -12.5f, 0.f,     12.5f,  25.f,    37.5f,  50.f,    62.5f,  75.f,    87.5f,  100.f,  112.5f, 125.f,
+    137.5f,  150.f,  162.5f,  175.f,  187.5f,  200.f,  212.5f,  225.f,  237.5f,  250.f,  262.5f, 275.f,  312.5f,
+    325.f,   387.5f, 400.f,   487.5f, 500.f,   587.5f, 600.f,   637.5f, 650.f,   687.5f, 700.f,  737.5f, 750.f};
+  if (!out || !geometry || geometry_bytes != VELO_GEOMETRY_BYTES) return (int) hipErrorInvalidValue;
+  std::memset(out, 0, sizeof(*out));
+  uint8_t patterns[256], sizes[256];
+  float fx[512], fy[512];
+  build_sp_tables(patterns, sizes, fx, fy);
+  static const uint8_t ks[9] = {0, 0, 1, 4, 4, 5, 5, 5, 5};
+  hipError_t err = hipSuccess;
+#define VELO_TRY(x) \
+  do {              \
+    err = (x);      \
+    if (err != hipSuccess) goto fail; \
+  } while (0)
+  VELO_TRY(hipMalloc((void**) &out->dev_velo_sp_patterns, 256));
+  VELO_TRY(hipMalloc((void**) &out->dev_velo_sp_fx, 512 * sizeof(float)));
+  VELO_TRY(hipMalloc((void**) &out->dev_velo_sp_fy, 512 * sizeof(float)));
+  VELO_TRY(hipMalloc((void**) &out->dev_velo_candidate_ks, 9));
+  VELO_TRY(hipMalloc((void**) &out->dev_velo_module_zs, VELO_N_MODULES * sizeof(float)));
+  VELO_TRY(hipMalloc((void**) &out->dev_velo_geometry, VELO_GEOMETRY_BYTES));
+  VELO_TRY(hipMemcpy(out->dev_velo_sp_patterns, patterns, 256, hipMemcpyHostToDevice));
+  VELO_TRY(hipMemcpy(out->dev_velo_sp_fx, fx, sizeof(fx), hipMemcpyHostToDevice));
+  VELO_TRY(hipMemcpy(out->dev_velo_sp_fy, fy, sizeof(fy), hipMemcpyHostToDevice));
+  VELO_TRY(hipMemcpy(out->dev_velo_candidate_ks, ks, 9, hipMemcpyHostToDevice));
+  VELO_TRY(hipMemcpy(out->dev_velo_module_zs, module_zs ? module_zs : default_zs, VELO_N_MODULES * sizeof(float),
+                     hipMemcpyHostToDevice));
+  VELO_TRY(hipMemcpy(out->dev_velo_geometry, geometry, VELO_GEOMETRY_BYTES, hipMemcpyHostToDevice));
+#undef VELO_TRY
+  return 0;
+fail:
+  velo_constants_free(out);
+  return (int) err;
+}
+
+int velo_constants_free(velo_constants* c)
+{
+  if (!c) return (int) hipErrorInvalidValue;
+  void* ptrs[] = {c->dev_velo_sp_patterns, c->dev_velo_sp_fx, c->dev_velo_sp_fy, c->dev_velo_candidate_ks,
+                  c->dev_velo_module_zs, c->dev_velo_geometry};
+  for (void* p : ptrs)
+    if (p) (void) hipFree(p);
+  std::memset(c, 0, sizeof(*c));
+  return 0;
+}
+
+int velo_init_event_list_launch(const velo_init_event_list_args* a, void* stream)
+{
+  if (!a || (!a->dev_event_list && a->n_events)) return (int) hipErrorInvalidValue;
+  if (a->n_events == 0) return 0;
+  const uint32_t block = 256;
+  hipLaunchKernelGGL(init_event_list_kernel, dim3((a->n_events + block - 1) / block), dim3(block), 0,
+                     (hipStream_t) stream, a->dev_event_list, a->n_events);
+  return (int) hipGetLastError();
+}
+
+int velo_estimate_input_size_launch(const velo_estimate_input_size_args* in, void* stream)
+{
+  if (!in || !in->dev_raw_input || !in->dev_raw_input_offsets || !in->dev_estimated_input_size ||
+      !in->dev_module_cluster_num || !in->dev_event_candidate_num || !in->dev_cluster_candidates)
+    return (int) hipErrorInvalidValue;
+  if (in->n_events == 0) return 0;
+  velo_estimate_input_size_args a = *in;
+  const uint32_t block = block_of(a.tuning);
+  if (!block) return (int) hipErrorInvalidValue;
+  if (!a.tuning.lds_tile) a.tuning.lds_tile = kDefaultBankTile;
+  const size_t shmem = (size_t) (block / kWave) * a.tuning.lds_tile * sizeof(uint32_t);
+  hipLaunchKernelGGL(estimate_input_size_kernel, dim3(a.n_events), dim3(block), shmem, (hipStream_t) stream, a);
+  return (int) hipGetLastError();
+}
+
+int velo_masked_clustering_launch(const velo_masked_clustering_args* in, void* stream)
+{
+  if (!in || !in->dev_raw_input || !in->dev_raw_input_offsets || !in->dev_module_cluster_start ||
+      !in->dev_module_cluster_num || !in->dev_velo_cluster_container || !in->dev_velo_geometry ||
+      !in->dev_velo_sp_patterns || !in->dev_velo_sp_fx || !in->dev_velo_sp_fy)
+    return (int) hipErrorInvalidValue;
+  if (in->n_events == 0) return 0;
+  velo_masked_clustering_args a = *in;
+  const uint32_t block = block_of(a.tuning);
+  if (!block) return (int) hipErrorInvalidValue;
+  if (!a.tuning.lds_tile) a.tuning.lds_tile = kDefaultBankTile;
+  const size_t shmem = (size_t) (block / kWave) * a.tuning.lds_tile * sizeof(uint32_t);
+  hipLaunchKernelGGL(masked_clustering_kernel, dim3(a.n_events), dim3(block), shmem, (hipStream_t) stream, a);
+  return (int) hipGetLastError();
+}
+
+} // extern "C"

@@ -1,0 +1,200 @@
+// calculate_phi_and_sort (CalculatePhiAndSort.cu:6-52) and fill_candidates
+// (FillCandidates.cu:8-97) for gfx950.
+//
+// One workgroup per event, wave w owns modules w, w+W, ...  Phi sort: the
+// wave computes the module's phis into its LDS slice, ranks every hit
+// against all others (stable: ties broken by slot, CalculatePhi.cu:47-53),
+// then gathers x,y,z,id by the permutation into registers and, after a wave
+// barrier, writes them to their sorted positions.  Modules larger than the
+// register budget (512 hits; the reference caps a module at 500) fall back
+// to the reference's slot-by-slot permutation through global memory.
+#include "velo_device.h"
+
+namespace velo {
+
+constexpr uint32_t kSortCap = 512;
+constexpr uint32_t kSortChunks = kSortCap / kWave;
+constexpr uint32_t kSortBlock = 256;
+constexpr uint32_t kHoleBits = 0x7FC00000u; // quiet NaN
+
+__device__ inline float module_phi(uint32_t m, float x, float y) { return (m & 1u) ? phi_odd(x, y) : phi_even(x, y); }
+
+__global__ void __launch_bounds__(1024) calculate_phi_and_sort_kernel(velo_calculate_phi_and_sort_args a)
+{
+  extern __shared__ uint32_t lds[];
+  const uint32_t e = blockIdx.x;
+  const uint32_t waves = blockDim.x / kWave;
+  const uint32_t wave = threadIdx.x / kWave;
+  const uint32_t lane = lane_id();
+  const uint32_t total = a.dev_module_cluster_start[VELO_N_MODULES * a.n_events];
+  uint32_t* X = a.dev_velo_cluster_container;
+  uint32_t* Y = X + total;
+  uint32_t* Z = Y + total;
+  uint32_t* ID = Z + total;
+  uint32_t* PHI = ID + total;
+  uint32_t* TMP = PHI + total;
+  float* phi = reinterpret_cast<float*>(lds) + wave * kSortCap;
+  uint16_t* perm = reinterpret_cast<uint16_t*>(lds + waves * kSortCap) + wave * kSortCap;
+
+  for (uint32_t m = wave; m < VELO_N_MODULES; m += waves) {
+    const uint32_t s = a.dev_module_cluster_start[e * VELO_N_MODULES + m];
+    const uint32_t n = a.dev_module_cluster_num[e * VELO_N_MODULES + m];
+    // Slots estimated but not filled ("holes") read as NaN in every sorted
+    // slot, so no search can ever select them (the reference leaves them
+    // uninitialised).
+    const uint32_t hole_end = a.dev_module_cluster_start[e * VELO_N_MODULES + m + 1];
+    for (uint32_t p = s + n + lane; p < hole_end; p += kWave) {
+      X[p] = kHoleBits;
+      Y[p] = kHoleBits;
+      Z[p] = kHoleBits;
+      PHI[p] = kHoleBits;
+      TMP[p] = kHoleBits;
+    }
+    if (n == 0) continue;
+    if (n <= kSortCap) {
+      for (uint32_t i = lane; i < n; i += kWave) phi[i] = module_phi(m, __uint_as_float(X[s + i]), __uint_as_float(Y[s + i]));
+      wave_sync();
+      for (uint32_t i = lane; i < n; i += kWave) {
+        const float p = phi[i];
+        uint32_t pos = 0;
+        for (uint32_t j = 0; j < n; ++j) {
+          const float q = phi[j];
+          pos += (p > q) || (p == q && i > j);
+        }
+        perm[pos] = (uint16_t) i;
+        a.dev_hit_permutation[s + pos] = s + i;
+        PHI[s + pos] = __float_as_uint(p);
+      }
+      wave_sync();
+      uint32_t vx[kSortChunks], vy[kSortChunks], vz[kSortChunks], vid[kSortChunks];
+#pragma unroll
+      for (uint32_t c = 0; c < kSortChunks; ++c) {
+        const uint32_t p = c * kWave + lane;
+        if (p < n) {
+          const uint32_t i = s + perm[p];
+          vx[c] = X[i];
+          vy[c] = Y[i];
+          vz[c] = Z[i];
+          vid[c] = ID[i];
+        }
+      }
+      wave_sync(); // every read of the module is done before slots 0-2 are overwritten
+#pragma unroll
+      for (uint32_t c = 0; c < kSortChunks; ++c) {
+        const uint32_t p = c * kWave + lane;
+        if (p < n) {
+          TMP[s + p] = vx[c];
+          X[s + p] = vy[c];
+          Y[s + p] = vz[c];
+          Z[s + p] = vid[c];
+        }
+      }
+    }
+    else {
+      // phis staged in slot 5 (it receives X only at the end)
+      for (uint32_t i = lane; i < n; i += kWave)
+        TMP[s + i] = __float_as_uint(module_phi(m, __uint_as_float(X[s + i]), __uint_as_float(Y[s + i])));
+      wave_sync();
+      for (uint32_t i = lane; i < n; i += kWave) {
+        const float p = __uint_as_float(TMP[s + i]);
+        uint32_t pos = 0;
+        for (uint32_t j = 0; j < n; ++j) {
+          const float q = __uint_as_float(TMP[s + j]);
+          pos += (p > q) || (p == q && i > j);
+        }
+        a.dev_hit_permutation[s + pos] = s + i;
+        PHI[s + pos] = __float_as_uint(p);
+      }
+      wave_sync();
+      // SortByPhi.cu:21-33 order: X->5, Y->0, Z->1, ID->2
+      for (uint32_t p = lane; p < n; p += kWave) TMP[s + p] = X[a.dev_hit_permutation[s + p]];
+      wave_sync();
+      for (uint32_t p = lane; p < n; p += kWave) X[s + p] = Y[a.dev_hit_permutation[s + p]];
+      wave_sync();
+      for (uint32_t p = lane; p < n; p += kWave) Y[s + p] = Z[a.dev_hit_permutation[s + p]];
+      wave_sync();
+      for (uint32_t p = lane; p < n; p += kWave) Z[s + p] = ID[a.dev_hit_permutation[s + p]];
+    }
+    wave_sync();
+  }
+}
+
+// fill_candidates: wave w owns modules 2+w, 2+w+W, ... up to 49.
+__global__ void fill_candidates_kernel(velo_fill_candidates_args a)
+{
+  const uint32_t e = blockIdx.x;
+  const uint32_t waves = blockDim.x / kWave;
+  const uint32_t wave = threadIdx.x / kWave;
+  const uint32_t lane = lane_id();
+  const uint32_t total = a.dev_module_cluster_start[VELO_N_MODULES * a.n_events];
+  const uint32_t* starts = a.dev_module_cluster_start + e * VELO_N_MODULES;
+  const uint32_t* nums = a.dev_module_cluster_num + e * VELO_N_MODULES;
+  const uint32_t off = starts[0];
+  const float* phis = reinterpret_cast<const float*>(a.dev_velo_cluster_container + 4 * (size_t) total + off);
+  int16_t* h0c = a.dev_h0_candidates + 2 * (size_t) off;
+  int16_t* h2c = a.dev_h2_candidates + 2 * (size_t) off;
+  for (uint32_t m = 2 + wave; m < 50; m += waves) {
+    const uint32_t n1 = nums[m];
+    const uint32_t m0s = starts[m + 2] - off, m0n = nums[m + 2];
+    const uint32_t m2s = starts[m - 2] - off, m2n = nums[m - 2];
+    const uint32_t s1 = starts[m] - off;
+    for (uint32_t r = lane; r < n1; r += kWave) {
+      const uint32_t h1 = s1 + r;
+      const float phi = phis[h1];
+      int f0 = -1, l0 = -1, f2 = -1, l2 = -1;
+      if (m0n > 0) {
+        f0 = bs_first_candidate(phis + m0s, m0n, phi, kPhiExtrapolation);
+        if (f0 != -1) {
+          l0 = bs_second_candidate(phis + m0s + f0, m0n - (uint32_t) f0, phi, kPhiExtrapolation);
+          f0 += (int) m0s;
+          l0 = l0 == 0 ? f0 + 1 : f0 + l0;
+        }
+      }
+      if (m2n > 0) {
+        f2 = bs_first_candidate(phis + m2s, m2n, phi, kPhiExtrapolation);
+        if (f2 != -1) {
+          l2 = bs_second_candidate(phis + m2s + f2, m2n - (uint32_t) f2, phi, kPhiExtrapolation);
+          f2 += (int) m2s;
+          l2 = l2 == 0 ? f2 + 1 : f2 + l2;
+        }
+      }
+      h0c[2 * h1] = (int16_t) f0;
+      h0c[2 * h1 + 1] = (int16_t) l0;
+      h2c[2 * h1] = (int16_t) f2;
+      h2c[2 * h1 + 1] = (int16_t) l2;
+    }
+  }
+}
+
+} // namespace velo
+
+using namespace velo;
+
+extern "C" {
+
+int velo_calculate_phi_and_sort_launch(const velo_calculate_phi_and_sort_args* a, void* stream)
+{
+  if (!a || !a->dev_module_cluster_start || !a->dev_module_cluster_num || !a->dev_velo_cluster_container ||
+      !a->dev_hit_permutation)
+    return (int) hipErrorInvalidValue;
+  if (a->n_events == 0) return 0;
+  const uint32_t block = a->tuning.block_dim ? a->tuning.block_dim : kSortBlock;
+  if (block % kWave || block > 1024) return (int) hipErrorInvalidValue;
+  const size_t shmem = (size_t) (block / kWave) * kSortCap * (sizeof(float) + sizeof(uint16_t));
+  hipLaunchKernelGGL(calculate_phi_and_sort_kernel, dim3(a->n_events), dim3(block), shmem, (hipStream_t) stream, *a);
+  return (int) hipGetLastError();
+}
+
+int velo_fill_candidates_launch(const velo_fill_candidates_args* a, void* stream)
+{
+  if (!a || !a->dev_velo_cluster_container || !a->dev_module_cluster_start || !a->dev_module_cluster_num ||
+      !a->dev_h0_candidates || !a->dev_h2_candidates)
+    return (int) hipErrorInvalidValue;
+  if (a->n_events == 0) return 0;
+  const uint32_t block = a->tuning.block_dim ? a->tuning.block_dim : 256;
+  if (block % kWave || block > 1024) return (int) hipErrorInvalidValue;
+  hipLaunchKernelGGL(fill_candidates_kernel, dim3(a->n_events), dim3(block), 0, (hipStream_t) stream, *a);
+  return (int) hipGetLastError();
+}
+
+} // extern "C"

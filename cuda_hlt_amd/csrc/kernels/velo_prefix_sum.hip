@@ -1,0 +1,283 @@
+// Prefix sums between the VELO steps (PrefixSumHandler.cuh:15-62,
+// PrefixSumSingleBlock.cu:130-171) and the track consolidation
+// (ConsolidateVelo.cu:6-170) for gfx950.
+//
+// Exclusive scan of n u32 in three launches: per-tile sums (tile = 256
+// threads x 8 elements), one workgroup scanning the tile sums, then each
+// tile scanned in LDS and offset.  HBM traffic: 4 B read + 4 B read + 4 B
+// written per element.
+#include "velo_device.h"
+
+namespace velo {
+
+constexpr uint32_t kScanThreads = 256;
+constexpr uint32_t kScanItems = 8;
+constexpr uint32_t kScanTile = kScanThreads * kScanItems;
+
+// Exclusive scan of one value per thread over a 256-thread workgroup.
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* total, uint32_t* wave_sums)
+{
+  const uint32_t lane = lane_id();
+  const uint32_t wave = threadIdx.x / kWave;
+  const uint32_t waves = blockDim.x / kWave;
+  const uint32_t inc = wave_inclusive_scan(v);
+  if (lane == kWave - 1) wave_sums[wave] = inc;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+  for (uint32_t w = 0; w < waves; ++w) {
+    const uint32_t s = wave_sums[w];
+    if (w < wave) before += s;
+    all += s;
+  }
+  __syncthreads();
+  *total = all;
+  return before + inc - v;
+}
+
+__global__ void __launch_bounds__(kScanThreads) scan_reduce_kernel(const uint32_t* a, uint32_t* aux, uint32_t n)
+{
+  __shared__ uint32_t wave_sums[kScanThreads / kWave];
+  const uint32_t base = blockIdx.x * kScanTile;
+  uint32_t s = 0;
+  for (uint32_t k = 0; k < kScanItems; ++k) {
+    const uint32_t i = base + k * kScanThreads + threadIdx.x;
+    if (i < n) s += a[i];
+  }
+  s = wave_sum(s);
+  if (lane_id() == 0) wave_sums[threadIdx.x / kWave] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (uint32_t w = 0; w < kScanThreads / kWave; ++w) t += wave_sums[w];
+    aux[blockIdx.x] = t;
+  }
+}
+
+// Single workgroup: exclusive scan of m elements in place, total to *total_out
+// (when non-null).  Processes chunks of 1024 with a running carry.
+__global__ void __launch_bounds__(1024)
+  scan_single_block_kernel(uint32_t* total_out, const uint32_t* in, uint32_t* out, uint32_t m)
+{
+  __shared__ uint32_t wave_sums[1024 / kWave];
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < m; base += blockDim.x) {
+    const uint32_t i = base + threadIdx.x;
+    const uint32_t v = i < m ? in[i] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_exclusive_scan(v, &tot, wave_sums);
+    if (i < m) out[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0 && total_out) *total_out = carry;
+}
+
+__global__ void __launch_bounds__(kScanThreads)
+  scan_tiles_kernel(uint32_t* a, const uint32_t* aux, uint32_t n, uint32_t n_tiles)
+{
+  __shared__ uint32_t tile[kScanTile];
+  __shared__ uint32_t wave_sums[kScanThreads / kWave];
+  const uint32_t base = blockIdx.x * kScanTile;
+  for (uint32_t k = 0; k < kScanItems; ++k) {
+    const uint32_t j = k * kScanThreads + threadIdx.x;
+    tile[j] = base + j < n ? a[base + j] : 0u;
+  }
+  __syncthreads();
+  uint32_t v[kScanItems];
+  uint32_t s = 0;
+  for (uint32_t k = 0; k < kScanItems; ++k) {
+    v[k] = tile[threadIdx.x * kScanItems + k];
+    s += v[k];
+  }
+  uint32_t tot;
+  uint32_t run = block_exclusive_scan(s, &tot, wave_sums) + aux[blockIdx.x];
+  for (uint32_t k = 0; k < kScanItems; ++k) {
+    tile[threadIdx.x * kScanItems + k] = run;
+    run += v[k];
+  }
+  __syncthreads();
+  for (uint32_t k = 0; k < kScanItems; ++k) {
+    const uint32_t j = k * kScanThreads + threadIdx.x;
+    if (base + j < n) a[base + j] = tile[j];
+  }
+  if (blockIdx.x == n_tiles - 1 && threadIdx.x == 0) a[n] = aux[blockIdx.x] + tot;
+}
+
+__global__ void store_zero_kernel(uint32_t* p) { *p = 0u; }
+
+// K11: one workgroup per event (PrefixSumSingleBlock.cu:153-171).
+__global__ void copy_track_hit_number_kernel(velo_copy_velo_track_hit_number_args a)
+{
+  const uint32_t e = blockIdx.x;
+  const int32_t acc = a.dev_atomics_storage[a.n_events + e];
+  const int32_t n = a.dev_atomics_storage[e];
+  const uint8_t* tracks = static_cast<const uint8_t*>(a.dev_tracks) + (size_t) e * VELO_MAX_TRACKS * VELO_TRACK_HITS_BYTES;
+  for (int32_t i = threadIdx.x; i < n; i += blockDim.x)
+    a.dev_velo_track_hit_number[acc + i] = *reinterpret_cast<const uint16_t*>(tracks + (size_t) i * VELO_TRACK_HITS_BYTES);
+}
+
+// K15: one workgroup per event, one lane per track.
+__global__ void consolidate_kernel(velo_consolidate_velo_tracks_args a)
+{
+  const uint32_t N = a.n_events;
+  const uint32_t e = blockIdx.x;
+  const uint32_t total_tracks = (uint32_t) a.dev_atomics_velo[2 * N];
+  const uint32_t total_hits = a.dev_velo_track_hit_number[total_tracks];
+  const uint32_t n = (uint32_t) a.dev_atomics_velo[e];
+  const uint32_t toff = (uint32_t) a.dev_atomics_velo[N + e];
+  const uint32_t total = a.dev_module_cluster_start[VELO_N_MODULES * N];
+  const uint32_t off = a.dev_module_cluster_start[e * VELO_N_MODULES];
+  const float* Ys = reinterpret_cast<const float*>(a.dev_velo_cluster_container + off);
+  const float* Zs = reinterpret_cast<const float*>(a.dev_velo_cluster_container + total + off);
+  const uint32_t* IDs = a.dev_velo_cluster_container + 2 * (size_t) total + off;
+  const float* Xs = reinterpret_cast<const float*>(a.dev_velo_cluster_container + 5 * (size_t) total + off);
+  float* cx = reinterpret_cast<float*>(a.dev_velo_track_hits);
+  float* cy = cx + total_hits;
+  float* cz = cy + total_hits;
+  uint32_t* cid = reinterpret_cast<uint32_t*>(cz + total_hits);
+  float* S = reinterpret_cast<float*>(a.dev_velo_states);
+  uint8_t* backward = reinterpret_cast<uint8_t*>(a.dev_velo_states + 12 * sizeof(float) * (size_t) total_tracks);
+  const uint8_t* tracks = static_cast<const uint8_t*>(a.dev_tracks) + (size_t) e * VELO_MAX_TRACKS * VELO_TRACK_HITS_BYTES;
+
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const uint16_t* t = reinterpret_cast<const uint16_t*>(tracks + (size_t) i * VELO_TRACK_HITS_BYTES);
+    const uint32_t nh = t[0];
+    const uint32_t ho = a.dev_velo_track_hit_number[toff + i];
+    float s0 = 0.0f, sx = 0.0f, sz = 0.0f, sxz = 0.0f, sz2 = 0.0f;
+    float u0 = 0.0f, uy = 0.0f, uz = 0.0f, uyz = 0.0f, uz2 = 0.0f;
+    for (uint32_t h = 0; h < nh; ++h) {
+      const uint32_t hi = t[1 + h];
+      const float x = Xs[hi], y = Ys[hi], z = Zs[hi];
+      cx[ho + h] = x;
+      cy[ho + h] = y;
+      cz[ho + h] = z;
+      cid[ho + h] = IDs[hi];
+      const float wx = kParamW;
+      const float wx_t_x = wx * x;
+      const float wx_t_z = wx * z;
+      s0 += wx;
+      sx += wx_t_x;
+      sz += wx_t_z;
+      sxz += wx_t_x * z;
+      sz2 += wx_t_z * z;
+      const float wy = kParamW;
+      const float wy_t_y = wy * y;
+      const float wy_t_z = wy * z;
+      u0 += wy;
+      uy += wy_t_y;
+      uz += wy_t_z;
+      uyz += wy_t_y * z;
+      uz2 += wy_t_z * z;
+    }
+    const float dens = 1.0f / (sz2 * s0 - sz * sz);
+    const float tx = (sxz * s0 - sx * sz) * dens;
+    float x0 = (sx * sz2 - sxz * sz) * dens;
+    const float denu = 1.0f / (uz2 * u0 - uz * uz);
+    const float ty = (uyz * u0 - uy * uz) * denu;
+    float y0 = (uy * uz2 - uyz * uz) * denu;
+    const float zb = -(x0 * tx + y0 * ty) / (tx * tx + ty * ty);
+    const float z_first = Zs[t[1]];
+    const bool bwd = zb > z_first;
+    const float m00 = s0;
+    const float m11 = u0;
+    const float m20 = sz - zb * s0;
+    const float m31 = uz - zb * u0;
+    const float m22 = sz2 - 2.0f * zb * sz + zb * zb * s0;
+    const float m33 = uz2 - 2.0f * zb * uz + zb * zb * u0;
+    const float den20 = 1.0f / (m22 * m00 - m20 * m20);
+    const float den31 = 1.0f / (m33 * m11 - m31 * m31);
+    float ch = 0.0f;
+    int ndof = -4;
+    for (uint32_t h = 0; h < nh; ++h) {
+      const uint32_t hi = t[1 + h];
+      const float z = Zs[hi];
+      const float x = x0 + tx * z;
+      const float y = y0 + ty * z;
+      const float dx = x - Xs[hi];
+      const float dy = y - Ys[hi];
+      ch += dx * dx * kParamW + dy * dy * kParamW;
+      ndof += 2;
+    }
+    const float chi2 = ch / (float) ndof;
+    x0 = x0 + tx * zb;
+    y0 = y0 + ty * zb;
+    const size_t ti = toff + i;
+    const size_t T = total_tracks;
+    S[0 * T + ti] = x0;
+    S[1 * T + ti] = y0;
+    S[2 * T + ti] = tx;
+    S[3 * T + ti] = ty;
+    S[4 * T + ti] = m22 * den20;
+    S[5 * T + ti] = -m20 * den20;
+    S[6 * T + ti] = m00 * den20;
+    S[7 * T + ti] = m33 * den31;
+    S[8 * T + ti] = -m31 * den31;
+    S[9 * T + ti] = m11 * den31;
+    S[10 * T + ti] = chi2;
+    S[11 * T + ti] = zb;
+    backward[ti] = bwd ? 1 : 0;
+    if (a.dev_accepted_velo_tracks) a.dev_accepted_velo_tracks[ti] = 1;
+  }
+}
+
+} // namespace velo
+
+using namespace velo;
+
+extern "C" {
+
+uint32_t velo_prefix_sum_aux_size(uint32_t n)
+{
+  const uint32_t s = (n + kScanTile - 1) / kScanTile;
+  return s > 0 ? s : 1u;
+}
+
+int velo_prefix_sum_launch(const velo_prefix_sum_args* a, void* stream)
+{
+  if (!a || !a->dev_array || !a->dev_auxiliary_array) return (int) hipErrorInvalidValue;
+  hipStream_t s = (hipStream_t) stream;
+  const uint32_t n = a->array_size;
+  if (n == 0) {
+    hipLaunchKernelGGL(store_zero_kernel, dim3(1), dim3(1), 0, s, a->dev_array);
+    return (int) hipGetLastError();
+  }
+  const uint32_t tiles = velo_prefix_sum_aux_size(n);
+  hipLaunchKernelGGL(scan_reduce_kernel, dim3(tiles), dim3(kScanThreads), 0, s, a->dev_array, a->dev_auxiliary_array, n);
+  hipLaunchKernelGGL(scan_single_block_kernel, dim3(1), dim3(1024), 0, s, (uint32_t*) nullptr,
+                     a->dev_auxiliary_array, a->dev_auxiliary_array, tiles);
+  hipLaunchKernelGGL(scan_tiles_kernel, dim3(tiles), dim3(kScanThreads), 0, s, a->dev_array, a->dev_auxiliary_array,
+                     n, tiles);
+  return (int) hipGetLastError();
+}
+
+int velo_copy_and_prefix_sum_single_block_launch(const velo_copy_and_prefix_sum_single_block_args* a, void* stream)
+{
+  if (!a || !a->dev_total_sum || (a->array_size && (!a->dev_input_array || !a->dev_output_array)))
+    return (int) hipErrorInvalidValue;
+  hipLaunchKernelGGL(scan_single_block_kernel, dim3(1), dim3(1024), 0, (hipStream_t) stream, a->dev_total_sum,
+                     a->dev_input_array, a->dev_output_array, a->array_size);
+  return (int) hipGetLastError();
+}
+
+int velo_copy_velo_track_hit_number_launch(const velo_copy_velo_track_hit_number_args* a, void* stream)
+{
+  if (!a || !a->dev_tracks || !a->dev_atomics_storage || !a->dev_velo_track_hit_number)
+    return (int) hipErrorInvalidValue;
+  if (a->n_events == 0) return 0;
+  const uint32_t block = a->tuning.block_dim ? a->tuning.block_dim : 256;
+  hipLaunchKernelGGL(copy_track_hit_number_kernel, dim3(a->n_events), dim3(block), 0, (hipStream_t) stream, *a);
+  return (int) hipGetLastError();
+}
+
+int velo_consolidate_velo_tracks_launch(const velo_consolidate_velo_tracks_args* a, void* stream)
+{
+  if (!a || !a->dev_atomics_velo || !a->dev_tracks || !a->dev_velo_track_hit_number ||
+      !a->dev_velo_cluster_container || !a->dev_module_cluster_start || !a->dev_velo_track_hits ||
+      !a->dev_velo_states)
+    return (int) hipErrorInvalidValue;
+  if (a->n_events == 0) return 0;
+  const uint32_t block = a->tuning.block_dim ? a->tuning.block_dim : 256;
+  hipLaunchKernelGGL(consolidate_kernel, dim3(a->n_events), dim3(block), 0, (hipStream_t) stream, *a);
+  return (int) hipGetLastError();
+}
+
+} // extern "C"

@@ -1,0 +1,578 @@
+// search_by_triplet (SearchByTriplet.cu:7-111, ProcessModules.cu:8-163,
+// TrackSeeding.cu:7-200, TrackForwarding.cu:12-182) and weak_tracks_adder
+// (WeakTracksAdder.cu:8-157) for gfx950.
+//
+// One wave (64 lanes) per event.  The 24 module-pair steps run in sequence;
+// within a step every tracks-to-follow entry (forwarding) and every h1
+// candidate (seeding) gets a lane.  Appends (tracks, tracks-to-follow,
+// weak tracks) take their slots from a ballot prefix in entry order, so the
+// output is deterministic and equal to the CPU oracle's sequential order.
+//
+// State: hit_used as a bitmask in LDS (events up to 16384 estimated hits;
+// larger events use the global byte array directly); the tracks-to-follow
+// ring lives in dev_tracks_to_follow (2000 entries of 8 bytes per event):
+//   word0 = h[n-2] | h[n-1] << 16
+//   word1 = (tracklet ? first hit : track number) | n << 16 | skipped << 28
+//           | tracklet << 31
+// so forwarding never has to look a track up.
+#include "velo_device.h"
+
+namespace velo {
+
+constexpr uint32_t kUsedBits = 16384;
+constexpr uint32_t kUsedWords = kUsedBits / 32;
+constexpr uint32_t kTtfTrackletFlag = 0x80000000u;
+
+struct UsedSet {
+  uint32_t* bits;
+  uint8_t* bytes;
+  bool in_lds;
+  __device__ bool test(int h) const
+  {
+    return in_lds ? ((bits[(uint32_t) h >> 5] >> ((uint32_t) h & 31u)) & 1u) != 0u : bytes[h] != 0;
+  }
+  __device__ void set(uint32_t h) const
+  {
+    if (in_lds) atomicOr(&bits[h >> 5], 1u << (h & 31u));
+    else bytes[h] = 1;
+  }
+};
+
+struct ModuleData {
+  uint32_t start, num;
+  float z;
+};
+
+struct EventHits {
+  const float *x, *y, *z, *phi;
+};
+
+__device__ inline void load_modules(
+  ModuleData md[6], uint32_t first, const uint32_t* starts, const uint32_t* nums, uint32_t off, const float* zs)
+{
+#pragma unroll
+  for (uint32_t i = 0; i < 6; ++i) {
+    const uint32_t m = first - i;
+    md[i].start = starts[m] - off;
+    md[i].num = nums[m];
+    md[i].z = zs[m];
+  }
+}
+
+// Best (h0, h2) for one h1 (TrackSeeding.cu:266-328); h0 then h2 ascending,
+// so equal scatters resolve to the lexicographically smallest pair.
+__device__ inline float seed_best(
+  uint32_t h1, const EventHits& H, const int16_t* h0c, const int16_t* h2c, const UsedSet& used, uint32_t* b0,
+  uint32_t* b2)
+{
+  const float h1x = H.x[h1], h1y = H.y[h1], h1z = H.z[h1];
+  const int f0 = h0c[2 * h1], l0 = h0c[2 * h1 + 1];
+  const int f2 = h2c[2 * h1], l2 = h2c[2 * h1 + 1];
+  float best = kFltMax;
+  uint32_t bb0 = 0, bb2 = 0;
+  for (int h0 = f0; h0 < l0; ++h0) {
+    if (used.test(h0)) continue;
+    const float h0x = H.x[h0], h0y = H.y[h0], h0z = H.z[h0];
+    for (int h2 = f2; h2 < l2; ++h2) {
+      if (used.test(h2)) continue;
+      const float h2x = H.x[h2], h2y = H.y[h2], h2z = H.z[h2];
+      const float dmax = kMaxSlope * (h0z - h1z);
+      const float dz21 = h2z - h1z;
+      const float scatter_denom2 = 1.0f / (dz21 * dz21);
+      const float z2_tz = (h2z - h0z) / (h1z - h0z);
+      const float x = h0x + (h1x - h0x) * z2_tz;
+      const float y = h0y + (h1y - h0y) * z2_tz;
+      const float dx = x - h2x;
+      const float dy = y - h2y;
+      const float scatter_num = (dx * dx) + (dy * dy);
+      const float scatter = scatter_num * scatter_denom2;
+      const bool cond = fabsf(h1x - h0x) < dmax && fabsf(h1y - h0y) < dmax && fabsf(dx) < kTolerance &&
+                        fabsf(dy) < kTolerance && scatter < kMaxScatterSeeding && scatter < best;
+      if (cond) {
+        best = scatter;
+        bb0 = (uint32_t) h0;
+        bb2 = (uint32_t) h2;
+      }
+    }
+  }
+  *b0 = bb0;
+  *b2 = bb2;
+  return best;
+}
+
+// TrackForwarding.cuh:28-58
+__device__ inline void forward_window(
+  const ModuleData& md, float tx, float ty, const float* phis, float h0x, float h0y, float h0z, bool odd, int* f,
+  int* l)
+{
+  const float dz = md.z - h0z;
+  const float predx = tx * dz;
+  const float predy = ty * dz;
+  const float xp = h0x + predx;
+  const float yp = h0y + predy;
+  const float phi = odd ? phi_odd(xp, yp) : phi_even(xp, yp);
+  int first = bs_first_candidate(phis + md.start, md.num, phi, kForwardPhiTolerance);
+  int last = -1;
+  if (first != -1) {
+    last = bs_second_candidate(phis + md.start + first, md.num - (uint32_t) first, phi, kForwardPhiTolerance);
+    first += (int) md.start;
+    last = last == 0 ? first + 1 : first + last;
+  }
+  *f = first;
+  *l = last;
+}
+
+struct Ring {
+  uint32_t* base; // 2 words per entry
+  __device__ uint2 get(uint32_t pos) const
+  {
+    const uint32_t p = pos % VELO_TTF_MODULO;
+    return make_uint2(base[2 * p], base[2 * p + 1]);
+  }
+  __device__ void put(uint32_t pos, uint32_t w0, uint32_t w1) const
+  {
+    const uint32_t p = pos % VELO_TTF_MODULO;
+    base[2 * p] = w0;
+    base[2 * p + 1] = w1;
+  }
+};
+
+struct SearchState {
+  uint32_t n_tracks;
+  uint32_t n_weak;
+  uint32_t n_tracklets;
+  uint32_t status;
+  uint32_t next_begin; // ring position of the first entry of the next step
+  uint32_t next_n;     // entries already pushed for the next step
+  uint32_t cap;        // entries the next step may receive
+};
+
+// Appends the 3-hit weak track (h0, h1, h2) of lanes with `want`, in lane order.
+__device__ inline void append_weak(bool want, uint32_t h0, uint32_t h1, uint32_t h2, uint8_t* weak, SearchState& st)
+{
+  const uint64_t m = ballot(want);
+  if (!m) return;
+  const uint32_t slot = st.n_weak + count_below(m);
+  if (want && slot < VELO_MAX_WEAK_TRACKS) {
+    uint16_t* w = reinterpret_cast<uint16_t*>(weak + (size_t) slot * VELO_TRACKLET_BYTES);
+    w[0] = (uint16_t) h0;
+    w[1] = (uint16_t) h1;
+    w[2] = (uint16_t) h2;
+  }
+  const uint32_t cnt = (uint32_t) __popcll(m);
+  if (st.n_weak + cnt > VELO_MAX_WEAK_TRACKS) {
+    st.status |= VELO_STATUS_WEAK_OVERFLOW;
+    st.n_weak = VELO_MAX_WEAK_TRACKS;
+  }
+  else {
+    st.n_weak += cnt;
+  }
+}
+
+// Pushes (w0, w1) of lanes with `want` to the next step's entries, in lane order.
+__device__ inline void push_ttf(bool want, uint32_t w0, uint32_t w1, const Ring& ring, SearchState& st)
+{
+  const uint64_t m = ballot(want);
+  if (!m) return;
+  const uint32_t slot = st.next_n + count_below(m);
+  if (want && slot < st.cap) ring.put(st.next_begin + slot, w0, w1);
+  const uint32_t cnt = (uint32_t) __popcll(m);
+  if (st.next_n + cnt > st.cap) {
+    st.status |= VELO_STATUS_TTF_OVERFLOW;
+    st.next_n = st.cap;
+  }
+  else {
+    st.next_n += cnt;
+  }
+}
+
+// Seeds the first k (<= 64) staged h1 hits, one per lane.
+__device__ inline uint32_t seed_batch(
+  const uint16_t* stage, uint32_t k, const EventHits& H, const int16_t* h0c, const int16_t* h2c, const UsedSet& used,
+  const Ring& ring, SearchState& st)
+{
+  const uint32_t lane = lane_id();
+  bool emit = false;
+  uint32_t h1 = 0, b0 = 0, b2 = 0;
+  if (lane < k) {
+    h1 = stage[lane];
+    emit = seed_best(h1, H, h0c, h2c, used, &b0, &b2) < kFltMax;
+  }
+  const uint32_t cnt = (uint32_t) __popcll(ballot(emit));
+  st.n_tracklets += cnt;
+  push_ttf(emit, h1 | (b2 << 16), b0 | (3u << 16) | kTtfTrackletFlag, ring, st);
+  return cnt;
+}
+
+// TrackSeeding.cu:7-363: h1 candidates of module_data[2] then [3].
+__device__ inline uint32_t seeding(
+  const ModuleData* md, uint16_t* stage, const EventHits& H, const int16_t* h0c, const int16_t* h2c,
+  const UsedSet& used, const Ring& ring, SearchState& st)
+{
+  const uint32_t lane = lane_id();
+  uint32_t staged = 0, tracklets = 0;
+  for (int side = 2; side <= 3; ++side) {
+    for (uint32_t base = 0; base < md[side].num; base += kWave) {
+      const uint32_t r = base + lane;
+      const uint32_t h1 = md[side].start + r;
+      bool cand = false;
+      if (r < md[side].num) cand = !used.test((int) h1) && h0c[2 * h1] != -1 && h2c[2 * h1] != -1;
+      const uint64_t m = ballot(cand);
+      if (cand) stage[staged + count_below(m)] = (uint16_t) h1;
+      staged += (uint32_t) __popcll(m);
+      wave_sync();
+      if (staged >= kWave) {
+        tracklets += seed_batch(stage, kWave, H, h0c, h2c, used, ring, st);
+        wave_sync();
+        const uint32_t rest = staged - kWave;
+        uint16_t carry = 0;
+        if (lane < rest) carry = stage[kWave + lane];
+        wave_sync();
+        if (lane < rest) stage[lane] = carry;
+        staged = rest;
+        wave_sync();
+      }
+    }
+  }
+  if (staged) tracklets += seed_batch(stage, staged, H, h0c, h2c, used, ring, st);
+  wave_sync();
+  return tracklets;
+}
+
+// TrackForwarding.cu:41-182 over the entries [cur_begin, cur_begin + cur_n).
+__device__ inline void forwarding(
+  const ModuleData* md, uint32_t cur_begin, uint32_t cur_n, const EventHits& H, const UsedSet& used, const Ring& ring,
+  uint8_t* tracks, uint8_t* weak, SearchState& st)
+{
+  const uint32_t lane = lane_id();
+  for (uint32_t base = 0; base < cur_n; base += kWave) {
+    const uint32_t idx = base + lane;
+    const bool active = idx < cur_n;
+    uint2 ent = make_uint2(0, 0);
+    if (active) ent = ring.get(cur_begin + idx);
+    const bool tracklet = (ent.y & kTtfTrackletFlag) != 0u;
+    const uint32_t skipped = (ent.y >> 28) & 7u;
+    const uint32_t nh = (ent.y >> 16) & 0x1Fu;
+    const uint32_t ha = ent.x & 0xFFFFu; // hits[n-2]
+    const uint32_t hb = ent.x >> 16;     // hits[n-1]
+    const uint32_t low = ent.y & 0xFFFFu; // first hit or track number
+
+    bool found = false;
+    uint32_t best_h2 = 0;
+    if (active) {
+      const float h0x = H.x[ha], h0y = H.y[ha], h0z = H.z[ha];
+      const float h1x = H.x[hb], h1y = H.y[hb], h1z = H.z[hb];
+      const float td = 1.0f / (h1z - h0z);
+      const float txn = h1x - h0x;
+      const float tyn = h1y - h0y;
+      const float tx = txn * td;
+      const float ty = tyn * td;
+      int of, ol, ef, el;
+      forward_window(md[4], tx, ty, H.phi, h0x, h0y, h0z, true, &of, &ol);
+      forward_window(md[5], tx, ty, H.phi, h0x, h0y, h0z, false, &ef, &el);
+      const int n_odd = ol - of;
+      const int n_all = n_odd + (el - ef);
+      float best = kFltMax;
+      for (int j = 0; j < n_all; ++j) {
+        const int h2 = j < n_odd ? of + j : ef + j - n_odd;
+        const float h2x = H.x[h2], h2y = H.y[h2], h2z = H.z[h2];
+        const float dz = h2z - h0z;
+        const float predx = tx * dz;
+        const float predy = ty * dz;
+        const float dz21 = h2z - h1z;
+        const float scatter_denom2 = 1.0f / (dz21 * dz21);
+        const float xp = h0x + predx;
+        const float dx = fabsf(xp - h2x);
+        const float yp = h0y + predy;
+        const float dy = fabsf(yp - h2y);
+        const float scatter_num = (dx * dx) + (dy * dy);
+        const float scatter = scatter_num * scatter_denom2;
+        const bool cond = dx < kTolerance && dy < kTolerance && scatter < kMaxScatterForwarding;
+        const float fit = cond ? scatter : kFltMax;
+        if (fit < best) {
+          best = fit;
+          best_h2 = (uint32_t) h2;
+        }
+      }
+      found = best != kFltMax;
+    }
+
+    // hits become used when forwarded (TrackForwarding.cu:128-147)
+    if (found) {
+      used.set(best_h2);
+      if (tracklet) {
+        used.set(low);
+        used.set(ha);
+        used.set(hb);
+      }
+    }
+
+    // new tracks for promoted tracklets, numbered in entry order
+    const bool alloc = found && tracklet;
+    const uint64_t am = ballot(alloc);
+    uint32_t trackno = low;
+    bool track_ok = found;
+    if (am) {
+      const uint32_t slot = st.n_tracks + count_below(am);
+      if (alloc) {
+        trackno = slot;
+        track_ok = slot < VELO_MAX_TRACKS;
+      }
+      const uint32_t cnt = (uint32_t) __popcll(am);
+      if (st.n_tracks + cnt > VELO_MAX_TRACKS) {
+        st.status |= VELO_STATUS_TRACKS_OVERFLOW;
+        st.n_tracks = VELO_MAX_TRACKS;
+      }
+      else {
+        st.n_tracks += cnt;
+      }
+    }
+    if (found && track_ok) {
+      uint16_t* t = reinterpret_cast<uint16_t*>(tracks + (size_t) trackno * VELO_TRACK_HITS_BYTES);
+      if (tracklet) {
+        t[0] = 4;
+        t[1] = (uint16_t) low;
+        t[2] = (uint16_t) ha;
+        t[3] = (uint16_t) hb;
+        t[4] = (uint16_t) best_h2;
+      }
+      else {
+        t[1 + nh] = (uint16_t) best_h2;
+        t[0] = (uint16_t) (nh + 1);
+      }
+    }
+
+    // next step's entries: forwarded tracks and once-skipped tracks, in entry order
+    const bool skip = active && !found && skipped < kMaxSkippedModules;
+    const bool push = (found && track_ok) || skip;
+    uint32_t w0 = 0, w1 = 0;
+    if (found) {
+      w0 = hb | (best_h2 << 16);
+      w1 = trackno | ((nh + 1) << 16);
+    }
+    else if (skip) {
+      w0 = ent.x;
+      w1 = (ent.y & 0x8FFFFFFFu) | ((skipped + 1u) << 28);
+    }
+    push_ttf(push, w0, w1, ring, st);
+
+    // tracklets that cannot be followed any more become weak tracks
+    const bool to_weak = active && !found && !skip && nh == 3;
+    append_weak(to_weak, low, ha, hb, weak, st);
+  }
+}
+
+__global__ void __launch_bounds__(64) search_by_triplet_kernel(velo_search_by_triplet_args a)
+{
+  __shared__ uint32_t used_bits[kUsedWords];
+  __shared__ uint16_t stage[2 * kWave];
+  const uint32_t N = a.n_events;
+  const uint32_t e = blockIdx.x;
+  const uint32_t lane = lane_id();
+  const uint32_t total = a.dev_module_cluster_start[VELO_N_MODULES * N];
+  const uint32_t* starts = a.dev_module_cluster_start + e * VELO_N_MODULES;
+  const uint32_t* nums = a.dev_module_cluster_num + e * VELO_N_MODULES;
+  const uint32_t off = starts[0];
+  const uint32_t ev_hits = starts[VELO_N_MODULES] - off;
+  EventHits H;
+  H.y = reinterpret_cast<const float*>(a.dev_velo_cluster_container + off);
+  H.z = reinterpret_cast<const float*>(a.dev_velo_cluster_container + total + off);
+  H.phi = reinterpret_cast<const float*>(a.dev_velo_cluster_container + 4 * (size_t) total + off);
+  H.x = reinterpret_cast<const float*>(a.dev_velo_cluster_container + 5 * (size_t) total + off);
+  const int16_t* h0c = a.dev_h0_candidates + 2 * (size_t) off;
+  const int16_t* h2c = a.dev_h2_candidates + 2 * (size_t) off;
+  uint8_t* tracks = static_cast<uint8_t*>(a.dev_tracks) + (size_t) e * VELO_MAX_TRACKS * VELO_TRACK_HITS_BYTES;
+  uint8_t* weak = static_cast<uint8_t*>(a.dev_weak_tracks) + (size_t) e * VELO_MAX_WEAK_TRACKS * VELO_TRACKLET_BYTES;
+  Ring ring {a.dev_tracks_to_follow + (size_t) e * VELO_TTF_MODULO * VELO_TTF_ENTRY_WORDS};
+
+  UsedSet used {used_bits, a.dev_hit_used + off, ev_hits <= kUsedBits};
+  if (used.in_lds) {
+    for (uint32_t w = lane; w < kUsedWords; w += kWave) used_bits[w] = 0;
+  }
+  else {
+    for (uint32_t i = lane; i < ev_hits; i += kWave) used.bytes[i] = 0;
+  }
+  wave_sync();
+
+  SearchState st {0, 0, 0, 0, 0, 0, VELO_TTF_MODULO};
+  ModuleData md[6];
+  uint32_t first = VELO_N_MODULES - 1;
+  load_modules(md, first, starts, nums, off, a.dev_velo_module_zs);
+  uint32_t prev_tracklets = seeding(md, stage, H, h0c, h2c, used, ring, st);
+  uint32_t cur_begin = 0, cur_n = st.next_n;
+  first -= 2;
+  while (first >= 4) {
+    load_modules(md, first, starts, nums, off, a.dev_velo_module_zs);
+    st.next_begin = (cur_begin + cur_n) % VELO_TTF_MODULO;
+    st.next_n = 0;
+    st.cap = VELO_TTF_MODULO - cur_n;
+    forwarding(md, cur_begin, cur_n, H, used, ring, tracks, weak, st);
+    wave_sync(); // hit_used marks of forwarding are read by seeding
+    const uint32_t tl = seeding(md, stage, H, h0c, h2c, used, ring, st);
+    if (tl + prev_tracklets > VELO_TTF_MODULO) st.status |= VELO_STATUS_TRACKLETS_OVERFLOW;
+    prev_tracklets = tl;
+    cur_begin = st.next_begin;
+    cur_n = st.next_n;
+    first -= 2;
+    wave_sync();
+  }
+
+  // ProcessModules.cu:141-162: tracklets still followed become weak tracks
+  for (uint32_t base = 0; base < cur_n; base += kWave) {
+    const uint32_t idx = base + lane;
+    uint2 ent = make_uint2(0, 0);
+    if (idx < cur_n) ent = ring.get(cur_begin + idx);
+    const bool tl = idx < cur_n && (ent.y & kTtfTrackletFlag);
+    append_weak(tl, ent.y & 0xFFFFu, ent.x & 0xFFFFu, ent.x >> 16, weak, st);
+  }
+
+  if (lane == 0) {
+    a.dev_atomics_velo[e] = (int32_t) st.n_tracks;
+    const uint32_t ip = N + e * (VELO_NUM_ATOMICS - 1u);
+    a.dev_atomics_velo[ip + 0] = (int32_t) st.n_weak;
+    a.dev_atomics_velo[ip + 1] = (int32_t) st.n_tracklets;
+    a.dev_atomics_velo[ip + 2] = 0;
+    a.dev_atomics_velo[ip + 3] = 0;
+    if (a.dev_velo_event_status && st.status) atomicOr(a.dev_velo_event_status + e, st.status);
+  }
+  if (used.in_lds) {
+    wave_sync();
+    for (uint32_t i = lane; i < ev_hits; i += kWave) used.bytes[i] = (uint8_t) used.test((int) i);
+  }
+}
+
+// WeakTracksAdder.cu:8-86
+__device__ inline float weak_chi2(const EventHits& H, uint32_t a0, uint32_t a1, uint32_t a2)
+{
+  float s0 = 0.0f, sx = 0.0f, sz = 0.0f, sxz = 0.0f, sz2 = 0.0f;
+  float u0 = 0.0f, uy = 0.0f, uz = 0.0f, uyz = 0.0f, uz2 = 0.0f;
+  const uint32_t hs[3] = {a0, a1, a2};
+#pragma unroll
+  for (int h = 0; h < 3; ++h) {
+    const uint32_t i = hs[h];
+    const float x = H.x[i], y = H.y[i], z = H.z[i];
+    const float wx = kParamW;
+    const float wx_t_x = wx * x;
+    const float wx_t_z = wx * z;
+    s0 += wx;
+    sx += wx_t_x;
+    sz += wx_t_z;
+    sxz += wx_t_x * z;
+    sz2 += wx_t_z * z;
+    const float wy = kParamW;
+    const float wy_t_y = wy * y;
+    const float wy_t_z = wy * z;
+    u0 += wy;
+    uy += wy_t_y;
+    uz += wy_t_z;
+    uyz += wy_t_y * z;
+    uz2 += wy_t_z * z;
+  }
+  const float dens = 1.0f / (sz2 * s0 - sz * sz);
+  const float tx = (sxz * s0 - sx * sz) * dens;
+  const float x0 = (sx * sz2 - sxz * sz) * dens;
+  const float denu = 1.0f / (uz2 * u0 - uz * uz);
+  const float ty = (uyz * u0 - uy * uz) * denu;
+  const float y0 = (uy * uz2 - uyz * uz) * denu;
+  float ch = 0.0f;
+  int ndof = -4;
+#pragma unroll
+  for (int h = 0; h < 3; ++h) {
+    const uint32_t i = hs[h];
+    const float z = H.z[i];
+    const float x = x0 + tx * z;
+    const float y = y0 + ty * z;
+    const float dx = x - H.x[i];
+    const float dy = y - H.y[i];
+    ch += dx * dx * kParamW + dy * dy * kParamW;
+    ndof += 2;
+  }
+  return ch / (float) ndof;
+}
+
+__global__ void __launch_bounds__(64) weak_tracks_adder_kernel(velo_weak_tracks_adder_args a)
+{
+  const uint32_t N = a.n_events;
+  const uint32_t e = blockIdx.x;
+  const uint32_t lane = lane_id();
+  const uint32_t total = a.dev_module_cluster_start[VELO_N_MODULES * N];
+  const uint32_t off = a.dev_module_cluster_start[e * VELO_N_MODULES];
+  EventHits H;
+  H.y = reinterpret_cast<const float*>(a.dev_velo_cluster_container + off);
+  H.z = reinterpret_cast<const float*>(a.dev_velo_cluster_container + total + off);
+  H.phi = nullptr;
+  H.x = reinterpret_cast<const float*>(a.dev_velo_cluster_container + 5 * (size_t) total + off);
+  const uint8_t* used = a.dev_hit_used + off;
+  uint8_t* tracks = static_cast<uint8_t*>(a.dev_tracks) + (size_t) e * VELO_MAX_TRACKS * VELO_TRACK_HITS_BYTES;
+  const uint8_t* weak =
+    static_cast<const uint8_t*>(a.dev_weak_tracks) + (size_t) e * VELO_MAX_WEAK_TRACKS * VELO_TRACKLET_BYTES;
+  const uint32_t ip = N + e * (VELO_NUM_ATOMICS - 1u);
+  uint32_t n_weak = (uint32_t) a.dev_atomics_velo[ip];
+  if (n_weak > VELO_MAX_WEAK_TRACKS) n_weak = VELO_MAX_WEAK_TRACKS;
+  uint32_t n_tracks = (uint32_t) a.dev_atomics_velo[e];
+  uint32_t status = 0;
+  for (uint32_t base = 0; base < n_weak; base += kWave) {
+    const uint32_t w = base + lane;
+    bool accept = false;
+    uint32_t t0 = 0, t1 = 0, t2 = 0;
+    if (w < n_weak) {
+      const uint16_t* t = reinterpret_cast<const uint16_t*>(weak + (size_t) w * VELO_TRACKLET_BYTES);
+      t0 = t[0];
+      t1 = t[1];
+      t2 = t[2];
+      const bool any_used = used[t0] || used[t1] || used[t2];
+      const float chi2 = weak_chi2(H, t0, t1, t2);
+      accept = !any_used && chi2 < kMaxChi2;
+    }
+    const uint64_t m = ballot(accept);
+    const uint32_t slot = n_tracks + count_below(m);
+    if (accept && slot < VELO_MAX_TRACKS) {
+      uint16_t* d = reinterpret_cast<uint16_t*>(tracks + (size_t) slot * VELO_TRACK_HITS_BYTES);
+      d[0] = 3;
+      d[1] = (uint16_t) t0;
+      d[2] = (uint16_t) t1;
+      d[3] = (uint16_t) t2;
+    }
+    const uint32_t cnt = (uint32_t) __popcll(m);
+    if (n_tracks + cnt > VELO_MAX_TRACKS) {
+      status |= VELO_STATUS_TRACKS_OVERFLOW;
+      n_tracks = VELO_MAX_TRACKS;
+    }
+    else {
+      n_tracks += cnt;
+    }
+  }
+  if (lane == 0) {
+    a.dev_atomics_velo[e] = (int32_t) n_tracks;
+    if (a.dev_velo_event_status && status) atomicOr(a.dev_velo_event_status + e, status);
+  }
+}
+
+} // namespace velo
+
+using namespace velo;
+
+extern "C" {
+
+int velo_search_by_triplet_launch(const velo_search_by_triplet_args* a, void* stream)
+{
+  if (!a || !a->dev_velo_cluster_container || !a->dev_module_cluster_start || !a->dev_module_cluster_num ||
+      !a->dev_tracks || !a->dev_tracks_to_follow || !a->dev_weak_tracks || !a->dev_hit_used ||
+      !a->dev_atomics_velo || !a->dev_h0_candidates || !a->dev_h2_candidates || !a->dev_velo_module_zs)
+    return (int) hipErrorInvalidValue;
+  if (a->n_events == 0) return 0;
+  hipLaunchKernelGGL(search_by_triplet_kernel, dim3(a->n_events), dim3(kWave), 0, (hipStream_t) stream, *a);
+  return (int) hipGetLastError();
+}
+
+int velo_weak_tracks_adder_launch(const velo_weak_tracks_adder_args* a, void* stream)
+{
+  if (!a || !a->dev_velo_cluster_container || !a->dev_module_cluster_start || !a->dev_tracks ||
+      !a->dev_weak_tracks || !a->dev_hit_used || !a->dev_atomics_velo)
+    return (int) hipErrorInvalidValue;
+  if (a->n_events == 0) return 0;
+  hipLaunchKernelGGL(weak_tracks_adder_kernel, dim3(a->n_events), dim3(kWave), 0, (hipStream_t) stream, *a);
+  return (int) hipGetLastError();
+}
+
+} // extern "C"

@@ -1,0 +1,181 @@
+"""ctypes binding of the C ABI in include/velo_hlt.h (libvelo_hlt.so).
+
+The structures below mirror the header field for field.  Device pointers
+are passed as integers (e.g. torch.Tensor.data_ptr()), streams as the raw
+hipStream_t value (torch.cuda.Stream.cuda_stream) or None for the default
+stream.  Every launcher call is checked; a non-zero hipError_t raises
+VeloError.  There is no CPU fallback: if the library is missing, loading
+raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libvelo_hlt.so"
+
+N_MODULES = 52
+N_SENSORS = 208
+MAX_CANDIDATES_EVENT = 3000
+MAX_NUMHITS_IN_MODULE = 500
+MAX_HITS_PER_EVENT = 9500
+MAX_TRACKS = 1200
+MAX_TRACK_SIZE = 26
+TTF_MODULO = 2000
+MAX_WEAK_TRACKS = 500
+NUM_ATOMICS = 5
+GEOMETRY_BYTES = 19456
+TRACK_HITS_BYTES = 54
+TRACKLET_BYTES = 6
+STATE_BYTES = 49
+HIT_BYTES = 16
+TTF_ENTRY_WORDS = 2
+
+STATUS = {
+    "SP_INDEX_TRUNCATED": 0x001, "CANDIDATES_OVERFLOW": 0x002, "MODULE_HITS_OVERFLOW": 0x004,
+    "EVENT_HITS_OVERFLOW": 0x008, "TTF_OVERFLOW": 0x010, "TRACKLETS_OVERFLOW": 0x020,
+    "WEAK_OVERFLOW": 0x040, "TRACKS_OVERFLOW": 0x080, "BAD_BANK": 0x100,
+}
+
+P = C.c_void_p
+U32 = C.c_uint32
+
+
+class VeloError(RuntimeError):
+    pass
+
+
+class Tuning(C.Structure):
+    _fields_ = [("block_dim", U32), ("lds_tile", U32)]
+
+
+class Constants(C.Structure):
+    _fields_ = [("dev_velo_sp_patterns", P), ("dev_velo_sp_fx", P), ("dev_velo_sp_fy", P),
+                ("dev_velo_candidate_ks", P), ("dev_velo_module_zs", P), ("dev_velo_geometry", P)]
+
+
+def _args(name, fields):
+    return type(name, (C.Structure,), {"_fields_": [(f, P) for f in fields] + [("n_events", U32), ("tuning", Tuning)]})
+
+
+InitEventListArgs = type("InitEventListArgs", (C.Structure,),
+                         {"_fields_": [("dev_event_list", P), ("n_events", U32), ("tuning", Tuning)]})
+EstimateInputSizeArgs = _args("EstimateInputSizeArgs", [
+    "dev_raw_input", "dev_raw_input_offsets", "dev_estimated_input_size", "dev_module_cluster_num",
+    "dev_event_candidate_num", "dev_cluster_candidates", "dev_event_list", "dev_event_order",
+    "dev_velo_candidate_ks", "dev_velo_event_status"])
+PrefixSumArgs = type("PrefixSumArgs", (C.Structure,), {"_fields_": [
+    ("dev_array", P), ("dev_auxiliary_array", P), ("array_size", U32), ("tuning", Tuning)]})
+MaskedClusteringArgs = _args("MaskedClusteringArgs", [
+    "dev_raw_input", "dev_raw_input_offsets", "dev_module_cluster_start", "dev_module_cluster_num",
+    "dev_event_candidate_num", "dev_cluster_candidates", "dev_velo_cluster_container", "dev_event_list",
+    "dev_event_order", "dev_velo_geometry", "dev_velo_sp_patterns", "dev_velo_sp_fx", "dev_velo_sp_fy",
+    "dev_velo_event_status"])
+CalculatePhiAndSortArgs = _args("CalculatePhiAndSortArgs", [
+    "dev_module_cluster_start", "dev_module_cluster_num", "dev_velo_cluster_container", "dev_hit_permutation"])
+FillCandidatesArgs = _args("FillCandidatesArgs", [
+    "dev_velo_cluster_container", "dev_module_cluster_start", "dev_module_cluster_num", "dev_h0_candidates",
+    "dev_h2_candidates"])
+SearchByTripletArgs = _args("SearchByTripletArgs", [
+    "dev_velo_cluster_container", "dev_module_cluster_start", "dev_module_cluster_num", "dev_tracks",
+    "dev_tracklets", "dev_tracks_to_follow", "dev_weak_tracks", "dev_hit_used", "dev_atomics_velo",
+    "dev_h0_candidates", "dev_h2_candidates", "dev_rel_indices", "dev_velo_module_zs", "dev_velo_event_status"])
+WeakTracksAdderArgs = _args("WeakTracksAdderArgs", [
+    "dev_velo_cluster_container", "dev_module_cluster_start", "dev_tracks", "dev_weak_tracks", "dev_hit_used",
+    "dev_atomics_velo", "dev_velo_event_status"])
+CopyAndPrefixSumArgs = type("CopyAndPrefixSumArgs", (C.Structure,), {"_fields_": [
+    ("dev_total_sum", P), ("dev_input_array", P), ("dev_output_array", P), ("array_size", U32),
+    ("tuning", Tuning)]})
+CopyTrackHitNumberArgs = _args("CopyTrackHitNumberArgs", [
+    "dev_tracks", "dev_atomics_storage", "dev_velo_track_hit_number"])
+ConsolidateArgs = _args("ConsolidateArgs", [
+    "dev_atomics_velo", "dev_tracks", "dev_velo_track_hit_number", "dev_velo_cluster_container",
+    "dev_module_cluster_start", "dev_module_cluster_num", "dev_velo_track_hits", "dev_velo_states",
+    "dev_accepted_velo_tracks"])
+
+LAUNCHERS = {
+    "velo_init_event_list_launch": InitEventListArgs,
+    "velo_estimate_input_size_launch": EstimateInputSizeArgs,
+    "velo_prefix_sum_launch": PrefixSumArgs,
+    "velo_masked_clustering_launch": MaskedClusteringArgs,
+    "velo_calculate_phi_and_sort_launch": CalculatePhiAndSortArgs,
+    "velo_fill_candidates_launch": FillCandidatesArgs,
+    "velo_search_by_triplet_launch": SearchByTripletArgs,
+    "velo_weak_tracks_adder_launch": WeakTracksAdderArgs,
+    "velo_copy_and_prefix_sum_single_block_launch": CopyAndPrefixSumArgs,
+    "velo_copy_velo_track_hit_number_launch": CopyTrackHitNumberArgs,
+    "velo_consolidate_velo_tracks_launch": ConsolidateArgs,
+}
+
+_lib = None
+
+
+def lib():
+    """Load libvelo_hlt.so (raises when it is not built: no fallback)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise VeloError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(str(LIB_PATH))
+        for name, st in LAUNCHERS.items():
+            f = getattr(L, name)
+            f.argtypes = [C.POINTER(st), P]
+            f.restype = C.c_int
+        L.velo_prefix_sum_aux_size.argtypes = [U32]
+        L.velo_prefix_sum_aux_size.restype = U32
+        L.velo_constants_init.argtypes = [C.c_char_p, C.c_size_t, P, C.POINTER(Constants)]
+        L.velo_constants_init.restype = C.c_int
+        L.velo_constants_free.argtypes = [C.POINTER(Constants)]
+        L.velo_constants_free.restype = C.c_int
+        L.velo_sp_tables.argtypes = [P, P, P, P]
+        L.velo_hlt_version.restype = C.c_char_p
+        L.velo_error_string.argtypes = [C.c_int]
+        L.velo_error_string.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def check(code: int, what: str) -> None:
+    if code != 0:
+        msg = lib().velo_error_string(code).decode()
+        raise VeloError(f"{what} failed: hipError {code} ({msg})")
+
+
+def launch(name: str, args, stream=None) -> None:
+    """Call one C-ABI launcher with a filled argument structure."""
+    check(getattr(lib(), name)(C.byref(args), stream), name)
+
+
+def prefix_sum_aux_size(n: int) -> int:
+    return int(lib().velo_prefix_sum_aux_size(n))
+
+
+def sp_tables():
+    import numpy as np
+    p = np.zeros(256, np.uint8); s = np.zeros(256, np.uint8)
+    fx = np.zeros(512, np.float32); fy = np.zeros(512, np.float32)
+    lib().velo_sp_tables(p.ctypes.data, s.ctypes.data, fx.ctypes.data, fy.ctypes.data)
+    return p, s, fx, fy
+
+
+class DeviceConstants:
+    """Device tables (SP patterns, candidate ks, module z, geometry)."""
+
+    def __init__(self, geometry: bytes, module_zs=None):
+        import numpy as np
+        self.c = Constants()
+        zs = None
+        if module_zs is not None:
+            self._zs = np.ascontiguousarray(module_zs, np.float32)
+            zs = self._zs.ctypes.data
+        check(lib().velo_constants_init(geometry, len(geometry), zs, C.byref(self.c)), "velo_constants_init")
+
+    def free(self):
+        if self.c.dev_velo_geometry:
+            lib().velo_constants_free(C.byref(self.c))
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -1,0 +1,230 @@
+"""ctypes binding of the CPU oracle (oracle/velo_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, the smoke check and the bench's
+CPU-baseline leg as the checker.  The product never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "_build" / "libvelo_oracle.so"
+REF_SP_PATH = HERE / "_ref" / "libref_sp.so"
+
+N_MODULES = 52
+MAX_CANDIDATES_EVENT = 3000
+MAX_TRACKS = 1200
+MAX_WEAK_TRACKS = 500
+NUM_ATOMICS = 5
+
+TRACK_DTYPE = np.dtype([("hitsNum", "<u2"), ("hits", "<u2", (26,))])  # 54 B, VeloEventModel.cuh:58-69
+TRACKLET_DTYPE = np.dtype([("hits", "<u2", (3,))])                   # 6 B
+
+STATUS_BITS = {
+    "SP_INDEX_TRUNCATED": 0x001, "CANDIDATES_OVERFLOW": 0x002, "MODULE_HITS_OVERFLOW": 0x004,
+    "EVENT_HITS_OVERFLOW": 0x008, "TTF_OVERFLOW": 0x010, "TRACKLETS_OVERFLOW": 0x020,
+    "WEAK_OVERFLOW": 0x040, "TRACKS_OVERFLOW": 0x080, "BAD_BANK": 0x100,
+}
+
+_lib = None
+_P = C.c_void_p
+
+
+def _ptr(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(str(LIB_PATH))
+        L.oracle_atan2f.argtypes = [C.c_float, C.c_float]
+        L.oracle_atan2f.restype = C.c_float
+        L.oracle_phi_odd.argtypes = [C.c_float, C.c_float]
+        L.oracle_phi_odd.restype = C.c_float
+        L.oracle_phi_even.argtypes = [C.c_float, C.c_float]
+        L.oracle_phi_even.restype = C.c_float
+        L.oracle_sp_lut.argtypes = [_P] * 4
+        L.oracle_estimate_input_size.argtypes = [_P, _P, C.c_uint32, _P, _P, _P, _P, _P]
+        L.oracle_prefix_sum.argtypes = [_P, C.c_uint32]
+        L.oracle_masked_clustering.argtypes = [_P, _P, C.c_uint32, _P, _P, _P, _P, _P]
+        L.oracle_calculate_phi_and_sort.argtypes = [C.c_uint32, _P, _P, _P, _P, _P]
+        L.oracle_fill_candidates.argtypes = [C.c_uint32, _P, _P, _P, _P, _P]
+        L.oracle_search_by_triplet.argtypes = [C.c_uint32] + [_P] * 11
+        L.oracle_weak_tracks_adder.argtypes = [C.c_uint32] + [_P] * 7
+        L.oracle_copy_and_prefix_sum.argtypes = [C.c_uint32, _P]
+        L.oracle_copy_velo_track_hit_number.argtypes = [C.c_uint32, _P, _P, _P]
+        L.oracle_consolidate_velo_tracks.argtypes = [C.c_uint32] + [_P] * 7
+        L.oracle_ccl_event.argtypes = [_P, _P, _P, _P, _P, _P, C.c_uint32]
+        L.oracle_ccl_event.restype = C.c_uint32
+        _lib = L
+    return _lib
+
+
+def ref_sp_lib():
+    """The reference's own cache_sp_patterns, or None when not built."""
+    if not REF_SP_PATH.exists():
+        return None
+    L = C.CDLL(str(REF_SP_PATH))
+    L.ref_cache_sp_patterns.argtypes = [_P] * 4
+    return L
+
+
+# --------------------------------------------------------------------------
+# stage wrappers (numpy in, numpy out; buffers use the reference layouts)
+# --------------------------------------------------------------------------
+
+def sp_lut():
+    p = np.zeros(256, np.uint8); s = np.zeros(256, np.uint8)
+    fx = np.zeros(512, np.float32); fy = np.zeros(512, np.float32)
+    lib().oracle_sp_lut(_ptr(p), _ptr(s), _ptr(fx), _ptr(fy))
+    return p, s, fx, fy
+
+
+def ref_sp_lut():
+    L = ref_sp_lib()
+    if L is None:
+        return None
+    p = np.zeros(256, np.uint8); s = np.zeros(256, np.uint8)
+    fx = np.zeros(512, np.float32); fy = np.zeros(512, np.float32)
+    L.ref_cache_sp_patterns(_ptr(p), _ptr(s), _ptr(fx), _ptr(fy))
+    return p, s, fx, fy
+
+
+def estimate_input_size(events: np.ndarray, offsets: np.ndarray):
+    n = len(offsets) - 1
+    est = np.zeros(n * N_MODULES + 1, np.uint32)
+    num = np.zeros(n * N_MODULES, np.uint32)
+    cnum = np.zeros(n, np.uint32)
+    cand = np.zeros(n * MAX_CANDIDATES_EVENT, np.uint32)
+    status = np.zeros(n, np.uint32)
+    lib().oracle_estimate_input_size(_ptr(events), _ptr(offsets), n, _ptr(est), _ptr(num), _ptr(cnum),
+                                     _ptr(cand), _ptr(status))
+    return est, num, cnum, cand, status
+
+
+def prefix_sum(a: np.ndarray) -> np.ndarray:
+    """Exclusive scan of a[:-1] in place semantics; returns a copy with total at [-1]."""
+    out = np.ascontiguousarray(a, dtype=np.uint32).copy()
+    lib().oracle_prefix_sum(_ptr(out), len(out) - 1)
+    return out
+
+
+def masked_clustering(events, offsets, starts, geometry: bytes, status=None):
+    n = len(offsets) - 1
+    total = int(starts[n * N_MODULES])
+    num = np.zeros(n * N_MODULES, np.uint32)
+    cont = np.zeros(6 * total + 1, np.uint32)
+    st = np.zeros(n, np.uint32) if status is None else status
+    g = np.frombuffer(geometry, np.uint8).copy()
+    lib().oracle_masked_clustering(_ptr(events), _ptr(offsets), n, _ptr(starts), _ptr(num), _ptr(g),
+                                   _ptr(cont), _ptr(st))
+    return num, cont, st
+
+
+def calculate_phi_and_sort(n, starts, nums, container):
+    cont = container.copy()
+    perm = np.zeros(max(1, int(starts[n * N_MODULES])), np.uint32)
+    st = np.zeros(n, np.uint32)
+    lib().oracle_calculate_phi_and_sort(n, _ptr(starts), _ptr(nums), _ptr(cont), _ptr(perm), _ptr(st))
+    return cont, perm
+
+
+def fill_candidates(n, container, starts, nums):
+    total = int(starts[n * N_MODULES])
+    h0 = np.full(2 * total + 2, -7, np.int16)
+    h2 = np.full(2 * total + 2, -7, np.int16)
+    lib().oracle_fill_candidates(n, _ptr(container), _ptr(starts), _ptr(nums), _ptr(h0), _ptr(h2))
+    return h0, h2
+
+
+def search_by_triplet(n, container, starts, nums, h0, h2, module_zs):
+    total = int(starts[n * N_MODULES])
+    tracks = np.zeros(n * MAX_TRACKS, TRACK_DTYPE)
+    weak = np.zeros(n * MAX_WEAK_TRACKS, TRACKLET_DTYPE)
+    used = np.zeros(total + 1, np.uint8)
+    atomics = np.zeros(n * NUM_ATOMICS + 1, np.int32)
+    st = np.zeros(n, np.uint32)
+    zs = np.ascontiguousarray(module_zs, np.float32)
+    lib().oracle_search_by_triplet(n, _ptr(container), _ptr(starts), _ptr(nums), _ptr(tracks), _ptr(weak),
+                                   _ptr(used), _ptr(atomics), _ptr(h0), _ptr(h2), _ptr(zs), _ptr(st))
+    return tracks, weak, used, atomics, st
+
+
+def weak_tracks_adder(n, container, starts, tracks, weak, used, atomics):
+    tracks = tracks.copy(); atomics = atomics.copy()
+    st = np.zeros(n, np.uint32)
+    lib().oracle_weak_tracks_adder(n, _ptr(container), _ptr(starts), _ptr(tracks), _ptr(weak), _ptr(used),
+                                   _ptr(atomics), _ptr(st))
+    return tracks, atomics, st
+
+
+def copy_and_prefix_sum(n, atomics):
+    a = atomics.astype(np.uint32).copy()
+    lib().oracle_copy_and_prefix_sum(n, _ptr(a))
+    return a
+
+
+def copy_velo_track_hit_number(n, tracks, atomics):
+    total = int(atomics[2 * n])
+    hn = np.zeros(total + 1, np.uint32)
+    lib().oracle_copy_velo_track_hit_number(n, _ptr(tracks), _ptr(atomics), _ptr(hn))
+    return hn
+
+
+def consolidate(n, atomics, tracks, hit_number, container, starts):
+    total_tracks = int(atomics[2 * n])
+    total_hits = int(hit_number[total_tracks])
+    hits = np.zeros(16 * total_hits + 16, np.uint8)
+    states = np.zeros(49 * total_tracks + 49, np.uint8)
+    lib().oracle_consolidate_velo_tracks(n, _ptr(atomics), _ptr(tracks), _ptr(hit_number), _ptr(container),
+                                         _ptr(starts), _ptr(hits), _ptr(states))
+    return hits, states
+
+
+def ccl_event(event: np.ndarray, capacity: int = 200000):
+    ids = np.zeros(capacity, np.uint32); mods = np.zeros(capacity, np.uint32)
+    sx = np.zeros(capacity, np.int32); sy = np.zeros(capacity, np.int32); npx = np.zeros(capacity, np.int32)
+    k = lib().oracle_ccl_event(_ptr(event), _ptr(ids), _ptr(mods), _ptr(sx), _ptr(sy), _ptr(npx), capacity)
+    k = min(k, capacity)
+    return ids[:k], mods[:k], sx[:k], sy[:k], npx[:k]
+
+
+# --------------------------------------------------------------------------
+# whole pipeline
+# --------------------------------------------------------------------------
+
+@dataclass
+class PipelineResult:
+    n_events: int
+    starts: np.ndarray           # N*52+1 module cluster starts (scan of estimates)
+    nums: np.ndarray             # N*52 clusters per module
+    container: np.ndarray        # 6*total sorted cluster container
+    status: np.ndarray
+    tracks: np.ndarray           # N*1200 TrackHits (after the weak adder)
+    atomics: np.ndarray          # after copy_and_prefix_sum
+    hit_number: np.ndarray       # T+1 exclusive offsets
+    track_hits: np.ndarray       # SoA x,y,z,id bytes
+    states: np.ndarray           # SoA state bytes
+
+
+def pipeline(events: np.ndarray, offsets: np.ndarray, geometry: bytes, module_zs: np.ndarray) -> PipelineResult:
+    n = len(offsets) - 1
+    est, _, _, _, st0 = estimate_input_size(events, offsets)
+    starts = prefix_sum(est)
+    nums, cont, st1 = masked_clustering(events, offsets, starts, geometry)
+    cont, _ = calculate_phi_and_sort(n, starts, nums, cont)
+    h0, h2 = fill_candidates(n, cont, starts, nums)
+    tracks, weak, used, atomics, st2 = search_by_triplet(n, cont, starts, nums, h0, h2, module_zs)
+    tracks, atomics, st3 = weak_tracks_adder(n, cont, starts, tracks, weak, used, atomics)
+    atomics = copy_and_prefix_sum(n, atomics)
+    hn = copy_velo_track_hit_number(n, tracks, atomics)
+    hn = prefix_sum(hn)
+    hits, states = consolidate(n, atomics, tracks, hn, cont, starts)
+    return PipelineResult(n, starts, nums, cont, st0 | st1 | st2 | st3, tracks, atomics, hn, hits, states)

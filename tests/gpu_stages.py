@@ -1,0 +1,161 @@
+"""Runs single launchers of libvelo_hlt.so on numpy inputs (GPU tests only).
+
+Each function uploads its inputs, calls exactly one C-ABI launcher (through
+cuda_hlt_amd.native) and downloads the outputs, so every stage can be fed
+the oracle's output of the previous stage and compared on its own.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from cuda_hlt_amd import native as nat
+
+N_MODULES = nat.N_MODULES
+
+
+def dev(a: np.ndarray) -> torch.Tensor:
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t: torch.Tensor) -> np.ndarray:
+    return t.cpu().numpy()
+
+
+def u32(n):
+    return torch.zeros(max(int(n), 1), dtype=torch.int32, device="cuda")
+
+
+def garbage_u32(n, seed=1):
+    """Buffers the kernels must fully define are pre-filled with noise."""
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    return torch.randint(-2**31, 2**31 - 1, (max(int(n), 1),), generator=g, dtype=torch.int32).cuda()
+
+
+def sync():
+    torch.cuda.synchronize()
+
+
+def estimate(events, offsets, consts: nat.DeviceConstants, tuning=(0, 0)):
+    n = len(offsets) - 1
+    raw, off = dev(events), dev(offsets.view(np.int32))
+    est, num, cnum = garbage_u32(n * N_MODULES + 1), garbage_u32(n * N_MODULES), garbage_u32(n)
+    cand, status = garbage_u32(n * nat.MAX_CANDIDATES_EVENT), u32(n)
+    a = nat.EstimateInputSizeArgs(raw.data_ptr(), off.data_ptr(), est.data_ptr(), num.data_ptr(), cnum.data_ptr(),
+                                  cand.data_ptr(), None, None, consts.c.dev_velo_candidate_ks, status.data_ptr(),
+                                  n, nat.Tuning(*tuning))
+    nat.launch("velo_estimate_input_size_launch", a)
+    sync()
+    return (host(est).view(np.uint32), host(num).view(np.uint32), host(cnum).view(np.uint32),
+            host(cand).view(np.uint32), host(status).view(np.uint32))
+
+
+def prefix_sum(arr: np.ndarray) -> np.ndarray:
+    """Exclusive scan of arr[:-1]; total written at [-1]."""
+    n = len(arr) - 1
+    d = dev(arr.view(np.int32))
+    aux = garbage_u32(nat.prefix_sum_aux_size(n))
+    nat.launch("velo_prefix_sum_launch", nat.PrefixSumArgs(d.data_ptr(), aux.data_ptr(), n, nat.Tuning(0, 0)))
+    sync()
+    return host(d).view(np.uint32)
+
+
+def clustering(events, offsets, starts, consts: nat.DeviceConstants, tuning=(0, 0)):
+    n = len(offsets) - 1
+    total = int(starts[n * N_MODULES])
+    raw, off, st = dev(events), dev(offsets.view(np.int32)), dev(starts.view(np.int32))
+    num, status = garbage_u32(n * N_MODULES), u32(n)
+    cont = garbage_u32(6 * total + 1)
+    c = consts.c
+    a = nat.MaskedClusteringArgs(raw.data_ptr(), off.data_ptr(), st.data_ptr(), num.data_ptr(), None, None,
+                                 cont.data_ptr(), None, None, c.dev_velo_geometry, c.dev_velo_sp_patterns,
+                                 c.dev_velo_sp_fx, c.dev_velo_sp_fy, status.data_ptr(), n, nat.Tuning(*tuning))
+    nat.launch("velo_masked_clustering_launch", a)
+    sync()
+    return host(num).view(np.uint32), host(cont).view(np.uint32), host(status).view(np.uint32)
+
+
+def phi_sort(n, starts, nums, container, tuning=(0, 0)):
+    total = int(starts[n * N_MODULES])
+    st, nm, cont = dev(starts.view(np.int32)), dev(nums.view(np.int32)), dev(container.view(np.int32))
+    perm = garbage_u32(total + 1)
+    a = nat.CalculatePhiAndSortArgs(st.data_ptr(), nm.data_ptr(), cont.data_ptr(), perm.data_ptr(), n,
+                                    nat.Tuning(*tuning))
+    nat.launch("velo_calculate_phi_and_sort_launch", a)
+    sync()
+    return host(cont).view(np.uint32), host(perm).view(np.uint32)
+
+
+def fill_candidates(n, container, starts, nums):
+    total = int(starts[n * N_MODULES])
+    st, nm, cont = dev(starts.view(np.int32)), dev(nums.view(np.int32)), dev(container.view(np.int32))
+    h0 = torch.full((2 * total + 2,), -7, dtype=torch.int16, device="cuda")
+    h2 = torch.full((2 * total + 2,), -7, dtype=torch.int16, device="cuda")
+    a = nat.FillCandidatesArgs(cont.data_ptr(), st.data_ptr(), nm.data_ptr(), h0.data_ptr(), h2.data_ptr(), n,
+                               nat.Tuning(0, 0))
+    nat.launch("velo_fill_candidates_launch", a)
+    sync()
+    return host(h0), host(h2)
+
+
+def search_by_triplet(n, container, starts, nums, h0, h2, consts: nat.DeviceConstants):
+    total = int(starts[n * N_MODULES])
+    st, nm, cont = dev(starts.view(np.int32)), dev(nums.view(np.int32)), dev(container.view(np.int32))
+    d0, d2 = dev(h0), dev(h2)
+    tracks = torch.zeros(n * nat.MAX_TRACKS * nat.TRACK_HITS_BYTES, dtype=torch.uint8, device="cuda")
+    weak = torch.zeros(n * nat.MAX_WEAK_TRACKS * nat.TRACKLET_BYTES, dtype=torch.uint8, device="cuda")
+    ttf = garbage_u32(n * nat.TTF_MODULO * nat.TTF_ENTRY_WORDS)
+    used = torch.full((total + 1,), 7, dtype=torch.uint8, device="cuda")
+    atomics, status = garbage_u32(n * nat.NUM_ATOMICS + 1), u32(n)
+    a = nat.SearchByTripletArgs(cont.data_ptr(), st.data_ptr(), nm.data_ptr(), tracks.data_ptr(), None,
+                                ttf.data_ptr(), weak.data_ptr(), used.data_ptr(), atomics.data_ptr(), d0.data_ptr(),
+                                d2.data_ptr(), None, consts.c.dev_velo_module_zs, status.data_ptr(), n,
+                                nat.Tuning(0, 0))
+    nat.launch("velo_search_by_triplet_launch", a)
+    sync()
+    return (host(tracks), host(weak), host(used), host(atomics), host(status).view(np.uint32))
+
+
+def weak_tracks_adder(n, container, starts, tracks, weak, used, atomics):
+    st, cont = dev(starts.view(np.int32)), dev(container.view(np.int32))
+    dt, dw, du, da = dev(tracks), dev(weak), dev(used), dev(atomics.view(np.int32))
+    status = u32(n)
+    a = nat.WeakTracksAdderArgs(cont.data_ptr(), st.data_ptr(), dt.data_ptr(), dw.data_ptr(), du.data_ptr(),
+                                da.data_ptr(), status.data_ptr(), n, nat.Tuning(0, 0))
+    nat.launch("velo_weak_tracks_adder_launch", a)
+    sync()
+    return host(dt), host(da), host(status).view(np.uint32)
+
+
+def copy_and_prefix_sum(n, atomics):
+    da = dev(atomics.view(np.int32))
+    base = da.data_ptr()
+    a = nat.CopyAndPrefixSumArgs(base + 8 * n, base, base + 4 * n, n, nat.Tuning(0, 0))
+    nat.launch("velo_copy_and_prefix_sum_single_block_launch", a)
+    sync()
+    return host(da).view(np.uint32)
+
+
+def copy_track_hit_number(n, tracks, atomics):
+    total = int(atomics[2 * n])
+    dt, da = dev(tracks), dev(atomics.view(np.int32))
+    hn = garbage_u32(total + 1)
+    a = nat.CopyTrackHitNumberArgs(dt.data_ptr(), da.data_ptr(), hn.data_ptr(), n, nat.Tuning(0, 0))
+    nat.launch("velo_copy_velo_track_hit_number_launch", a)
+    sync()
+    return host(hn).view(np.uint32)[:total + 1]
+
+
+def consolidate(n, atomics, tracks, hit_number, container, starts, nums):
+    total_tracks = int(atomics[2 * n])
+    total_hits = int(hit_number[total_tracks])
+    da, dt, dh = dev(atomics.view(np.int32)), dev(tracks), dev(hit_number.view(np.int32))
+    cont, st, nm = dev(container.view(np.int32)), dev(starts.view(np.int32)), dev(nums.view(np.int32))
+    hits = torch.zeros(16 * total_hits + 16, dtype=torch.uint8, device="cuda")
+    states = torch.zeros(49 * total_tracks + 49, dtype=torch.uint8, device="cuda")
+    acc = torch.zeros(total_tracks + 1, dtype=torch.uint8, device="cuda")
+    a = nat.ConsolidateArgs(da.data_ptr(), dt.data_ptr(), dh.data_ptr(), cont.data_ptr(), st.data_ptr(),
+                            nm.data_ptr(), hits.data_ptr(), states.data_ptr(), acc.data_ptr(), n, nat.Tuning(0, 0))
+    nat.launch("velo_consolidate_velo_tracks_launch", a)
+    sync()
+    return host(hits), host(states), host(acc)

@@ -1,0 +1,81 @@
+"""Builds raw VP events from pixel lists (test helper).
+
+Encodes the reference's raw layout (ClusteringDefinitions.cu:3-21): 208 banks
+in sensor order, SP words sorted by address, bit 31 set exactly when none of
+the 8 neighbouring SPs of the same sensor is occupied.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def encode(pixels, n_sensors=208, sort=True):
+    """pixels: iterable of (sensor, col, row). Returns (events uint8, offsets uint32)."""
+    sps = {}
+    for s, c, r in pixels:
+        key = (int(s), (int(c) // 2 << 6) | (int(r) // 4))
+        sps[key] = sps.get(key, 0) | (1 << ((int(c) % 2) * 4 + int(r) % 4))
+    words = []
+    payload = []
+    offs = []
+    pos = 0
+    for s in range(n_sensors):
+        keys = [k for k in sps if k[0] == s]
+        if sort:
+            keys.sort(key=lambda k: k[1])
+        occ = {k[1] for k in keys}
+        bank = [s, len(keys)]
+        for k in keys:
+            addr = k[1]
+            col, row = addr >> 6, addr & 0x3F
+            neigh = any(((col + dc) << 6 | (row + dr)) in occ
+                        for dc in (-1, 0, 1) for dr in (-1, 0, 1)
+                        if (dc or dr) and 0 <= col + dc < 384 and 0 <= row + dr < 64)
+            bank.append((0 if neigh else 0x80000000) | (addr << 8) | sps[k])
+        offs.append(pos * 4)
+        payload.extend(bank)
+        pos += len(bank)
+    offs.append(pos * 4)
+    ev = np.array([n_sensors] + offs + payload, dtype=np.uint32)
+    return ev.view(np.uint8), np.array([0, ev.nbytes], np.uint32)
+
+
+def concat(events):
+    """Concatenate (events, offsets) pairs of single events."""
+    parts = [e for e, _ in events]
+    offs = np.zeros(len(parts) + 1, np.uint32)
+    offs[1:] = np.cumsum([p.nbytes for p in parts])
+    return np.concatenate(parts) if parts else np.zeros(0, np.uint8), offs
+
+
+def sensor_frames(geometry: bytes):
+    """(R 3x3, T 3) per sensor from the geometry blob's ltg (row-major R, T at 9..11)."""
+    g = np.frombuffer(geometry, np.float32)
+    ltg = g[1536:].reshape(208, 16).astype(np.float64)
+    return [(l[:9].reshape(3, 3), l[9:12]) for l in ltg], g[:768].astype(np.float64), g[768:1536].astype(np.float64)
+
+
+def line_pixels(geometry: bytes, x0, y0, z0, tx, ty, forward=True, rmin=5.1, rmax=42.0):
+    """Pixels (sensor, col, row) where the line through (x0,y0,z0) with slopes
+    (tx, ty) crosses an active sensor; one pixel per crossing."""
+    frames, local_x, pitch = sensor_frames(geometry)
+    edges = np.concatenate([[0.0], np.cumsum(pitch)])
+    out = []
+    for s, (R, T) in enumerate(frames):
+        z = T[2]
+        if (z - z0 <= 0) == forward:
+            continue
+        x = x0 + tx * (z - z0)
+        y = y0 + ty * (z - z0)
+        r = np.hypot(x, y)
+        if r < rmin or r > rmax:
+            continue
+        d = np.array([x, y, z]) - T
+        lx = R[0, 0] * d[0] + R[1, 0] * d[1]
+        ly = R[0, 1] * d[0] + R[1, 1] * d[1]
+        if lx < 0 or lx >= edges[-1] or ly < 0 or ly >= 256 * 0.055:
+            continue
+        col = int(np.searchsorted(edges, lx, side="right") - 1)
+        row = int(ly / 0.055)
+        out.append((s, col, row))
+    return out

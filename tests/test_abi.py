@@ -1,0 +1,42 @@
+"""The C-ABI libraries load and export every function include/*.h declares
+(no GPU needed: nothing is launched)."""
+import re
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared(header):
+    text = (ROOT / "include" / header).read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*[\w\s\*]+?\b(velo_\w+)\s*\(", text, flags=re.M)))
+
+
+@pytest.mark.parametrize("header,lib", [("velo_hlt.h", "libvelo_hlt.so"), ("velo_gen.h", "libvelo_gen.so")])
+def test_library_exports_declared_symbols(header, lib):
+    import ctypes
+    names = declared(header)
+    assert names, header
+    L = ctypes.CDLL(str(ROOT / "cuda_hlt_amd" / "lib" / lib))
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_python_structs_match_header_layout():
+    """Field order of the ctypes mirrors equals the header's structs."""
+    from cuda_hlt_amd import native as nat
+    text = (ROOT / "include" / "velo_hlt.h").read_text()
+    for m in re.finditer(r"typedef struct \{([^{}]*)\} (velo_\w+_args);", text):
+        body, name = m.groups()
+        fields = re.findall(r"\b(\w+);", re.sub(r"/\*.*?\*/", "", body, flags=re.S))
+        launcher = name[:-5] + "_launch"
+        st = nat.LAUNCHERS[launcher]
+        assert [f for f, _ in st._fields_] == fields, name
+
+
+def test_error_strings():
+    from cuda_hlt_amd import native as nat
+    assert nat.lib().velo_error_string(0)
+    assert b"gfx950" in nat.lib().velo_hlt_version()

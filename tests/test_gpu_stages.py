@@ -1,0 +1,282 @@
+"""Stage-by-stage parity of the gfx950 launchers against the CPU oracle.
+
+Every stage gets the oracle's output of the previous stage as input, so a
+mismatch names the stage that caused it.  Integer and index outputs must be
+identical; float outputs (x, y, z, phi, track states) must be identical bit
+for bit, which is stronger than the 1e-5 mm coordinate tolerance of the
+specification (asserted separately in test_coordinates_within_tolerance).
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+N_MODULES = 52
+
+
+@pytest.fixture(scope="module")
+def stages():
+    from tests import gpu_stages
+    return gpu_stages
+
+
+@pytest.fixture(scope="module")
+def consts(gpu, geometry, module_zs):
+    from cuda_hlt_amd import native as nat
+    c = nat.DeviceConstants(geometry, module_zs)
+    yield c
+    c.free()
+
+
+def module_mask(n, starts, nums):
+    """Boolean mask over [0, total) of slots that hold clusters."""
+    total = int(starts[n * N_MODULES])
+    m = np.zeros(total, bool)
+    s = starts[: n * N_MODULES].astype(np.int64)
+    k = nums[: n * N_MODULES].astype(np.int64)
+    for a, b in zip(s[k > 0], (s + k)[k > 0]):
+        m[a:b] = True
+    return m
+
+
+class Chain:
+    """Oracle outputs of every stage for one event sample."""
+
+    def __init__(self, ev, geometry, module_zs):
+        self.ev = ev
+        self.n = ev.n_events
+        n = self.n
+        self.est, self.num0, self.cnum, self.cand, self.st_est = po.estimate_input_size(ev.events, ev.offsets)
+        self.starts = po.prefix_sum(self.est)
+        self.nums, self.cont_raw, self.st_clu = po.masked_clustering(ev.events, ev.offsets, self.starts, geometry)
+        self.cont, self.perm = po.calculate_phi_and_sort(n, self.starts, self.nums, self.cont_raw)
+        self.h0, self.h2 = po.fill_candidates(n, self.cont, self.starts, self.nums)
+        (self.tracks_s, self.weak, self.used, self.atomics_s, self.st_search) = po.search_by_triplet(
+            n, self.cont, self.starts, self.nums, self.h0, self.h2, module_zs)
+        self.tracks, self.atomics_w, self.st_weak = po.weak_tracks_adder(
+            n, self.cont, self.starts, self.tracks_s, self.weak, self.used, self.atomics_s)
+        self.atomics = po.copy_and_prefix_sum(n, self.atomics_w)
+        self.hn_raw = po.copy_velo_track_hit_number(n, self.tracks, self.atomics)
+        self.hn = po.prefix_sum(self.hn_raw)
+        self.hits, self.states = po.consolidate(n, self.atomics, self.tracks, self.hn, self.cont, self.starts)
+        self.mask = module_mask(n, self.starts, self.nums)
+
+
+@pytest.fixture(scope="module")
+def chain(minbias_1000, geometry, module_zs):
+    return Chain(minbias_1000, geometry, module_zs)
+
+
+@pytest.fixture(scope="module")
+def chain_uneven(geometry, module_zs):
+    from cuda_hlt_amd import gen
+    return Chain(gen.uneven(300, seed=11), geometry, module_zs)
+
+
+def tracks_view(raw_bytes, n):
+    return np.frombuffer(raw_bytes.tobytes(), po.TRACK_DTYPE)[: n * po.MAX_TRACKS].reshape(n, po.MAX_TRACKS)
+
+
+def assert_tracks_equal(a, b, counts):
+    for e, c in enumerate(counts):
+        c = int(c)
+        np.testing.assert_array_equal(a[e, :c]["hitsNum"], b[e, :c]["hitsNum"], err_msg=f"event {e}")
+        for t in range(c):
+            k = int(a[e, t]["hitsNum"])
+            assert np.array_equal(a[e, t]["hits"][:k], b[e, t]["hits"][:k]), f"event {e} track {t}"
+
+
+def _check_estimate(stages, consts, c: Chain):
+    est, num, cnum, cand, st = stages.estimate(c.ev.events, c.ev.offsets, consts)
+    n = c.n
+    np.testing.assert_array_equal(est[: n * N_MODULES], c.est[: n * N_MODULES])
+    assert not num.any()
+    np.testing.assert_array_equal(cnum, c.cnum)
+    for e in range(n):
+        k = int(c.cnum[e])
+        np.testing.assert_array_equal(cand[e * 3000: e * 3000 + k], c.cand[e * 3000: e * 3000 + k])
+    np.testing.assert_array_equal(st, c.st_est)
+
+
+def test_estimate_input_size(stages, consts, chain):
+    _check_estimate(stages, consts, chain)
+
+
+def test_estimate_input_size_uneven(stages, consts, chain_uneven):
+    _check_estimate(stages, consts, chain_uneven)
+
+
+@pytest.mark.parametrize("block,tile", [(64, 64), (128, 256), (512, 1024)])
+def test_estimate_input_size_launch_shapes(stages, consts, chain, block, tile):
+    est, *_ = stages.estimate(chain.ev.events, chain.ev.offsets, consts, tuning=(block, tile))
+    np.testing.assert_array_equal(est[: chain.n * N_MODULES], chain.est[: chain.n * N_MODULES])
+
+
+def test_prefix_sum_cluster_offsets(stages, chain):
+    np.testing.assert_array_equal(stages.prefix_sum(chain.est), chain.starts)
+
+
+@pytest.mark.parametrize("size", [0, 1, 2, 511, 512, 513, 2047, 2048, 2049, 4097, 100003, 1000000])
+def test_prefix_sum_sizes(stages, size):
+    rng = np.random.default_rng(size)
+    a = np.zeros(size + 1, np.uint32)
+    a[:size] = rng.integers(0, 1000, size)
+    got = stages.prefix_sum(a)
+    want = np.zeros(size + 1, np.uint64)
+    want[1:] = np.cumsum(a[:size].astype(np.uint64))
+    np.testing.assert_array_equal(got.astype(np.uint64), want)
+
+
+def _check_clustering(stages, consts, c: Chain, tuning=(0, 0)):
+    num, cont, st = stages.clustering(c.ev.events, c.ev.offsets, c.starts, consts, tuning)
+    n = c.n
+    total = int(c.starts[n * N_MODULES])
+    np.testing.assert_array_equal(num[: n * N_MODULES], c.nums)
+    for s in range(4):
+        np.testing.assert_array_equal(cont[s * total: (s + 1) * total][c.mask],
+                                      c.cont_raw[s * total: (s + 1) * total][c.mask], err_msg=f"slot {s}")
+    np.testing.assert_array_equal(st, c.st_clu)
+
+
+def test_masked_clustering(stages, consts, chain):
+    _check_clustering(stages, consts, chain)
+
+
+def test_masked_clustering_uneven(stages, consts, chain_uneven):
+    _check_clustering(stages, consts, chain_uneven)
+
+
+@pytest.mark.parametrize("block,tile", [(64, 32), (512, 512)])
+def test_masked_clustering_launch_shapes(stages, consts, chain, block, tile):
+    _check_clustering(stages, consts, chain, (block, tile))
+
+
+def _check_sort(stages, c: Chain, tuning=(0, 0)):
+    cont, perm = stages.phi_sort(c.n, c.starts, c.nums, c.cont_raw, tuning)
+    total = int(c.starts[c.n * N_MODULES])
+    for s in (0, 1, 2, 4, 5):  # sorted slots, holes included (NaN)
+        np.testing.assert_array_equal(cont[s * total: (s + 1) * total], c.cont[s * total: (s + 1) * total],
+                                      err_msg=f"slot {s}")
+    np.testing.assert_array_equal(cont[3 * total: 4 * total][c.mask], c.cont[3 * total: 4 * total][c.mask])
+    np.testing.assert_array_equal(perm[:total][c.mask], c.perm[:total][c.mask])
+
+
+def test_calculate_phi_and_sort(stages, chain):
+    _check_sort(stages, chain)
+
+
+def test_calculate_phi_and_sort_uneven(stages, chain_uneven):
+    _check_sort(stages, chain_uneven, (512, 0))
+
+
+def _check_fill(stages, c: Chain):
+    h0, h2 = stages.fill_candidates(c.n, c.cont, c.starts, c.nums)
+    total = int(c.starts[c.n * N_MODULES])
+    sel = np.zeros(total, bool)
+    for e in range(c.n):
+        for m in range(2, 50):
+            s, k = int(c.starts[e * 52 + m]), int(c.nums[e * 52 + m])
+            sel[s:s + k] = True
+    idx = np.nonzero(sel)[0]
+    both = np.concatenate([2 * idx, 2 * idx + 1])
+    np.testing.assert_array_equal(h0[both], c.h0[both])
+    np.testing.assert_array_equal(h2[both], c.h2[both])
+
+
+def test_fill_candidates(stages, chain):
+    _check_fill(stages, chain)
+
+
+def test_fill_candidates_uneven(stages, chain_uneven):
+    _check_fill(stages, chain_uneven)
+
+
+def _check_search(stages, consts, c: Chain):
+    n = c.n
+    tracks, weak, used, atomics, st = stages.search_by_triplet(n, c.cont, c.starts, c.nums, c.h0, c.h2, consts)
+    atomics = atomics.view(np.int32)
+    np.testing.assert_array_equal(atomics[:n], c.atomics_s[:n])
+    assert_tracks_equal(tracks_view(tracks, n), c.tracks_s.reshape(n, po.MAX_TRACKS), c.atomics_s[:n])
+    nw = c.atomics_s[n: 5 * n: 4]
+    np.testing.assert_array_equal(atomics[n: 5 * n: 4], nw)
+    wv = np.frombuffer(weak.tobytes(), po.TRACKLET_DTYPE).reshape(n, po.MAX_WEAK_TRACKS)
+    ow = c.weak.reshape(n, po.MAX_WEAK_TRACKS)
+    for e in range(n):
+        np.testing.assert_array_equal(wv[e, :nw[e]]["hits"], ow[e, :nw[e]]["hits"], err_msg=f"event {e}")
+    total = int(c.starts[n * N_MODULES])
+    np.testing.assert_array_equal(used[:total], c.used[:total])
+    np.testing.assert_array_equal(st, c.st_search)
+    return tracks
+
+
+def test_search_by_triplet(stages, consts, chain):
+    _check_search(stages, consts, chain)
+
+
+def test_search_by_triplet_uneven(stages, consts, chain_uneven):
+    _check_search(stages, consts, chain_uneven)
+
+
+def test_weak_tracks_adder(stages, chain):
+    n = chain.n
+    tracks_in = chain.tracks_s.view(np.uint8)
+    weak_in = chain.weak.view(np.uint8)
+    tracks, atomics, st = stages.weak_tracks_adder(n, chain.cont, chain.starts, tracks_in, weak_in, chain.used,
+                                                   chain.atomics_s)
+    atomics = atomics.view(np.int32)
+    np.testing.assert_array_equal(atomics[:n], chain.atomics_w[:n])
+    assert_tracks_equal(tracks_view(tracks, n), chain.tracks.reshape(n, po.MAX_TRACKS), chain.atomics_w[:n])
+    np.testing.assert_array_equal(st, chain.st_weak)
+
+
+def test_copy_and_prefix_sum(stages, chain):
+    n = chain.n
+    got = stages.copy_and_prefix_sum(n, chain.atomics_w.astype(np.uint32))
+    np.testing.assert_array_equal(got[: 2 * n + 1], chain.atomics[: 2 * n + 1])
+
+
+def test_copy_track_hit_number_and_scan(stages, chain):
+    n = chain.n
+    hn = stages.copy_track_hit_number(n, chain.tracks.view(np.uint8), chain.atomics)
+    total = int(chain.atomics[2 * n])
+    np.testing.assert_array_equal(hn[:total], chain.hn_raw[:total])
+    np.testing.assert_array_equal(stages.prefix_sum(chain.hn_raw), chain.hn)
+
+
+def test_consolidate(stages, chain):
+    n = chain.n
+    hits, states, acc = stages.consolidate(n, chain.atomics, chain.tracks.view(np.uint8), chain.hn, chain.cont,
+                                           chain.starts, chain.nums)
+    T = int(chain.atomics[2 * n])
+    Ht = int(chain.hn[T])
+    np.testing.assert_array_equal(hits[: 16 * Ht], chain.hits[: 16 * Ht])
+    np.testing.assert_array_equal(states[: 49 * T], chain.states[: 49 * T])
+    assert acc[:T].all()
+
+
+def test_coordinates_within_tolerance(stages, consts, chain):
+    """Cluster x, y, z agree with the oracle within 1e-5 mm."""
+    n = chain.n
+    _, cont, _ = stages.clustering(chain.ev.events, chain.ev.offsets, chain.starts, consts)
+    total = int(chain.starts[n * N_MODULES])
+    for s in range(3):
+        g = cont[s * total: (s + 1) * total][chain.mask].view(np.float32)
+        o = chain.cont_raw[s * total: (s + 1) * total][chain.mask].view(np.float32)
+        assert np.max(np.abs(g.astype(np.float64) - o)) <= 1e-5
+
+
+def test_empty_batch_is_a_no_op(stages, consts):
+    from cuda_hlt_amd import native as nat
+    a = nat.EstimateInputSizeArgs()
+    a.n_events = 0
+    a.dev_raw_input = a.dev_raw_input_offsets = a.dev_estimated_input_size = 1
+    a.dev_module_cluster_num = a.dev_event_candidate_num = a.dev_cluster_candidates = 1
+    nat.launch("velo_estimate_input_size_launch", a)
+
+
+def test_null_argument_is_rejected(stages):
+    from cuda_hlt_amd import native as nat
+    with pytest.raises(nat.VeloError):
+        nat.launch("velo_search_by_triplet_launch", nat.SearchByTripletArgs())
