@@ -1,0 +1,170 @@
+#!/usr/bin/env python3
+"""VELO reconstruction throughput on one MI355X.
+
+One step = one launch of the plain VELO sequence (raw SuperPixel banks to
+consolidated tracks, the C++ Stream over the configured Velo sequence) on a
+batch of --events-per-launch synthetic minimum-bias events (nu = 7.6) that
+are resident in HBM before the timed region.  The timed region runs
+--steps launches on distinct batches; value = events / second.
+
+Also reported: per-launch latency p50/p99, per-step kernel times from HIP
+events recorded on the sequence's stream, the HBM roofline of the dominant
+kernel from its algorithmic bytes, the tracking efficiency of the output,
+and the CPU oracle timed on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "VELO events/s on 1 MI355X, plain vs batched+NN at equal eff.; p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(S, H, C, T, Ht, N):
+    """HBM bytes each step must move at least once (per batch).
+    S SP words, H clusters, C candidates, T tracks, Ht hits on tracks, N events."""
+    raw = 4 * S + 2504 * N
+    return {
+        "velo_estimate_input_size_t": raw + 4 * C + N * (2 * 52 * 4 + 4),
+        "prefix_sum_velo_clusters_t": 8 * (52 * N),
+        "velo_masked_clustering_t": raw + 16 * H + N * (2 * 52 * 4),
+        "velo_calculate_phi_and_sort_t": 16 * H + 4 * H + 16 * H + 4 * H,
+        "velo_fill_candidates_t": 4 * H + 8 * H,
+        "velo_search_by_triplet_t": 16 * H + 12 * H + 54 * T,
+        "velo_weak_tracks_adder_t": 0,
+        "copy_and_prefix_sum_single_block_velo_t": 8 * N,
+        "copy_velo_track_hit_number_t": 2 * T + 4 * T,
+        "prefix_sum_velo_track_hit_number_t": 8 * T,
+        "consolidate_velo_tracks_t": 54 * T + 16 * Ht + 16 * Ht + 49 * T,
+    }
+
+
+def cpu_baseline(ev, geometry, zs, budget_s=12.0):
+    """The oracle (single thread, C) on the first events of the workload."""
+    from oracle import pyoracle as po
+    n = 200
+    while True:
+        sub = ev.subset(0, min(n, ev.n_events))
+        t0 = time.perf_counter()
+        po.pipeline_c(sub.events, sub.offsets, geometry, zs)
+        dt = time.perf_counter() - t0
+        if dt > budget_s / 4 or n >= ev.n_events:
+            break
+        n = min(ev.n_events, int(n * max(2.0, budget_s / 4 / max(dt, 1e-3))))
+    return {"value": sub.n_events / dt, "unit": "events/s", "cores": 1, "kind": "port",
+            "sample": f"first {sub.n_events} events of the batch, single-threaded C oracle, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--events-per-launch", type=int, default=10000)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(events shard across GPUs with no exchange; see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from cuda_hlt_amd import gen
+    from cuda_hlt_amd.stream import VeloSequence
+
+    B = args.events_per_launch
+    n_batches = args.steps + args.warmup
+    t0 = time.perf_counter()
+    ev = gen.minbias(B * n_batches, seed=args.seed)
+    log(f"[bench] generated {ev.n_events} events in {time.perf_counter() - t0:.1f} s "
+        f"({ev.events.nbytes / 1e9:.2f} GB raw)")
+    geometry, zs = gen.geometry(), gen.module_zs()
+
+    dev = torch.device("cuda:0")
+    raw = torch.from_numpy(ev.events).to(dev)
+    batch_offsets = []
+    for b in range(n_batches):
+        o = ev.offsets[b * B: (b + 1) * B + 1].astype(np.int64)
+        batch_offsets.append((int(o[0]), torch.from_numpy((o - o[0]).astype(np.uint32).view(np.int32)).to(dev)))
+    torch.cuda.synchronize()
+
+    seq = VeloSequence(geometry, zs, max_events=B, reserve_mb=int(os.environ.get("VELO_RESERVE_MB", "8192")))
+    seq.enable_step_timing(True)
+
+    def run(b):
+        base, off = batch_offsets[b]
+        return seq.run(B, dev_events=raw.data_ptr() + base, dev_offsets=off.data_ptr(), copy_outputs=False)
+
+    for w in range(args.warmup):
+        run(args.steps + w)
+    torch.cuda.synchronize()
+    lat, steps_ms = [], {}
+    totals = {"T": 0, "Ht": 0, "H_est": 0}
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        t1 = time.perf_counter()
+        o = run(k)
+        lat.append(time.perf_counter() - t1)
+        for name, ms in seq.step_times():
+            steps_ms[name] = steps_ms.get(name, 0.0) + ms
+        totals["T"] += o.total_tracks
+        totals["Ht"] += o.total_track_hits
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    value = args.steps * B / elapsed
+
+    # per-batch sizes for the algorithmic bytes (batch 0 stands for all)
+    from cuda_hlt_amd import native as nat
+    o0 = run(0)
+    host_nums = nat.d2h(o0.dev_module_cluster_num, 4 * B * 52).view(np.uint32)
+    H = int(host_nums.sum())
+    S = int(ev.sp_counts()[:B].sum())
+    C_cands = 0
+    ab = algorithmic_bytes(S, H, C_cands, o0.total_tracks, o0.total_track_hits, B)
+    per_step = {k: v / args.steps for k, v in steps_ms.items()}
+    dominant = max((k for k in per_step if k != "end"), key=lambda k: per_step[k])
+    kernels = {}
+    for k, ms in per_step.items():
+        b = ab.get(k, 0)
+        kernels[k] = {"ms": round(ms, 4), "algorithmic_bytes": b,
+                      "GBps": round(b / (ms * 1e-3) / 1e9, 1) if ms > 0 else None}
+    dom_ms = per_step[dominant]
+    achieved = ab[dominant] / (dom_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None}
+
+    result = {
+        "metric": METRIC, "value": round(value, 1), "unit": "events/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"plain VELO sequence, minbias nu=7.6, {B}-event launches, "
+                               f"{args.steps * B} distinct events timed (seed {args.seed})",
+                   "events_per_launch": B, "path": "plain"},
+        "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 3),
+                       "p99": round(1e3 * float(np.percentile(lat, 99)), 3)},
+        "roofline": roofline,
+        "kernels": kernels,
+        "batch": {"sp_words": S, "clusters": H, "tracks": o0.total_tracks, "track_hits": o0.total_track_hits},
+    }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(ev.subset(0, B), geometry, zs)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

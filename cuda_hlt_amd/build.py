@@ -54,7 +54,7 @@ def kernel_sources() -> list[Path]:
 
 
 def host_sources() -> list[Path]:
-    return sorted((CSRC / "host").glob("*.cpp"))
+    return sorted((CSRC / "host").rglob("*.cpp"))
 
 
 def build_velo_hlt(jobs: int = 8) -> Path:

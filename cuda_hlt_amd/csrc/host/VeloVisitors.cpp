@@ -1,0 +1,457 @@
+// SequenceVisitor specialisations of the VELO sequence.  Sizing and the
+// three host reads of totals follow the reference's visitors
+// (stream/visitors/velo/src/*.cu, InitEventListVisitor.cu:4-89); launches go
+// through the C ABI and are checked.
+#include <cstring>
+
+#include "sequence/SequenceVisitor.h"
+
+namespace {
+template<typename S>
+void tune(S& state, const velo_tuning& t)
+{
+  state.handler.set_tuning(t);
+}
+} // namespace
+
+// ---- init_event_list_t ------------------------------------------------------
+template<>
+void SequenceVisitor::set_arguments_size<init_event_list_t>(
+  init_event_list_t::arguments_t arguments,
+  const RuntimeOptions& runtime_options,
+  const Constants&,
+  const HostBuffers&)
+{
+  if (runtime_options.dev_velopix_events) {
+    arguments.set_external<dev_velo_raw_input>(
+      const_cast<char*>(runtime_options.dev_velopix_events), runtime_options.host_velopix_events_size);
+    arguments.set_external<dev_velo_raw_input_offsets>(
+      const_cast<unsigned*>(runtime_options.dev_velopix_event_offsets),
+      (runtime_options.number_of_events + 1) * sizeof(unsigned));
+  }
+  else {
+    arguments.clear_external<dev_velo_raw_input>();
+    arguments.clear_external<dev_velo_raw_input_offsets>();
+    arguments.set_size<dev_velo_raw_input>(runtime_options.host_velopix_events_size);
+    arguments.set_size<dev_velo_raw_input_offsets>(runtime_options.host_velopix_event_offsets_size);
+  }
+  arguments.set_size<dev_event_list>(runtime_options.number_of_events);
+  arguments.set_size<dev_number_of_selected_events>(1);
+}
+
+template<>
+void SequenceVisitor::visit<init_event_list_t>(
+  init_event_list_t& state,
+  const init_event_list_t::arguments_t& arguments,
+  const RuntimeOptions& runtime_options,
+  const Constants&,
+  HostBuffers&,
+  hipStream_t& stream,
+  hipEvent_t&)
+{
+  if (!runtime_options.dev_velopix_events) {
+    hipCheck(hipMemcpyAsync(
+      arguments.offset<dev_velo_raw_input>(), runtime_options.host_velopix_events,
+      arguments.size<dev_velo_raw_input>(), hipMemcpyHostToDevice, stream));
+    hipCheck(hipMemcpyAsync(
+      arguments.offset<dev_velo_raw_input_offsets>(), runtime_options.host_velopix_event_offsets,
+      arguments.size<dev_velo_raw_input_offsets>(), hipMemcpyHostToDevice, stream));
+  }
+  state.set_opts(dim3(runtime_options.number_of_events), dim3(256), stream);
+  state.set_arguments(arguments.offset<dev_event_list>());
+  state.invoke();
+}
+
+// ---- velo_estimate_input_size_t ----------------------------------------------
+template<>
+void SequenceVisitor::set_arguments_size<velo_estimate_input_size_t>(
+  velo_estimate_input_size_t::arguments_t arguments,
+  const RuntimeOptions&,
+  const Constants&,
+  const HostBuffers& host_buffers)
+{
+  const size_t n = host_buffers.host_number_of_selected_events[0];
+  arguments.set_size<dev_estimated_input_size>(n * VELO_N_MODULES + 1);
+  arguments.set_size<dev_module_cluster_num>(n * VELO_N_MODULES);
+  arguments.set_size<dev_module_candidate_num>(n);
+  arguments.set_size<dev_cluster_candidates>(n * VELO_MAX_CANDIDATES_EVENT);
+  arguments.set_size<dev_event_order>(n);
+  arguments.set_size<dev_velo_event_status>(n);
+}
+
+template<>
+void SequenceVisitor::visit<velo_estimate_input_size_t>(
+  velo_estimate_input_size_t& state,
+  const velo_estimate_input_size_t::arguments_t& arguments,
+  const RuntimeOptions& runtime_options,
+  const Constants& constants,
+  HostBuffers& host_buffers,
+  hipStream_t& stream,
+  hipEvent_t&)
+{
+  state.set_opts(dim3(host_buffers.host_number_of_selected_events[0]), dim3(256), stream);
+  state.set_arguments(
+    arguments.offset<dev_velo_raw_input>(),
+    arguments.offset<dev_velo_raw_input_offsets>(),
+    arguments.offset<dev_estimated_input_size>(),
+    arguments.offset<dev_module_cluster_num>(),
+    arguments.offset<dev_module_candidate_num>(),
+    arguments.offset<dev_cluster_candidates>(),
+    arguments.offset<dev_event_list>(),
+    arguments.offset<dev_event_order>(),
+    constants.dev_velo_candidate_ks,
+    arguments.offset<dev_velo_event_status>());
+  tune(state, runtime_options.tuning.estimate_input_size);
+  state.invoke();
+}
+
+// ---- prefix_sum_velo_clusters_t (host sync #1) --------------------------------
+template<>
+void SequenceVisitor::set_arguments_size<prefix_sum_velo_clusters_t>(
+  prefix_sum_velo_clusters_t::arguments_t arguments,
+  const RuntimeOptions&,
+  const Constants&,
+  const HostBuffers& host_buffers)
+{
+  arguments.set_size<dev_cluster_offset>(
+    prefix_sum_velo_clusters_t::aux_array_size(host_buffers.host_number_of_selected_events[0] * VELO_N_MODULES));
+}
+
+template<>
+void SequenceVisitor::visit<prefix_sum_velo_clusters_t>(
+  prefix_sum_velo_clusters_t& state,
+  const prefix_sum_velo_clusters_t::arguments_t& arguments,
+  const RuntimeOptions&,
+  const Constants&,
+  HostBuffers& host_buffers,
+  hipStream_t& stream,
+  hipEvent_t& event)
+{
+  const size_t n = host_buffers.host_number_of_selected_events[0];
+  state.set_size(n * VELO_N_MODULES);
+  state.set_opts(stream);
+  state.set_arguments(arguments.offset<dev_estimated_input_size>(), arguments.offset<dev_cluster_offset>());
+  state.invoke();
+  hipCheck(hipMemcpyAsync(
+    host_buffers.host_total_number_of_velo_clusters, arguments.offset<dev_estimated_input_size>() + n * VELO_N_MODULES,
+    sizeof(unsigned), hipMemcpyDeviceToHost, stream));
+  hipCheck(hipEventRecord(event, stream));
+  hipCheck(hipEventSynchronize(event));
+}
+
+// ---- velo_masked_clustering_t -------------------------------------------------
+template<>
+void SequenceVisitor::set_arguments_size<velo_masked_clustering_t>(
+  velo_masked_clustering_t::arguments_t arguments,
+  const RuntimeOptions&,
+  const Constants&,
+  const HostBuffers& host_buffers)
+{
+  arguments.set_size<dev_velo_cluster_container>(6 * (size_t) host_buffers.host_total_number_of_velo_clusters[0]);
+}
+
+template<>
+void SequenceVisitor::visit<velo_masked_clustering_t>(
+  velo_masked_clustering_t& state,
+  const velo_masked_clustering_t::arguments_t& arguments,
+  const RuntimeOptions& runtime_options,
+  const Constants& constants,
+  HostBuffers& host_buffers,
+  hipStream_t& stream,
+  hipEvent_t&)
+{
+  state.set_opts(dim3(host_buffers.host_number_of_selected_events[0]), dim3(256), stream);
+  state.set_arguments(
+    arguments.offset<dev_velo_raw_input>(),
+    arguments.offset<dev_velo_raw_input_offsets>(),
+    arguments.offset<dev_estimated_input_size>(),
+    arguments.offset<dev_module_cluster_num>(),
+    arguments.offset<dev_module_candidate_num>(),
+    arguments.offset<dev_cluster_candidates>(),
+    arguments.offset<dev_velo_cluster_container>(),
+    arguments.offset<dev_event_list>(),
+    arguments.offset<dev_event_order>(),
+    constants.dev_velo_geometry,
+    constants.dev_velo_sp_patterns,
+    constants.dev_velo_sp_fx,
+    constants.dev_velo_sp_fy,
+    arguments.offset<dev_velo_event_status>());
+  tune(state, runtime_options.tuning.masked_clustering);
+  state.invoke();
+}
+
+// ---- velo_calculate_phi_and_sort_t --------------------------------------------
+template<>
+void SequenceVisitor::set_arguments_size<velo_calculate_phi_and_sort_t>(
+  velo_calculate_phi_and_sort_t::arguments_t arguments,
+  const RuntimeOptions&,
+  const Constants&,
+  const HostBuffers& host_buffers)
+{
+  arguments.set_size<dev_hit_permutation>(host_buffers.host_total_number_of_velo_clusters[0]);
+}
+
+template<>
+void SequenceVisitor::visit<velo_calculate_phi_and_sort_t>(
+  velo_calculate_phi_and_sort_t& state,
+  const velo_calculate_phi_and_sort_t::arguments_t& arguments,
+  const RuntimeOptions& runtime_options,
+  const Constants&,
+  HostBuffers& host_buffers,
+  hipStream_t& stream,
+  hipEvent_t&)
+{
+  state.set_opts(dim3(host_buffers.host_number_of_selected_events[0]), dim3(256), stream);
+  state.set_arguments(
+    arguments.offset<dev_estimated_input_size>(),
+    arguments.offset<dev_module_cluster_num>(),
+    arguments.offset<dev_velo_cluster_container>(),
+    arguments.offset<dev_hit_permutation>());
+  tune(state, runtime_options.tuning.calculate_phi_and_sort);
+  state.invoke();
+}
+
+// ---- velo_fill_candidates_t ----------------------------------------------------
+template<>
+void SequenceVisitor::set_arguments_size<velo_fill_candidates_t>(
+  velo_fill_candidates_t::arguments_t arguments,
+  const RuntimeOptions&,
+  const Constants&,
+  const HostBuffers& host_buffers)
+{
+  arguments.set_size<dev_h0_candidates>(2 * (size_t) host_buffers.host_total_number_of_velo_clusters[0]);
+  arguments.set_size<dev_h2_candidates>(2 * (size_t) host_buffers.host_total_number_of_velo_clusters[0]);
+}
+
+template<>
+void SequenceVisitor::visit<velo_fill_candidates_t>(
+  velo_fill_candidates_t& state,
+  const velo_fill_candidates_t::arguments_t& arguments,
+  const RuntimeOptions& runtime_options,
+  const Constants&,
+  HostBuffers& host_buffers,
+  hipStream_t& stream,
+  hipEvent_t&)
+{
+  state.set_opts(dim3(host_buffers.host_number_of_selected_events[0]), dim3(256), stream);
+  state.set_arguments(
+    arguments.offset<dev_velo_cluster_container>(),
+    arguments.offset<dev_estimated_input_size>(),
+    arguments.offset<dev_module_cluster_num>(),
+    arguments.offset<dev_h0_candidates>(),
+    arguments.offset<dev_h2_candidates>());
+  tune(state, runtime_options.tuning.fill_candidates);
+  state.invoke();
+}
+
+// ---- velo_search_by_triplet_t --------------------------------------------------
+template<>
+void SequenceVisitor::set_arguments_size<velo_search_by_triplet_t>(
+  velo_search_by_triplet_t::arguments_t arguments,
+  const RuntimeOptions&,
+  const Constants&,
+  const HostBuffers& host_buffers)
+{
+  const size_t n = host_buffers.host_number_of_selected_events[0];
+  arguments.set_size<dev_tracks>(n * VELO_MAX_TRACKS);
+  arguments.set_size<dev_tracklets>(0); // tracklet hits travel in the tracks-to-follow entries
+  arguments.set_size<dev_tracks_to_follow>(n * VELO_TTF_MODULO * VELO_TTF_ENTRY_WORDS);
+  arguments.set_size<dev_weak_tracks>(n * VELO_MAX_WEAK_TRACKS);
+  arguments.set_size<dev_hit_used>(host_buffers.host_total_number_of_velo_clusters[0]);
+  arguments.set_size<dev_atomics_velo>(n * VELO_NUM_ATOMICS + 1);
+  arguments.set_size<dev_rel_indices>(0); // h1 candidates are staged in LDS
+}
+
+template<>
+void SequenceVisitor::visit<velo_search_by_triplet_t>(
+  velo_search_by_triplet_t& state,
+  const velo_search_by_triplet_t::arguments_t& arguments,
+  const RuntimeOptions& runtime_options,
+  const Constants& constants,
+  HostBuffers& host_buffers,
+  hipStream_t& stream,
+  hipEvent_t&)
+{
+  state.set_opts(dim3(host_buffers.host_number_of_selected_events[0]), dim3(64), stream);
+  state.set_arguments(
+    arguments.offset<dev_velo_cluster_container>(),
+    arguments.offset<dev_estimated_input_size>(),
+    arguments.offset<dev_module_cluster_num>(),
+    arguments.offset<dev_tracks>(),
+    arguments.offset<dev_tracklets>(),
+    arguments.offset<dev_tracks_to_follow>(),
+    arguments.offset<dev_weak_tracks>(),
+    arguments.offset<dev_hit_used>(),
+    arguments.offset<dev_atomics_velo>(),
+    arguments.offset<dev_h0_candidates>(),
+    arguments.offset<dev_h2_candidates>(),
+    arguments.offset<dev_rel_indices>(),
+    constants.dev_velo_module_zs,
+    arguments.offset<dev_velo_event_status>());
+  tune(state, runtime_options.tuning.search_by_triplet);
+  state.invoke();
+}
+
+// ---- velo_weak_tracks_adder_t ---------------------------------------------------
+DEFINE_EMPTY_SET_ARGUMENTS_SIZE(velo_weak_tracks_adder_t)
+
+template<>
+void SequenceVisitor::visit<velo_weak_tracks_adder_t>(
+  velo_weak_tracks_adder_t& state,
+  const velo_weak_tracks_adder_t::arguments_t& arguments,
+  const RuntimeOptions&,
+  const Constants&,
+  HostBuffers& host_buffers,
+  hipStream_t& stream,
+  hipEvent_t&)
+{
+  state.set_opts(dim3(host_buffers.host_number_of_selected_events[0]), dim3(64), stream);
+  state.set_arguments(
+    arguments.offset<dev_velo_cluster_container>(),
+    arguments.offset<dev_estimated_input_size>(),
+    arguments.offset<dev_tracks>(),
+    arguments.offset<dev_weak_tracks>(),
+    arguments.offset<dev_hit_used>(),
+    arguments.offset<dev_atomics_velo>(),
+    arguments.offset<dev_velo_event_status>());
+  state.invoke();
+}
+
+// ---- copy_and_prefix_sum_single_block_velo_t (host sync #2) ----------------------
+DEFINE_EMPTY_SET_ARGUMENTS_SIZE(copy_and_prefix_sum_single_block_velo_t)
+
+template<>
+void SequenceVisitor::visit<copy_and_prefix_sum_single_block_velo_t>(
+  copy_and_prefix_sum_single_block_velo_t& state,
+  const copy_and_prefix_sum_single_block_velo_t::arguments_t& arguments,
+  const RuntimeOptions&,
+  const Constants&,
+  HostBuffers& host_buffers,
+  hipStream_t& stream,
+  hipEvent_t& event)
+{
+  const unsigned n = host_buffers.host_number_of_selected_events[0];
+  unsigned* atomics = reinterpret_cast<unsigned*>(arguments.offset<dev_atomics_velo>());
+  state.set_opts(dim3(1), dim3(1024), stream);
+  state.set_arguments(atomics + 2 * n, atomics, atomics + n, n);
+  state.invoke();
+  hipCheck(hipMemcpyAsync(
+    host_buffers.host_number_of_reconstructed_velo_tracks, atomics + 2 * n, sizeof(unsigned), hipMemcpyDeviceToHost,
+    stream));
+  hipCheck(hipEventRecord(event, stream));
+  hipCheck(hipEventSynchronize(event));
+}
+
+// ---- copy_velo_track_hit_number_t ------------------------------------------------
+template<>
+void SequenceVisitor::set_arguments_size<copy_velo_track_hit_number_t>(
+  copy_velo_track_hit_number_t::arguments_t arguments,
+  const RuntimeOptions&,
+  const Constants&,
+  const HostBuffers& host_buffers)
+{
+  arguments.set_size<dev_velo_track_hit_number>(host_buffers.velo_track_hit_number_size());
+}
+
+template<>
+void SequenceVisitor::visit<copy_velo_track_hit_number_t>(
+  copy_velo_track_hit_number_t& state,
+  const copy_velo_track_hit_number_t::arguments_t& arguments,
+  const RuntimeOptions&,
+  const Constants&,
+  HostBuffers& host_buffers,
+  hipStream_t& stream,
+  hipEvent_t&)
+{
+  state.set_opts(dim3(host_buffers.host_number_of_selected_events[0]), dim3(256), stream);
+  state.set_arguments(
+    arguments.offset<dev_tracks>(), arguments.offset<dev_atomics_velo>(), arguments.offset<dev_velo_track_hit_number>());
+  state.invoke();
+}
+
+// ---- prefix_sum_velo_track_hit_number_t (host sync #3) ----------------------------
+template<>
+void SequenceVisitor::set_arguments_size<prefix_sum_velo_track_hit_number_t>(
+  prefix_sum_velo_track_hit_number_t::arguments_t arguments,
+  const RuntimeOptions&,
+  const Constants&,
+  const HostBuffers& host_buffers)
+{
+  arguments.set_size<dev_prefix_sum_auxiliary_array_2>(
+    prefix_sum_velo_track_hit_number_t::aux_array_size(host_buffers.host_number_of_reconstructed_velo_tracks[0]));
+}
+
+template<>
+void SequenceVisitor::visit<prefix_sum_velo_track_hit_number_t>(
+  prefix_sum_velo_track_hit_number_t& state,
+  const prefix_sum_velo_track_hit_number_t::arguments_t& arguments,
+  const RuntimeOptions&,
+  const Constants&,
+  HostBuffers& host_buffers,
+  hipStream_t& stream,
+  hipEvent_t& event)
+{
+  const unsigned t = host_buffers.host_number_of_reconstructed_velo_tracks[0];
+  state.set_size(t);
+  state.set_opts(stream);
+  state.set_arguments(arguments.offset<dev_velo_track_hit_number>(), arguments.offset<dev_prefix_sum_auxiliary_array_2>());
+  state.invoke();
+  hipCheck(hipMemcpyAsync(
+    host_buffers.host_accumulated_number_of_hits_in_velo_tracks, arguments.offset<dev_velo_track_hit_number>() + t,
+    sizeof(unsigned), hipMemcpyDeviceToHost, stream));
+  hipCheck(hipEventRecord(event, stream));
+  hipCheck(hipEventSynchronize(event));
+}
+
+// ---- consolidate_velo_tracks_t ------------------------------------------------------
+template<>
+void SequenceVisitor::set_arguments_size<consolidate_velo_tracks_t>(
+  consolidate_velo_tracks_t::arguments_t arguments,
+  const RuntimeOptions&,
+  const Constants&,
+  const HostBuffers& host_buffers)
+{
+  const size_t t = host_buffers.host_number_of_reconstructed_velo_tracks[0];
+  arguments.set_size<dev_velo_track_hits>(host_buffers.host_accumulated_number_of_hits_in_velo_tracks[0] * sizeof(Velo::Hit));
+  arguments.set_size<dev_velo_states>(t * sizeof(VeloState));
+  arguments.set_size<dev_accepted_velo_tracks>(t);
+}
+
+template<>
+void SequenceVisitor::visit<consolidate_velo_tracks_t>(
+  consolidate_velo_tracks_t& state,
+  const consolidate_velo_tracks_t::arguments_t& arguments,
+  const RuntimeOptions& runtime_options,
+  const Constants&,
+  HostBuffers& host_buffers,
+  hipStream_t& stream,
+  hipEvent_t&)
+{
+  const unsigned n = host_buffers.host_number_of_selected_events[0];
+  state.set_opts(dim3(n), dim3(256), stream);
+  state.set_arguments(
+    arguments.offset<dev_atomics_velo>(),
+    arguments.offset<dev_tracks>(),
+    arguments.offset<dev_velo_track_hit_number>(),
+    arguments.offset<dev_velo_cluster_container>(),
+    arguments.offset<dev_estimated_input_size>(),
+    arguments.offset<dev_module_cluster_num>(),
+    arguments.offset<dev_velo_track_hits>(),
+    arguments.offset<dev_velo_states>(),
+    arguments.offset<dev_accepted_velo_tracks>());
+  tune(state, runtime_options.tuning.consolidate);
+  state.invoke();
+  if (!runtime_options.copy_outputs_to_host) return;
+  // ConsolidateVeloVisitor.cu:47-66: device to host
+  const size_t t = host_buffers.host_number_of_reconstructed_velo_tracks[0];
+  const size_t h = host_buffers.host_accumulated_number_of_hits_in_velo_tracks[0];
+  host_buffers.ensure_outputs(t, h);
+  hipCheck(hipMemcpyAsync(host_buffers.host_atomics_velo, arguments.offset<dev_atomics_velo>(),
+                          (2 * (size_t) n + 1) * sizeof(unsigned), hipMemcpyDeviceToHost, stream));
+  hipCheck(hipMemcpyAsync(host_buffers.host_velo_track_hit_number, arguments.offset<dev_velo_track_hit_number>(),
+                          (t + 1) * sizeof(unsigned), hipMemcpyDeviceToHost, stream));
+  hipCheck(hipMemcpyAsync(host_buffers.host_velo_track_hits, arguments.offset<dev_velo_track_hits>(), h * sizeof(Velo::Hit),
+                          hipMemcpyDeviceToHost, stream));
+  hipCheck(hipMemcpyAsync(host_buffers.host_velo_states, arguments.offset<dev_velo_states>(), t * VELO_STATE_BYTES,
+                          hipMemcpyDeviceToHost, stream));
+}

@@ -154,19 +154,26 @@ private:
 namespace Sch {
 
   // Runs every algorithm of the sequence: set_arguments_size, setup, visit
-  // (reference: SchedulerMachinery.cuh:274-393).
-  template<typename Scheduler, typename Visitor, size_t... Is, typename... SizeArgs, typename... VisitArgs>
+  // (reference: SchedulerMachinery.cuh:274-393).  `hook(i, name)` is called
+  // before step i and once more with i = number of steps at the end.
+  struct NoHook {
+    void operator()(size_t, const char*) const {}
+  };
+
+  template<typename Scheduler, typename Visitor, typename Hook, size_t... Is, typename... SizeArgs, typename... VisitArgs>
   void run_sequence_impl(
-    Scheduler& scheduler, Visitor& visitor, std::index_sequence<Is...>, std::tuple<SizeArgs...> size_args,
+    Scheduler& scheduler, Visitor& visitor, Hook& hook, std::index_sequence<Is...>, std::tuple<SizeArgs...> size_args,
     std::tuple<VisitArgs...> visit_args)
   {
+    using seq_t = std::remove_reference_t<decltype(scheduler.sequence_tuple)>;
     auto step = [&](auto idx) {
       constexpr size_t I = decltype(idx)::value;
-      using T = std::tuple_element_t<I, typename std::remove_reference_t<decltype(scheduler.sequence_tuple)>>;
+      using T = std::tuple_element_t<I, seq_t>;
       std::apply(
         [&](auto&&... a) { visitor.template set_arguments_size<T>(scheduler.template arguments<T>(), a...); },
         size_args);
       scheduler.template setup<I, T>();
+      hook(I, T::name);
       std::apply(
         [&](auto&&... a) {
           visitor.template visit<T>(std::get<I>(scheduler.sequence_tuple), scheduler.template arguments<T>(), a...);
@@ -174,15 +181,17 @@ namespace Sch {
         visit_args);
     };
     (step(std::integral_constant<size_t, Is> {}), ...);
+    hook(sizeof...(Is), "end");
   }
 
-  template<typename Scheduler, typename Visitor, typename... SizeArgs, typename... VisitArgs>
+  template<typename Scheduler, typename Visitor, typename Hook, typename... SizeArgs, typename... VisitArgs>
   void run_sequence(
-    Scheduler& scheduler, Visitor& visitor, std::tuple<SizeArgs...> size_args, std::tuple<VisitArgs...> visit_args)
+    Scheduler& scheduler, Visitor& visitor, Hook& hook, std::tuple<SizeArgs...> size_args,
+    std::tuple<VisitArgs...> visit_args)
   {
     using seq_t = std::remove_reference_t<decltype(scheduler.sequence_tuple)>;
     run_sequence_impl(
-      scheduler, visitor, std::make_index_sequence<std::tuple_size_v<seq_t>> {}, size_args, visit_args);
+      scheduler, visitor, hook, std::make_index_sequence<std::tuple_size_v<seq_t>> {}, size_args, visit_args);
   }
 
 } // namespace Sch

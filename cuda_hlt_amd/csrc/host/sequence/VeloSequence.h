@@ -1,0 +1,150 @@
+// The algorithms of the VELO sequence and the configured sequence itself
+// (configuration/sequences/Velo.h:5-18 without the global event cut, which
+// needs UT and SciFi banks).  Each ALGORITHM binds a C-ABI launcher of
+// include/velo_hlt.h with the reference's dependency list (plus the status
+// word).
+#pragma once
+
+#include <tuple>
+
+#include "../gear/ArgumentManager.h"
+#include "../gear/Handler.h"
+#include "ArgumentsVelo.h"
+
+ALGORITHM(
+  velo_init_event_list_launch,
+  init_event_list_t,
+  ARGUMENTS(dev_velo_raw_input, dev_velo_raw_input_offsets, dev_number_of_selected_events, dev_event_list))
+
+ALGORITHM(
+  velo_estimate_input_size_launch,
+  velo_estimate_input_size_t,
+  ARGUMENTS(
+    dev_velo_raw_input,
+    dev_velo_raw_input_offsets,
+    dev_estimated_input_size,
+    dev_module_cluster_num,
+    dev_module_candidate_num,
+    dev_cluster_candidates,
+    dev_event_list,
+    dev_event_order,
+    dev_velo_event_status))
+
+PREFIX_SUM_ALGORITHM(prefix_sum_velo_clusters_t, ARGUMENTS(dev_estimated_input_size, dev_cluster_offset))
+
+ALGORITHM(
+  velo_masked_clustering_launch,
+  velo_masked_clustering_t,
+  ARGUMENTS(
+    dev_velo_raw_input,
+    dev_velo_raw_input_offsets,
+    dev_estimated_input_size,
+    dev_module_cluster_num,
+    dev_module_candidate_num,
+    dev_cluster_candidates,
+    dev_velo_cluster_container,
+    dev_event_list,
+    dev_event_order,
+    dev_velo_event_status))
+
+ALGORITHM(
+  velo_calculate_phi_and_sort_launch,
+  velo_calculate_phi_and_sort_t,
+  ARGUMENTS(dev_estimated_input_size, dev_module_cluster_num, dev_velo_cluster_container, dev_hit_permutation))
+
+ALGORITHM(
+  velo_fill_candidates_launch,
+  velo_fill_candidates_t,
+  ARGUMENTS(
+    dev_velo_cluster_container,
+    dev_estimated_input_size,
+    dev_module_cluster_num,
+    dev_h0_candidates,
+    dev_h2_candidates))
+
+ALGORITHM(
+  velo_search_by_triplet_launch,
+  velo_search_by_triplet_t,
+  ARGUMENTS(
+    dev_velo_cluster_container,
+    dev_estimated_input_size,
+    dev_module_cluster_num,
+    dev_tracks,
+    dev_tracklets,
+    dev_tracks_to_follow,
+    dev_weak_tracks,
+    dev_hit_used,
+    dev_atomics_velo,
+    dev_h0_candidates,
+    dev_h2_candidates,
+    dev_rel_indices,
+    dev_velo_event_status))
+
+ALGORITHM(
+  velo_weak_tracks_adder_launch,
+  velo_weak_tracks_adder_t,
+  ARGUMENTS(
+    dev_velo_cluster_container,
+    dev_estimated_input_size,
+    dev_tracks,
+    dev_weak_tracks,
+    dev_hit_used,
+    dev_atomics_velo,
+    dev_velo_event_status))
+
+ALGORITHM(
+  velo_copy_and_prefix_sum_single_block_launch,
+  copy_and_prefix_sum_single_block_velo_t,
+  ARGUMENTS(dev_atomics_velo))
+
+ALGORITHM(
+  velo_copy_velo_track_hit_number_launch,
+  copy_velo_track_hit_number_t,
+  ARGUMENTS(dev_tracks, dev_atomics_velo, dev_velo_track_hit_number))
+
+PREFIX_SUM_ALGORITHM(
+  prefix_sum_velo_track_hit_number_t,
+  ARGUMENTS(dev_velo_track_hit_number, dev_prefix_sum_auxiliary_array_2))
+
+ALGORITHM(
+  velo_consolidate_velo_tracks_launch,
+  consolidate_velo_tracks_t,
+  ARGUMENTS(
+    dev_atomics_velo,
+    dev_tracks,
+    dev_velo_track_hit_number,
+    dev_velo_cluster_container,
+    dev_estimated_input_size,
+    dev_module_cluster_num,
+    dev_velo_track_hits,
+    dev_velo_states,
+    dev_accepted_velo_tracks))
+
+#define SEQUENCE_T(...) typedef std::tuple<__VA_ARGS__> configured_sequence_t;
+
+SEQUENCE_T(
+  init_event_list_t,
+  velo_estimate_input_size_t,
+  prefix_sum_velo_clusters_t,
+  velo_masked_clustering_t,
+  velo_calculate_phi_and_sort_t,
+  velo_fill_candidates_t,
+  velo_search_by_triplet_t,
+  velo_weak_tracks_adder_t,
+  copy_and_prefix_sum_single_block_velo_t,
+  copy_velo_track_hit_number_t,
+  prefix_sum_velo_track_hit_number_t,
+  consolidate_velo_tracks_t)
+
+// Kept allocated from first use to the end of the sequence
+// (OutputArguments.cuh:16-34, VELO part).
+typedef std::tuple<
+  dev_atomics_velo,
+  dev_velo_track_hit_number,
+  dev_velo_track_hits,
+  dev_velo_states,
+  dev_accepted_velo_tracks,
+  dev_event_list,
+  dev_number_of_selected_events,
+  dev_velo_event_status>
+  output_arguments_t;

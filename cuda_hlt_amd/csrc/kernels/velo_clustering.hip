@@ -296,13 +296,15 @@ __global__ void estimate_input_size_kernel(velo_estimate_input_size_args a)
     if (total_cands > VELO_MAX_CANDIDATES_EVENT) status |= VELO_STATUS_CANDIDATES_OVERFLOW;
   }
   if (a.dev_velo_event_status) {
-    // OR the per-lane status bits of the whole workgroup into the event word
-    const uint64_t any = ballot(status != 0);
-    if (any) {
-      uint32_t s = status;
-      for (int o = 32; o > 0; o >>= 1) s |= __shfl_xor(s, o, 64);
-      if (lane == 0 && s) atomicOr(a.dev_velo_event_status + e, s);
-    }
+    // first writer of the event's status word: store (no memset needed)
+    __shared__ uint32_t block_status;
+    if (threadIdx.x == 0) block_status = 0;
+    __syncthreads();
+    uint32_t s = status;
+    for (int o = 32; o > 0; o >>= 1) s |= __shfl_xor(s, o, 64);
+    if (lane == 0 && s) atomicOr(&block_status, s);
+    __syncthreads();
+    if (threadIdx.x == 0) a.dev_velo_event_status[e] = block_status;
   }
 }
 

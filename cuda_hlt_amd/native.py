@@ -146,6 +146,22 @@ def launch(name: str, args, stream=None) -> None:
     check(getattr(lib(), name)(C.byref(args), stream), name)
 
 
+_hip = None
+
+
+def d2h(device_ptr: int, nbytes: int):
+    """Copy nbytes from a device pointer to a new numpy uint8 array."""
+    import numpy as np
+    global _hip
+    if _hip is None:
+        _hip = C.CDLL("libamdhip64.so")
+        _hip.hipMemcpy.argtypes = [P, P, C.c_size_t, C.c_int]
+        _hip.hipMemcpy.restype = C.c_int
+    out = np.empty(max(int(nbytes), 1), np.uint8)
+    check(_hip.hipMemcpy(out.ctypes.data, device_ptr, int(nbytes), 2), "hipMemcpy")  # 2 = DeviceToHost
+    return out[: int(nbytes)]
+
+
 def prefix_sum_aux_size(n: int) -> int:
     return int(lib().velo_prefix_sum_aux_size(n))
 

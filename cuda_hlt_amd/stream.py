@@ -1,0 +1,147 @@
+"""ctypes binding of include/velo_sequence.h: the C++ VELO sequence
+(Stream::run_sequence over the configured Velo sequence)."""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import native as nat
+
+P = C.c_void_p
+
+
+class SequenceConfig(C.Structure):
+    _fields_ = [("max_events", C.c_uint32), ("reserve_mb", C.c_size_t), ("print_memory_manager", C.c_int)]
+
+
+class SequenceInput(C.Structure):
+    _fields_ = [("host_events", P), ("host_offsets", P), ("events_size", C.c_size_t), ("dev_events", P),
+                ("dev_offsets", P), ("n_events", C.c_uint32), ("repetitions", C.c_uint32),
+                ("copy_outputs_to_host", C.c_int), ("tuning", nat.Tuning * 6)]
+
+
+class SequenceOutput(C.Structure):
+    _fields_ = [("n_events", C.c_uint32), ("total_estimated_hits", C.c_uint32), ("total_tracks", C.c_uint32),
+                ("total_track_hits", C.c_uint32), ("host_atomics_velo", P), ("host_track_hit_number", P),
+                ("host_track_hits", P), ("host_states", P), ("dev_atomics_velo", P), ("dev_track_hit_number", P),
+                ("dev_track_hits", P), ("dev_states", P), ("dev_accepted", P), ("dev_event_status", P),
+                ("dev_module_cluster_start", P), ("dev_module_cluster_num", P), ("dev_cluster_container", P),
+                ("memory_required", C.c_size_t)]
+
+
+TUNING_SLOTS = ("estimate_input_size", "masked_clustering", "calculate_phi_and_sort", "fill_candidates",
+                "search_by_triplet", "consolidate")
+
+_bound = False
+
+
+def _lib():
+    global _bound
+    L = nat.lib()
+    if not _bound:
+        L.velo_sequence_create.argtypes = [C.POINTER(SequenceConfig), C.c_char_p, C.c_size_t, P, C.c_char_p, C.c_size_t]
+        L.velo_sequence_create.restype = P
+        L.velo_sequence_destroy.argtypes = [P]
+        L.velo_sequence_run.argtypes = [P, C.POINTER(SequenceInput), C.c_char_p, C.c_size_t]
+        L.velo_sequence_run.restype = C.c_int
+        L.velo_sequence_output_get.argtypes = [P, C.POINTER(SequenceOutput)]
+        L.velo_sequence_output_get.restype = C.c_int
+        L.velo_sequence_stream.argtypes = [P]
+        L.velo_sequence_stream.restype = P
+        L.velo_sequence_enable_step_timing.argtypes = [P, C.c_int]
+        L.velo_sequence_enable_step_timing.restype = C.c_int
+        L.velo_sequence_step_times.argtypes = [P, C.POINTER(C.c_float), C.POINTER(C.c_char_p), C.c_int]
+        L.velo_sequence_step_times.restype = C.c_int
+        _bound = True
+    return L
+
+
+@dataclass
+class HostTracks:
+    """Consolidated output copied to the host (reference layouts)."""
+    n_events: int
+    atomics: np.ndarray       # uint32[2N+1]
+    hit_number: np.ndarray    # uint32[T+1]
+    track_hits: np.ndarray    # uint8[16 Ht]
+    states: np.ndarray        # uint8[49 T]
+
+
+class VeloSequence:
+    """One C++ Stream of the VELO sequence with its own device arena."""
+
+    def __init__(self, geometry: bytes, module_zs=None, max_events=10000, reserve_mb=4096, print_memory=False):
+        self._zs = None if module_zs is None else np.ascontiguousarray(module_zs, np.float32)
+        err = C.create_string_buffer(512)
+        cfg = SequenceConfig(max_events, reserve_mb, int(print_memory))
+        self.h = _lib().velo_sequence_create(C.byref(cfg), geometry, len(geometry),
+                                             None if self._zs is None else self._zs.ctypes.data, err, 512)
+        if not self.h:
+            raise nat.VeloError(f"velo_sequence_create: {err.value.decode()}")
+        self.max_events = max_events
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib().velo_sequence_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def stream(self) -> int:
+        return _lib().velo_sequence_stream(self.h)
+
+    def run(self, n_events, host_events=None, host_offsets=None, dev_events=None, dev_offsets=None,
+            copy_outputs=True, repetitions=1, tuning=None) -> SequenceOutput:
+        inp = SequenceInput()
+        if dev_events is not None:
+            inp.dev_events, inp.dev_offsets = int(dev_events), int(dev_offsets)
+            inp.events_size = 0
+        else:
+            self._keep = (np.ascontiguousarray(host_events), np.ascontiguousarray(host_offsets, np.uint32))
+            inp.host_events = self._keep[0].ctypes.data
+            inp.host_offsets = self._keep[1].ctypes.data
+            inp.events_size = self._keep[0].nbytes
+        inp.n_events = n_events
+        inp.repetitions = repetitions
+        inp.copy_outputs_to_host = int(copy_outputs)
+        for k, (b, t) in (tuning or {}).items():
+            inp.tuning[TUNING_SLOTS.index(k)] = nat.Tuning(b, t)
+        err = C.create_string_buffer(1024)
+        rc = _lib().velo_sequence_run(self.h, C.byref(inp), err, 1024)
+        if rc != 0:
+            raise nat.VeloError(f"velo_sequence_run: {err.value.decode()}")
+        return self.output()
+
+    def enable_step_timing(self, enable=True):
+        nat.check(_lib().velo_sequence_enable_step_timing(self.h, int(enable)), "enable_step_timing")
+
+    def step_times(self):
+        """[(step name, ms)] of the last run (step timing must be enabled)."""
+        ms = (C.c_float * 64)()
+        names = (C.c_char_p * 64)()
+        n = _lib().velo_sequence_step_times(self.h, ms, names, 64)
+        return [(names[i].decode(), float(ms[i])) for i in range(max(n, 0))]
+
+    def output(self) -> SequenceOutput:
+        out = SequenceOutput()
+        nat.check(_lib().velo_sequence_output_get(self.h, C.byref(out)), "velo_sequence_output_get")
+        return out
+
+    def host_tracks(self) -> HostTracks:
+        o = self.output()
+        if not o.host_atomics_velo:
+            raise nat.VeloError("outputs were not copied to the host")
+        n, T, Ht = o.n_events, o.total_tracks, o.total_track_hits
+
+        def arr(ptr, count, dtype):
+            return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(C.c_uint8)), (max(count, 1) * np.dtype(dtype).itemsize,)
+                                         ).view(dtype)[:count].copy()
+        return HostTracks(n, arr(o.host_atomics_velo, 2 * n + 1, np.uint32),
+                          arr(o.host_track_hit_number, T + 1, np.uint32),
+                          arr(o.host_track_hits, 16 * Ht, np.uint8), arr(o.host_states, 49 * T, np.uint8))

@@ -1,0 +1,86 @@
+/*
+ * velo_sequence.h -- C ABI of the VELO sequence (host side).
+ *
+ * Wraps one Stream of the C++ sequence layer: Stream::initialize and
+ * Stream::run_sequence (Stream.cu:16-83) over the configured Velo sequence
+ * (configuration/sequences/Velo.h:5-18 without the global event cut), with
+ * Constants::reserve_and_initialize (Constants.cu:123-154) done at creation.
+ * Errors are reported as a non-zero return and a message; no exception
+ * crosses the ABI.
+ */
+#ifndef VELO_SEQUENCE_H
+#define VELO_SEQUENCE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "velo_hlt.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct velo_sequence velo_sequence;
+
+typedef struct {
+  uint32_t max_events;      /* events per run (host buffers are sized for it) */
+  size_t reserve_mb;        /* device arena, MiB (the reference's -m) */
+  int print_memory_manager; /* print the arena after every step (-p) */
+} velo_sequence_config;
+
+typedef struct {
+  /* host input (copied to the device by init_event_list) ... */
+  const char* host_events;
+  const uint32_t* host_offsets; /* n_events+1 byte offsets */
+  size_t events_size;           /* bytes */
+  /* ... or device-resident input (used in place when non-NULL) */
+  const char* dev_events;
+  const uint32_t* dev_offsets;
+  uint32_t n_events;
+  uint32_t repetitions;        /* 0 = 1 */
+  int copy_outputs_to_host;    /* copy the consolidated tracks back */
+  velo_tuning tuning[6];       /* estimate, clustering, sort, fill, search, consolidate */
+} velo_sequence_input;
+
+typedef struct {
+  uint32_t n_events;
+  uint32_t total_estimated_hits;
+  uint32_t total_tracks;
+  uint32_t total_track_hits;
+  /* host copies (valid when copy_outputs_to_host was set) */
+  const uint32_t* host_atomics_velo;       /* 2N+1 */
+  const uint32_t* host_track_hit_number;   /* T+1 */
+  const char* host_track_hits;             /* 16*Ht: SoA x, y, z, LHCbID */
+  const char* host_states;                 /* 49*T: SoA 12 floats + bool */
+  /* device outputs (valid until the next run) */
+  const int32_t* dev_atomics_velo;
+  const uint32_t* dev_track_hit_number;
+  const char* dev_track_hits;
+  const char* dev_states;
+  const uint8_t* dev_accepted;
+  const uint32_t* dev_event_status;
+  const uint32_t* dev_module_cluster_start;
+  const uint32_t* dev_module_cluster_num;
+  const uint32_t* dev_cluster_container;
+  size_t memory_required; /* arena high-water mark, bytes */
+} velo_sequence_output;
+
+velo_sequence* velo_sequence_create(
+  const velo_sequence_config* cfg, const void* geometry, size_t geometry_bytes, const float* module_zs, char* err,
+  size_t err_len);
+void velo_sequence_destroy(velo_sequence* s);
+int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, char* err, size_t err_len);
+int velo_sequence_output_get(const velo_sequence* s, velo_sequence_output* out);
+void* velo_sequence_stream(const velo_sequence* s); /* the hipStream_t of the sequence */
+
+/* Per-step timing (no reference counterpart): when enabled, HIP events are
+ * recorded on the sequence's stream before every step; after a run,
+ * velo_sequence_step_times writes up to max_steps durations in ms and the
+ * step names, and returns the number of steps. */
+int velo_sequence_enable_step_timing(velo_sequence* s, int enable);
+int velo_sequence_step_times(const velo_sequence* s, float* ms, const char** names, int max_steps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -214,6 +214,28 @@ class PipelineResult:
     states: np.ndarray           # SoA state bytes
 
 
+class _CPipeline(C.Structure):
+    _fields_ = [("n_events", C.c_uint32), ("total_estimated_hits", C.c_uint32), ("total_tracks", C.c_uint32),
+                ("total_track_hits", C.c_uint32)] + [(f, C.c_void_p) for f in (
+                    "module_cluster_start", "module_cluster_num", "cluster_container", "atomics_velo",
+                    "velo_track_hit_number", "velo_track_hits", "velo_states", "status")]
+
+
+def pipeline_c(events: np.ndarray, offsets: np.ndarray, geometry: bytes, module_zs: np.ndarray):
+    """Whole oracle pipeline in one C call (the CPU baseline); returns
+    (total tracks, total hits on tracks)."""
+    L = lib()
+    L.oracle_pipeline.argtypes = [_P, _P, C.c_uint32, C.c_char_p, _P, C.POINTER(_CPipeline)]
+    L.oracle_pipeline.restype = C.c_int
+    L.oracle_pipeline_free.argtypes = [C.POINTER(_CPipeline)]
+    r = _CPipeline()
+    zs = np.ascontiguousarray(module_zs, np.float32)
+    L.oracle_pipeline(_ptr(events), _ptr(offsets), len(offsets) - 1, geometry, _ptr(zs), C.byref(r))
+    out = (int(r.total_tracks), int(r.total_track_hits))
+    L.oracle_pipeline_free(C.byref(r))
+    return out
+
+
 def pipeline(events: np.ndarray, offsets: np.ndarray, geometry: bytes, module_zs: np.ndarray) -> PipelineResult:
     n = len(offsets) - 1
     est, _, _, _, st0 = estimate_input_size(events, offsets)
