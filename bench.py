@@ -77,6 +77,7 @@ def main():
     ap.add_argument("--events-per-launch", type=int, default=10000)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tuning", default=None, help="launch shapes from cuda_hlt_amd.autotune (JSON)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -105,10 +106,15 @@ def main():
 
     seq = VeloSequence(geometry, zs, max_events=B, reserve_mb=int(os.environ.get("VELO_RESERVE_MB", "8192")))
     seq.enable_step_timing(True)
+    tuning = None
+    if args.tuning:
+        from cuda_hlt_amd import autotune
+        tuning = autotune.load(args.tuning)
 
     def run(b):
         base, off = batch_offsets[b]
-        return seq.run(B, dev_events=raw.data_ptr() + base, dev_offsets=off.data_ptr(), copy_outputs=False)
+        return seq.run(B, dev_events=raw.data_ptr() + base, dev_offsets=off.data_ptr(), copy_outputs=False,
+                       tuning=tuning)
 
     for w in range(args.warmup):
         run(args.steps + w)
@@ -154,7 +160,7 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"plain VELO sequence, minbias nu=7.6, {B}-event launches, "
                                f"{args.steps * B} distinct events timed (seed {args.seed})",
-                   "events_per_launch": B, "path": "plain"},
+                   "events_per_launch": B, "path": "plain", "tuning": tuning},
         "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 3),
                        "p99": round(1e3 * float(np.percentile(lat, 99)), 3)},
         "roofline": roofline,

@@ -87,40 +87,49 @@ __host__ __device__ inline float phi_even(float x, float y)
   return phi + shift;
 }
 
-// VeloTools.cuh:6-26 (array_size 0 still inspects array[0]).
-__device__ inline int bs_first_candidate(const float* a, uint32_t size, float value, float margin)
+// VeloTools.cuh:6-26 (array_size 0 still inspects array[0]).  `a(i)`
+// returns element i (a pointer or an LDS/global accessor).
+template<typename A>
+__device__ inline int bs_first_candidate(const A& a, uint32_t size, float value, float margin)
 {
   bool found = false;
   int l = 0;
   int r = (int) (size - 1u);
   while (l < r) {
     const int m = (l + r) / 2;
-    const float el = a[m];
+    const float el = a(m);
     const float d = value - el;
     found |= fabsf(d) < margin;
     const float lo = value - margin;
     if (lo > el) l = m + 1;
     else r = m;
   }
-  const float d = value - a[l];
+  const float d = value - a(l);
   found |= fabsf(d) < margin;
   return found ? l : -1;
 }
 
 // VeloTools.cuh:28-45
-__device__ inline int bs_second_candidate(const float* a, uint32_t size, float value, float margin)
+template<typename A>
+__device__ inline int bs_second_candidate(const A& a, uint32_t size, float value, float margin)
 {
   int l = 0;
   int r = (int) (size - 1u);
   while (l < r) {
     const int m = (l + r) / 2;
     const float hi = value + margin;
-    if (hi > a[m]) l = m + 1;
+    if (hi > a(m)) l = m + 1;
     else r = m;
   }
-  const float d = value - a[l];
+  const float d = value - a(l);
   return fabsf(d) < margin ? l + 1 : l;
 }
+
+// Element accessor over a plain array, offset by `base`.
+struct ArrayAt {
+  const float* p;
+  __device__ float operator()(int i) const { return p[i]; }
+};
 
 // ---------------------------------------------------------------------------
 // raw format (ClusteringDefinitions.cu:3-21)

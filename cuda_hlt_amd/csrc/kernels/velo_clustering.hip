@@ -1,13 +1,13 @@
 // Raw SuperPixel decoding: init_event_list, estimate_input_size and masked
 // clustering for gfx950, plus the device constants.
 //
-// Work layout: one workgroup per event; wave w of the workgroup owns modules
-// w, w+W, w+2W, ... (W = waves per workgroup).  A wave walks the four banks
-// of its module, stages each bank's SP words in its own LDS slice (coalesced
-// load), and gives one SP to each lane.  Each lane resolves its SP's
-// neighbourhood by scanning the staged bank; output slots come from a
-// ballot-based wave prefix sum, so clusters are written in bank, SP,
-// cluster order with no atomics and no cross-wave synchronisation.
+// Work layout: one workgroup per event.  The workgroup first copies the raw
+// event into LDS (coalesced); wave w then owns modules w, w+W, w+2W, ...
+// (W = waves per workgroup) and takes the SPs of a module's four banks as
+// one list, one SP per lane.  Each lane resolves its SP's neighbourhood by
+// scanning its bank in LDS; output slots come from a ballot-based wave
+// prefix sum, so clusters are written in bank, SP, cluster order with no
+// atomics and no cross-wave synchronisation.
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -17,7 +17,6 @@
 namespace velo {
 
 constexpr uint32_t kDefaultBlock = 256;
-constexpr uint32_t kDefaultBankTile = 512; // SP words staged per wave
 
 // Constants.cu:36
 __constant__ uint8_t c_candidate_ks[9] = {0, 0, 1, 4, 4, 5, 5, 5, 5};
@@ -188,57 +187,127 @@ __global__ void init_event_list_kernel(uint32_t* event_list, uint32_t n)
 }
 
 // ---------------------------------------------------------------------------
-// K1: estimate_input_size
+// Event staging and module iteration (K1 and K5)
 // ---------------------------------------------------------------------------
-// Stages bank b's words in this wave's LDS slice when they fit; returns the
-// pointer the lanes should scan.
-__device__ inline const uint32_t* stage_bank(const RawBank& bank, uint32_t* slice, uint32_t tile)
+// The raw event is copied word by word into LDS by the whole workgroup when
+// it fits (tuning.lds_tile words); otherwise the same code reads it from
+// global memory.  Each wave owns whole modules; the SPs of a module's four
+// banks are taken as one list in bank order, one SP per lane.
+constexpr uint32_t kDefaultEventWords = 6144; // 24 KB of LDS per workgroup
+
+__device__ inline uint32_t sel4(uint32_t k, uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3)
 {
-  if (bank.sp_count > tile) return bank.sp_word;
-  for (uint32_t i = lane_id(); i < bank.sp_count; i += kWave) slice[i] = bank.sp_word[i];
-  wave_sync();
-  return slice;
+  return k == 0 ? a0 : (k == 1 ? a1 : (k == 2 ? a2 : a3));
 }
 
-__global__ void estimate_input_size_kernel(velo_estimate_input_size_args a)
+struct ModuleBanks {
+  uint32_t b0, b1, b2, b3; // word index of sp_word[0] of banks 4m..4m+3
+  uint32_t n0, n1, n2, n3; // SP counts (0 for a missing or mislabelled bank)
+  uint32_t p1, p2, p3, total;
+  uint32_t status;
+};
+
+template<typename P>
+__device__ inline void bank_of(P ev, uint32_t n_banks, uint32_t b, uint32_t* base, uint32_t* n, uint32_t* status)
 {
-  extern __shared__ uint32_t lds[];
-  __shared__ uint32_t module_cands[VELO_N_MODULES];
-  __shared__ uint32_t module_cand_offset[VELO_N_MODULES + 1];
-  const uint32_t e = blockIdx.x;
+  *base = 0;
+  *n = 0;
+  if (b >= n_banks || b >= VELO_N_SENSORS) return;
+  const uint32_t h = n_banks + 2 + ev[1 + b] / 4u;
+  const uint32_t sensor = ev[h];
+  const uint32_t cnt = ev[h + 1];
+  if (sensor != b) {
+    *status |= VELO_STATUS_BAD_BANK;
+    return;
+  }
+  if (cnt > 256u) *status |= VELO_STATUS_SP_INDEX_TRUNCATED;
+  *base = h + 2;
+  *n = cnt;
+}
+
+template<typename P>
+__device__ inline ModuleBanks module_banks(P ev, uint32_t n_banks, uint32_t m)
+{
+  ModuleBanks mb;
+  mb.status = 0;
+  bank_of(ev, n_banks, 4 * m + 0, &mb.b0, &mb.n0, &mb.status);
+  bank_of(ev, n_banks, 4 * m + 1, &mb.b1, &mb.n1, &mb.status);
+  bank_of(ev, n_banks, 4 * m + 2, &mb.b2, &mb.n2, &mb.status);
+  bank_of(ev, n_banks, 4 * m + 3, &mb.b3, &mb.n3, &mb.status);
+  mb.p1 = mb.n0;
+  mb.p2 = mb.p1 + mb.n1;
+  mb.p3 = mb.p2 + mb.n2;
+  mb.total = mb.p3 + mb.n3;
+  return mb;
+}
+
+// Position j of the module's SP list -> (bank k, index in bank).
+__device__ inline uint32_t bank_slot(const ModuleBanks& mb, uint32_t j, uint32_t* k)
+{
+  *k = (j >= mb.p1 ? 1u : 0u) + (j >= mb.p2 ? 1u : 0u) + (j >= mb.p3 ? 1u : 0u);
+  return j - sel4(*k, 0u, mb.p1, mb.p2, mb.p3);
+}
+
+// Copies n words to LDS with the whole workgroup, 4 loads in flight per lane.
+__device__ inline void stage_words(uint32_t* dst, const uint32_t* src, uint32_t n)
+{
+  const uint32_t step = 4 * blockDim.x;
+  for (uint32_t base = threadIdx.x; base < n; base += step) {
+    uint32_t v[4];
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+      const uint32_t i = base + k * blockDim.x;
+      v[k] = i < n ? src[i] : 0u;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+      const uint32_t i = base + k * blockDim.x;
+      if (i < n) dst[i] = v[k];
+    }
+  }
+}
+
+__device__ inline uint32_t block_or(uint32_t v, uint32_t* shared_word)
+{
+  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
+  if (lane_id() == 0 && v) atomicOr(shared_word, v);
+  __syncthreads();
+  return *shared_word;
+}
+
+// ---------------------------------------------------------------------------
+// K1: estimate_input_size
+// ---------------------------------------------------------------------------
+template<typename P>
+__device__ __forceinline__ void estimate_event(
+  P ev, const velo_estimate_input_size_args& a, uint32_t e, uint32_t* module_cands, uint32_t* module_cand_offset,
+  uint32_t* block_status)
+{
   const uint32_t waves = blockDim.x / kWave;
   const uint32_t wave = threadIdx.x / kWave;
   const uint32_t lane = lane_id();
-  const uint32_t tile = a.tuning.lds_tile;
-  uint32_t* slice = lds + wave * tile;
-  const RawEvent ev(a.dev_raw_input + a.dev_raw_input_offsets[event_index(a.dev_event_list, e)]);
-  const uint32_t n_banks = ev.n_banks < VELO_N_SENSORS ? ev.n_banks : VELO_N_SENSORS;
+  const uint32_t n_banks = ev[0];
   uint32_t status = 0;
 
   // pass 1: estimates and candidate counts per module
   for (uint32_t m = wave; m < VELO_N_MODULES; m += waves) {
+    const ModuleBanks mb = module_banks(ev, n_banks, m);
+    status |= mb.status;
     uint32_t est = 0, ncand = 0;
-    for (uint32_t b = 4 * m; b < 4 * m + 4 && b < n_banks; ++b) {
-      const RawBank bank(ev, b);
-      if (bank.sensor_index != b) {
-        status |= VELO_STATUS_BAD_BANK;
-        continue;
+    for (uint32_t j = lane; j < mb.total; j += kWave) {
+      uint32_t k;
+      const uint32_t i = bank_slot(mb, j, &k);
+      const uint32_t base = sel4(k, mb.b0, mb.b1, mb.b2, mb.b3);
+      const uint32_t w = ev[base + i];
+      if (w & 0x80000000u) {
+        est += isolated_cluster_count(w & 0xFFu);
       }
-      if (bank.sp_count > 256u) status |= VELO_STATUS_SP_INDEX_TRUNCATED;
-      const uint32_t* words = stage_bank(bank, slice, tile);
-      for (uint32_t i = lane; i < bank.sp_count; i += kWave) {
-        const uint32_t w = words[i];
-        if (w & 0x80000000u) {
-          est += isolated_cluster_count(w & 0xFFu);
-        }
-        else {
-          const uint32_t c = candidates_from_pixels(neighbour_pixels(w, words, bank.sp_count));
-          const uint32_t f = ((c & 0x0Fu) ? 1u : 0u) + ((c & 0xF0u) ? 1u : 0u);
-          est += f;
-          ncand += f;
-        }
+      else {
+        const uint32_t c = candidates_from_pixels(neighbour_pixels(w, ev + base, sel4(k, mb.n0, mb.n1, mb.n2, mb.n3)));
+        const uint32_t f = ((c & 0x0Fu) ? 1u : 0u) + ((c & 0xF0u) ? 1u : 0u);
+        est += f;
+        ncand += f;
       }
-      wave_sync();
     }
     est = wave_sum(est);
     ncand = wave_sum(ncand);
@@ -250,7 +319,6 @@ __global__ void estimate_input_size_kernel(velo_estimate_input_size_args a)
   }
   __syncthreads();
   if (threadIdx.x < kWave) {
-    // exclusive scan of the 52 module candidate counts
     const uint32_t v = lane < VELO_N_MODULES ? module_cands[lane] : 0u;
     const uint32_t inc = wave_inclusive_scan(v);
     if (lane < VELO_N_MODULES) module_cand_offset[lane] = inc - v;
@@ -261,33 +329,31 @@ __global__ void estimate_input_size_kernel(velo_estimate_input_size_args a)
   // pass 2: candidates in bank, SP, low-then-high-nibble order
   uint32_t* cands = a.dev_cluster_candidates + (size_t) e * VELO_MAX_CANDIDATES_EVENT;
   for (uint32_t m = wave; m < VELO_N_MODULES; m += waves) {
-    uint32_t pos = module_cand_offset[m];
     if (module_cands[m] == 0) continue;
-    for (uint32_t b = 4 * m; b < 4 * m + 4 && b < n_banks; ++b) {
-      const RawBank bank(ev, b);
-      if (bank.sensor_index != b) continue;
-      const uint32_t* words = stage_bank(bank, slice, tile);
-      for (uint32_t base = 0; base < bank.sp_count; base += kWave) {
-        const uint32_t i = base + lane;
-        uint32_t c = 0;
-        if (i < bank.sp_count) {
-          const uint32_t w = words[i];
-          if (!(w & 0x80000000u)) c = candidates_from_pixels(neighbour_pixels(w, words, bank.sp_count));
-        }
-        const uint32_t cnt = ((c & 0x0Fu) ? 1u : 0u) + ((c & 0xF0u) ? 1u : 0u);
-        uint32_t total;
-        const uint32_t pre = wave_prefix_small(cnt, &total);
-        uint32_t slot = pos + pre;
-        if (c & 0x0Fu) {
-          if (slot < VELO_MAX_CANDIDATES_EVENT) cands[slot] = (i << 11) | (b << 3) | c_candidate_ks[c & 0x0Fu];
-          ++slot;
-        }
-        if (c & 0xF0u) {
-          if (slot < VELO_MAX_CANDIDATES_EVENT) cands[slot] = (i << 11) | (b << 3) | (c_candidate_ks[c >> 4] + 2u);
-        }
-        pos += total;
+    const ModuleBanks mb = module_banks(ev, n_banks, m);
+    uint32_t pos = module_cand_offset[m];
+    for (uint32_t jb = 0; jb < mb.total; jb += kWave) {
+      const uint32_t j = jb + lane;
+      uint32_t c = 0, i = 0, k = 0;
+      if (j < mb.total) {
+        i = bank_slot(mb, j, &k);
+        const uint32_t base = sel4(k, mb.b0, mb.b1, mb.b2, mb.b3);
+        const uint32_t w = ev[base + i];
+        if (!(w & 0x80000000u))
+          c = candidates_from_pixels(neighbour_pixels(w, ev + base, sel4(k, mb.n0, mb.n1, mb.n2, mb.n3)));
       }
-      wave_sync();
+      const uint32_t cnt = ((c & 0x0Fu) ? 1u : 0u) + ((c & 0xF0u) ? 1u : 0u);
+      uint32_t total;
+      const uint32_t pre = wave_prefix_small(cnt, &total);
+      uint32_t slot = pos + pre;
+      const uint32_t b = 4 * m + k;
+      if (c & 0x0Fu) {
+        if (slot < VELO_MAX_CANDIDATES_EVENT) cands[slot] = (i << 11) | (b << 3) | c_candidate_ks[c & 0x0Fu];
+        ++slot;
+      }
+      if ((c & 0xF0u) && slot < VELO_MAX_CANDIDATES_EVENT)
+        cands[slot] = (i << 11) | (b << 3) | (c_candidate_ks[c >> 4] + 2u);
+      pos += total;
     }
   }
   const uint32_t total_cands = module_cand_offset[VELO_N_MODULES];
@@ -297,31 +363,44 @@ __global__ void estimate_input_size_kernel(velo_estimate_input_size_args a)
   }
   if (a.dev_velo_event_status) {
     // first writer of the event's status word: store (no memset needed)
-    __shared__ uint32_t block_status;
-    if (threadIdx.x == 0) block_status = 0;
+    const uint32_t s = block_or(status, block_status);
+    if (threadIdx.x == 0) a.dev_velo_event_status[e] = s;
+  }
+}
+
+__global__ void __launch_bounds__(1024) estimate_input_size_kernel(velo_estimate_input_size_args a)
+{
+  extern __shared__ uint32_t lds[];
+  __shared__ uint32_t module_cands[VELO_N_MODULES];
+  __shared__ uint32_t module_cand_offset[VELO_N_MODULES + 1];
+  __shared__ uint32_t block_status;
+  const uint32_t e = blockIdx.x;
+  const uint32_t ei = event_index(a.dev_event_list, e);
+  const uint32_t off = a.dev_raw_input_offsets[ei];
+  const uint32_t nwords = (a.dev_raw_input_offsets[ei + 1] - off) / 4u;
+  const uint32_t* gev = reinterpret_cast<const uint32_t*>(a.dev_raw_input + off);
+  if (threadIdx.x == 0) block_status = 0;
+  if (nwords <= a.tuning.lds_tile) {
+    stage_words(lds, gev, nwords);
     __syncthreads();
-    uint32_t s = status;
-    for (int o = 32; o > 0; o >>= 1) s |= __shfl_xor(s, o, 64);
-    if (lane == 0 && s) atomicOr(&block_status, s);
+    estimate_event(static_cast<const uint32_t*>(lds), a, e, module_cands, module_cand_offset, &block_status);
+  }
+  else {
     __syncthreads();
-    if (threadIdx.x == 0) a.dev_velo_event_status[e] = block_status;
+    estimate_event(gev, a, e, module_cands, module_cand_offset, &block_status);
   }
 }
 
 // ---------------------------------------------------------------------------
 // K5: masked clustering
 // ---------------------------------------------------------------------------
-__global__ void masked_clustering_kernel(velo_masked_clustering_args a)
+template<typename P>
+__device__ __forceinline__ void cluster_event(P ev, const velo_masked_clustering_args& a, uint32_t e, uint32_t* block_words)
 {
-  extern __shared__ uint32_t lds[];
-  const uint32_t e = blockIdx.x;
   const uint32_t waves = blockDim.x / kWave;
   const uint32_t wave = threadIdx.x / kWave;
   const uint32_t lane = lane_id();
-  const uint32_t tile = a.tuning.lds_tile;
-  uint32_t* slice = lds + wave * tile;
-  const RawEvent ev(a.dev_raw_input + a.dev_raw_input_offsets[event_index(a.dev_event_list, e)]);
-  const uint32_t n_banks = ev.n_banks < VELO_N_SENSORS ? ev.n_banks : VELO_N_SENSORS;
+  const uint32_t n_banks = ev[0];
   const uint32_t total = a.dev_module_cluster_start[VELO_N_MODULES * a.n_events];
   float* xs = reinterpret_cast<float*>(a.dev_velo_cluster_container);
   float* ys = xs + total;
@@ -332,93 +411,109 @@ __global__ void masked_clustering_kernel(velo_masked_clustering_args a)
   uint32_t event_hits = 0;
 
   for (uint32_t m = wave; m < VELO_N_MODULES; m += waves) {
+    const ModuleBanks mb = module_banks(ev, n_banks, m);
     const uint32_t mstart = starts[m];
     uint32_t pos = 0;
-    for (uint32_t b = 4 * m; b < 4 * m + 4 && b < n_banks; ++b) {
-      const RawBank bank(ev, b);
-      if (bank.sensor_index != b) continue;
-      const uint32_t* words = stage_bank(bank, slice, tile);
-      const uint32_t n = bank.sp_count;
-      for (uint32_t base = 0; base < n; base += kWave) {
-        const uint32_t i = base + lane;
-        Hit h0 {}, h1 {};
-        uint32_t cnt = 0;
-        if (i < n) {
-          const uint32_t w = words[i];
-          const uint32_t sp_addr = (w & 0x007FFF00u) >> 8;
-          const uint32_t sp = w & 0xFFu;
-          if (w & 0x80000000u) {
-            // isolated SP: pre-computed pattern (MaskedVeloClustering.cu:105-167)
-            const uint32_t idx = a.dev_velo_sp_patterns[sp];
-            const uint32_t sp_row = sp_addr & 0x3Fu, sp_col = sp_addr >> 6;
-            h0 = make_hit(a.dev_velo_geometry, bank.sensor_index, sp_col * 2u + ((idx >> 2) & 1u),
-                          sp_row * 4u + (idx & 3u), a.dev_velo_sp_fx[2 * sp], a.dev_velo_sp_fy[2 * sp]);
-            cnt = 1;
-            if (idx & 8u) {
-              h1 = make_hit(a.dev_velo_geometry, bank.sensor_index, sp_col * 2u + ((idx >> 6) & 1u),
-                            sp_row * 4u + ((idx >> 4) & 3u), a.dev_velo_sp_fx[2 * sp + 1],
-                            a.dev_velo_sp_fy[2 * sp + 1]);
-              cnt = 2;
-            }
+    for (uint32_t jb = 0; jb < mb.total; jb += kWave) {
+      const uint32_t j = jb + lane;
+      Hit h0 {}, h1 {};
+      uint32_t cnt = 0;
+      if (j < mb.total) {
+        uint32_t k;
+        const uint32_t i = bank_slot(mb, j, &k);
+        const uint32_t base = sel4(k, mb.b0, mb.b1, mb.b2, mb.b3);
+        const uint32_t n = sel4(k, mb.n0, mb.n1, mb.n2, mb.n3);
+        const uint32_t sensor = 4 * m + k;
+        const uint32_t w = ev[base + i];
+        const uint32_t sp_addr = (w & 0x007FFF00u) >> 8;
+        const uint32_t sp = w & 0xFFu;
+        if (w & 0x80000000u) {
+          // isolated SP: pre-computed pattern (MaskedVeloClustering.cu:105-167)
+          const uint32_t idx = a.dev_velo_sp_patterns[sp];
+          const uint32_t sp_row = sp_addr & 0x3Fu, sp_col = sp_addr >> 6;
+          h0 = make_hit(a.dev_velo_geometry, sensor, sp_col * 2u + ((idx >> 2) & 1u), sp_row * 4u + (idx & 3u),
+                        a.dev_velo_sp_fx[2 * sp], a.dev_velo_sp_fy[2 * sp]);
+          cnt = 1;
+          if (idx & 8u) {
+            h1 = make_hit(a.dev_velo_geometry, sensor, sp_col * 2u + ((idx >> 6) & 1u), sp_row * 4u + ((idx >> 4) & 3u),
+                          a.dev_velo_sp_fx[2 * sp + 1], a.dev_velo_sp_fy[2 * sp + 1]);
+            cnt = 2;
           }
-          else {
-            const uint32_t c = candidates_from_pixels(neighbour_pixels(w, words, n));
-            if (c) {
-              // the reference decodes the candidate's SP index as uint8_t (:179)
-              const uint32_t ci = i & 0xFFu;
-              const uint32_t cw = ci == i ? w : words[ci];
-              const uint32_t caddr = (cw & 0x007FFF00u) >> 8;
-              const int32_t sp_row = (int32_t) (caddr & 0x3Fu), sp_col = (int32_t) (caddr >> 6);
-              uint32_t pa[3];
-              load_window(sp_row, sp_col, words, n, pa);
-              Hit t;
-              if ((c & 0x0Fu) && cluster_candidate(pa[0], pa[1], pa[2], sp_row, sp_col, c_candidate_ks[c & 0x0Fu],
-                                                   a.dev_velo_geometry, bank.sensor_index, &t)) {
-                h0 = t;
-                cnt = 1;
-              }
-              if ((c & 0xF0u) && cluster_candidate(pa[0], pa[1], pa[2], sp_row, sp_col,
-                                                   c_candidate_ks[c >> 4] + 2u, a.dev_velo_geometry,
-                                                   bank.sensor_index, &t)) {
-                if (cnt == 0) h0 = t;
-                else h1 = t;
-                ++cnt;
-              }
+        }
+        else {
+          const uint32_t c = candidates_from_pixels(neighbour_pixels(w, ev + base, n));
+          if (c) {
+            // the reference decodes the candidate's SP index as uint8_t (:179)
+            const uint32_t ci = i & 0xFFu;
+            const uint32_t cw = ci == i ? w : ev[base + ci];
+            const uint32_t caddr = (cw & 0x007FFF00u) >> 8;
+            const int32_t sp_row = (int32_t) (caddr & 0x3Fu), sp_col = (int32_t) (caddr >> 6);
+            uint32_t pa[3];
+            load_window(sp_row, sp_col, ev + base, n, pa);
+            Hit t;
+            if ((c & 0x0Fu) && cluster_candidate(pa[0], pa[1], pa[2], sp_row, sp_col, c_candidate_ks[c & 0x0Fu],
+                                                 a.dev_velo_geometry, sensor, &t)) {
+              h0 = t;
+              cnt = 1;
+            }
+            if ((c & 0xF0u) && cluster_candidate(pa[0], pa[1], pa[2], sp_row, sp_col, c_candidate_ks[c >> 4] + 2u,
+                                                 a.dev_velo_geometry, sensor, &t)) {
+              if (cnt == 0) h0 = t;
+              else h1 = t;
+              ++cnt;
             }
           }
         }
-        uint32_t tot;
-        const uint32_t pre = wave_prefix_small(cnt, &tot);
-        if (cnt > 0) {
-          const uint32_t slot = mstart + pos + pre;
-          xs[slot] = h0.x;
-          ys[slot] = h0.y;
-          zs[slot] = h0.z;
-          ids[slot] = h0.id;
-        }
-        if (cnt > 1) {
-          const uint32_t slot = mstart + pos + pre + 1;
-          xs[slot] = h1.x;
-          ys[slot] = h1.y;
-          zs[slot] = h1.z;
-          ids[slot] = h1.id;
-        }
-        pos += tot;
       }
-      wave_sync();
+      uint32_t tot;
+      const uint32_t pre = wave_prefix_small(cnt, &tot);
+      if (cnt > 0) {
+        const uint32_t slot = mstart + pos + pre;
+        xs[slot] = h0.x;
+        ys[slot] = h0.y;
+        zs[slot] = h0.z;
+        ids[slot] = h0.id;
+      }
+      if (cnt > 1) {
+        const uint32_t slot = mstart + pos + pre + 1;
+        xs[slot] = h1.x;
+        ys[slot] = h1.y;
+        zs[slot] = h1.z;
+        ids[slot] = h1.id;
+      }
+      pos += tot;
     }
     if (lane == 0) a.dev_module_cluster_num[e * VELO_N_MODULES + m] = pos;
     if (pos > VELO_MAX_NUMHITS_IN_MODULE) status |= VELO_STATUS_MODULE_HITS_OVERFLOW;
     event_hits += pos;
   }
   if (a.dev_velo_event_status) {
-    __shared__ uint32_t hits_sum;
-    if (threadIdx.x == 0) hits_sum = 0;
+    if (lane == 0) atomicAdd(&block_words[1], event_hits);
     __syncthreads();
-    if (lane == 0) atomicAdd(&hits_sum, event_hits);
+    if (threadIdx.x == 0 && block_words[1] >= VELO_MAX_HITS_PER_EVENT) status |= VELO_STATUS_EVENT_HITS_OVERFLOW;
+    const uint32_t s = block_or(status, &block_words[0]);
+    if (threadIdx.x == 0 && s) atomicOr(a.dev_velo_event_status + e, s);
+  }
+}
+
+__global__ void __launch_bounds__(1024) masked_clustering_kernel(velo_masked_clustering_args a)
+{
+  extern __shared__ uint32_t lds[];
+  __shared__ uint32_t block_words[2];
+  const uint32_t e = blockIdx.x;
+  const uint32_t ei = event_index(a.dev_event_list, e);
+  const uint32_t off = a.dev_raw_input_offsets[ei];
+  const uint32_t nwords = (a.dev_raw_input_offsets[ei + 1] - off) / 4u;
+  const uint32_t* gev = reinterpret_cast<const uint32_t*>(a.dev_raw_input + off);
+  if (threadIdx.x < 2) block_words[threadIdx.x] = 0;
+  if (nwords <= a.tuning.lds_tile) {
+    stage_words(lds, gev, nwords);
     __syncthreads();
-    if (threadIdx.x == 0 && hits_sum >= VELO_MAX_HITS_PER_EVENT) status |= VELO_STATUS_EVENT_HITS_OVERFLOW;
-    if (lane == 0 && status) atomicOr(a.dev_velo_event_status + e, status);
+    cluster_event(static_cast<const uint32_t*>(lds), a, e, block_words);
+  }
+  else {
+    __syncthreads();
+    cluster_event(gev, a, e, block_words);
   }
 }
 
@@ -561,8 +656,8 @@ int velo_estimate_input_size_launch(const velo_estimate_input_size_args* in, voi
   velo_estimate_input_size_args a = *in;
   const uint32_t block = block_of(a.tuning);
   if (!block) return (int) hipErrorInvalidValue;
-  if (!a.tuning.lds_tile) a.tuning.lds_tile = kDefaultBankTile;
-  const size_t shmem = (size_t) (block / kWave) * a.tuning.lds_tile * sizeof(uint32_t);
+  if (!a.tuning.lds_tile) a.tuning.lds_tile = kDefaultEventWords;
+  const size_t shmem = (size_t) a.tuning.lds_tile * sizeof(uint32_t);
   hipLaunchKernelGGL(estimate_input_size_kernel, dim3(a.n_events), dim3(block), shmem, (hipStream_t) stream, a);
   return (int) hipGetLastError();
 }
@@ -577,8 +672,8 @@ int velo_masked_clustering_launch(const velo_masked_clustering_args* in, void* s
   velo_masked_clustering_args a = *in;
   const uint32_t block = block_of(a.tuning);
   if (!block) return (int) hipErrorInvalidValue;
-  if (!a.tuning.lds_tile) a.tuning.lds_tile = kDefaultBankTile;
-  const size_t shmem = (size_t) (block / kWave) * a.tuning.lds_tile * sizeof(uint32_t);
+  if (!a.tuning.lds_tile) a.tuning.lds_tile = kDefaultEventWords;
+  const size_t shmem = (size_t) a.tuning.lds_tile * sizeof(uint32_t);
   hipLaunchKernelGGL(masked_clustering_kernel, dim3(a.n_events), dim3(block), shmem, (hipStream_t) stream, a);
   return (int) hipGetLastError();
 }

@@ -119,20 +119,18 @@ __global__ void __launch_bounds__(1024) calculate_phi_and_sort_kernel(velo_calcu
   }
 }
 
-// fill_candidates: wave w owns modules 2+w, 2+w+W, ... up to 49.
-__global__ void fill_candidates_kernel(velo_fill_candidates_args a)
+// fill_candidates: the event's phi slot is copied to LDS when it fits
+// (tuning.lds_tile floats); wave w owns modules 2+w, 2+w+W, ... up to 49.
+constexpr uint32_t kDefaultPhiTile = 4096;
+
+template<typename A>
+__device__ __forceinline__ void fill_event(
+  const A& phis, const velo_fill_candidates_args& a, const uint32_t* starts, const uint32_t* nums, uint32_t off,
+  int16_t* h0c, int16_t* h2c)
 {
-  const uint32_t e = blockIdx.x;
   const uint32_t waves = blockDim.x / kWave;
   const uint32_t wave = threadIdx.x / kWave;
   const uint32_t lane = lane_id();
-  const uint32_t total = a.dev_module_cluster_start[VELO_N_MODULES * a.n_events];
-  const uint32_t* starts = a.dev_module_cluster_start + e * VELO_N_MODULES;
-  const uint32_t* nums = a.dev_module_cluster_num + e * VELO_N_MODULES;
-  const uint32_t off = starts[0];
-  const float* phis = reinterpret_cast<const float*>(a.dev_velo_cluster_container + 4 * (size_t) total + off);
-  int16_t* h0c = a.dev_h0_candidates + 2 * (size_t) off;
-  int16_t* h2c = a.dev_h2_candidates + 2 * (size_t) off;
   for (uint32_t m = 2 + wave; m < 50; m += waves) {
     const uint32_t n1 = nums[m];
     const uint32_t m0s = starts[m + 2] - off, m0n = nums[m + 2];
@@ -140,20 +138,24 @@ __global__ void fill_candidates_kernel(velo_fill_candidates_args a)
     const uint32_t s1 = starts[m] - off;
     for (uint32_t r = lane; r < n1; r += kWave) {
       const uint32_t h1 = s1 + r;
-      const float phi = phis[h1];
+      const float phi = phis(h1);
       int f0 = -1, l0 = -1, f2 = -1, l2 = -1;
       if (m0n > 0) {
-        f0 = bs_first_candidate(phis + m0s, m0n, phi, kPhiExtrapolation);
+        const auto w0 = [&](int i) { return phis(m0s + i); };
+        f0 = bs_first_candidate(w0, m0n, phi, kPhiExtrapolation);
         if (f0 != -1) {
-          l0 = bs_second_candidate(phis + m0s + f0, m0n - (uint32_t) f0, phi, kPhiExtrapolation);
+          const auto w0f = [&](int i) { return phis(m0s + f0 + i); };
+          l0 = bs_second_candidate(w0f, m0n - (uint32_t) f0, phi, kPhiExtrapolation);
           f0 += (int) m0s;
           l0 = l0 == 0 ? f0 + 1 : f0 + l0;
         }
       }
       if (m2n > 0) {
-        f2 = bs_first_candidate(phis + m2s, m2n, phi, kPhiExtrapolation);
+        const auto w2 = [&](int i) { return phis(m2s + i); };
+        f2 = bs_first_candidate(w2, m2n, phi, kPhiExtrapolation);
         if (f2 != -1) {
-          l2 = bs_second_candidate(phis + m2s + f2, m2n - (uint32_t) f2, phi, kPhiExtrapolation);
+          const auto w2f = [&](int i) { return phis(m2s + f2 + i); };
+          l2 = bs_second_candidate(w2f, m2n - (uint32_t) f2, phi, kPhiExtrapolation);
           f2 += (int) m2s;
           l2 = l2 == 0 ? f2 + 1 : f2 + l2;
         }
@@ -163,6 +165,29 @@ __global__ void fill_candidates_kernel(velo_fill_candidates_args a)
       h2c[2 * h1] = (int16_t) f2;
       h2c[2 * h1 + 1] = (int16_t) l2;
     }
+  }
+}
+
+__global__ void __launch_bounds__(1024) fill_candidates_kernel(velo_fill_candidates_args a)
+{
+  extern __shared__ float s_phi[];
+  const uint32_t e = blockIdx.x;
+  const uint32_t total = a.dev_module_cluster_start[VELO_N_MODULES * a.n_events];
+  const uint32_t* starts = a.dev_module_cluster_start + e * VELO_N_MODULES;
+  const uint32_t* nums = a.dev_module_cluster_num + e * VELO_N_MODULES;
+  const uint32_t off = starts[0];
+  const uint32_t ev_hits = starts[VELO_N_MODULES] - off;
+  const float* phis = reinterpret_cast<const float*>(a.dev_velo_cluster_container + 4 * (size_t) total + off);
+  int16_t* h0c = a.dev_h0_candidates + 2 * (size_t) off;
+  int16_t* h2c = a.dev_h2_candidates + 2 * (size_t) off;
+  if (ev_hits <= a.tuning.lds_tile) {
+    for (uint32_t i = threadIdx.x; i < ev_hits; i += blockDim.x) s_phi[i] = phis[i];
+    __syncthreads();
+    const auto acc = [&](int i) { return s_phi[i]; };
+    fill_event(acc, a, starts, nums, off, h0c, h2c);
+  }
+  else {
+    fill_event(ArrayAt {phis}, a, starts, nums, off, h0c, h2c);
   }
 }
 
@@ -193,7 +218,10 @@ int velo_fill_candidates_launch(const velo_fill_candidates_args* a, void* stream
   if (a->n_events == 0) return 0;
   const uint32_t block = a->tuning.block_dim ? a->tuning.block_dim : 256;
   if (block % kWave || block > 1024) return (int) hipErrorInvalidValue;
-  hipLaunchKernelGGL(fill_candidates_kernel, dim3(a->n_events), dim3(block), 0, (hipStream_t) stream, *a);
+  velo_fill_candidates_args args = *a;
+  if (!args.tuning.lds_tile) args.tuning.lds_tile = kDefaultPhiTile;
+  hipLaunchKernelGGL(fill_candidates_kernel, dim3(args.n_events), dim3(block), args.tuning.lds_tile * sizeof(float),
+                     (hipStream_t) stream, args);
   return (int) hipGetLastError();
 }
 

@@ -115,29 +115,29 @@ __global__ void copy_track_hit_number_kernel(velo_copy_velo_track_hit_number_arg
     a.dev_velo_track_hit_number[acc + i] = *reinterpret_cast<const uint16_t*>(tracks + (size_t) i * VELO_TRACK_HITS_BYTES);
 }
 
-// K15: one workgroup per event, one lane per track.
-__global__ void consolidate_kernel(velo_consolidate_velo_tracks_args a)
+// K15: one workgroup per event, one lane per track.  The event's sorted x,
+// y, z, id are copied to LDS when they fit (tuning.lds_tile hits) so the
+// per-track gathers hit LDS.
+constexpr uint32_t kDefaultConsolidateHits = 2048;
+
+struct SlotView {
+  const float *x, *y, *z;
+  const uint32_t* id;
+};
+
+template<typename V>
+__device__ __forceinline__ void consolidate_event(
+  const V& hv, const velo_consolidate_velo_tracks_args& a, uint32_t e, uint32_t n, uint32_t toff,
+  uint32_t total_tracks, uint32_t total_hits)
 {
-  const uint32_t N = a.n_events;
-  const uint32_t e = blockIdx.x;
-  const uint32_t total_tracks = (uint32_t) a.dev_atomics_velo[2 * N];
-  const uint32_t total_hits = a.dev_velo_track_hit_number[total_tracks];
-  const uint32_t n = (uint32_t) a.dev_atomics_velo[e];
-  const uint32_t toff = (uint32_t) a.dev_atomics_velo[N + e];
-  const uint32_t total = a.dev_module_cluster_start[VELO_N_MODULES * N];
-  const uint32_t off = a.dev_module_cluster_start[e * VELO_N_MODULES];
-  const float* Ys = reinterpret_cast<const float*>(a.dev_velo_cluster_container + off);
-  const float* Zs = reinterpret_cast<const float*>(a.dev_velo_cluster_container + total + off);
-  const uint32_t* IDs = a.dev_velo_cluster_container + 2 * (size_t) total + off;
-  const float* Xs = reinterpret_cast<const float*>(a.dev_velo_cluster_container + 5 * (size_t) total + off);
   float* cx = reinterpret_cast<float*>(a.dev_velo_track_hits);
   float* cy = cx + total_hits;
   float* cz = cy + total_hits;
   uint32_t* cid = reinterpret_cast<uint32_t*>(cz + total_hits);
   float* S = reinterpret_cast<float*>(a.dev_velo_states);
   uint8_t* backward = reinterpret_cast<uint8_t*>(a.dev_velo_states + 12 * sizeof(float) * (size_t) total_tracks);
-  const uint8_t* tracks = static_cast<const uint8_t*>(a.dev_tracks) + (size_t) e * VELO_MAX_TRACKS * VELO_TRACK_HITS_BYTES;
-
+  const uint8_t* tracks =
+    static_cast<const uint8_t*>(a.dev_tracks) + (size_t) e * VELO_MAX_TRACKS * VELO_TRACK_HITS_BYTES;
   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
     const uint16_t* t = reinterpret_cast<const uint16_t*>(tracks + (size_t) i * VELO_TRACK_HITS_BYTES);
     const uint32_t nh = t[0];
@@ -146,11 +146,11 @@ __global__ void consolidate_kernel(velo_consolidate_velo_tracks_args a)
     float u0 = 0.0f, uy = 0.0f, uz = 0.0f, uyz = 0.0f, uz2 = 0.0f;
     for (uint32_t h = 0; h < nh; ++h) {
       const uint32_t hi = t[1 + h];
-      const float x = Xs[hi], y = Ys[hi], z = Zs[hi];
+      const float x = hv.x[hi], y = hv.y[hi], z = hv.z[hi];
       cx[ho + h] = x;
       cy[ho + h] = y;
       cz[ho + h] = z;
-      cid[ho + h] = IDs[hi];
+      cid[ho + h] = hv.id[hi];
       const float wx = kParamW;
       const float wx_t_x = wx * x;
       const float wx_t_z = wx * z;
@@ -175,7 +175,7 @@ __global__ void consolidate_kernel(velo_consolidate_velo_tracks_args a)
     const float ty = (uyz * u0 - uy * uz) * denu;
     float y0 = (uy * uz2 - uyz * uz) * denu;
     const float zb = -(x0 * tx + y0 * ty) / (tx * tx + ty * ty);
-    const float z_first = Zs[t[1]];
+    const float z_first = hv.z[t[1]];
     const bool bwd = zb > z_first;
     const float m00 = s0;
     const float m11 = u0;
@@ -189,11 +189,11 @@ __global__ void consolidate_kernel(velo_consolidate_velo_tracks_args a)
     int ndof = -4;
     for (uint32_t h = 0; h < nh; ++h) {
       const uint32_t hi = t[1 + h];
-      const float z = Zs[hi];
+      const float z = hv.z[hi];
       const float x = x0 + tx * z;
       const float y = y0 + ty * z;
-      const float dx = x - Xs[hi];
-      const float dy = y - Ys[hi];
+      const float dx = x - hv.x[hi];
+      const float dy = y - hv.y[hi];
       ch += dx * dx * kParamW + dy * dy * kParamW;
       ndof += 2;
     }
@@ -216,6 +216,44 @@ __global__ void consolidate_kernel(velo_consolidate_velo_tracks_args a)
     S[11 * T + ti] = zb;
     backward[ti] = bwd ? 1 : 0;
     if (a.dev_accepted_velo_tracks) a.dev_accepted_velo_tracks[ti] = 1;
+  }
+}
+
+__global__ void __launch_bounds__(1024) consolidate_kernel(velo_consolidate_velo_tracks_args a)
+{
+  extern __shared__ uint32_t s_hits[];
+  const uint32_t N = a.n_events;
+  const uint32_t e = blockIdx.x;
+  const uint32_t total_tracks = (uint32_t) a.dev_atomics_velo[2 * N];
+  const uint32_t total_hits = a.dev_velo_track_hit_number[total_tracks];
+  const uint32_t n = (uint32_t) a.dev_atomics_velo[e];
+  const uint32_t toff = (uint32_t) a.dev_atomics_velo[N + e];
+  const uint32_t total = a.dev_module_cluster_start[VELO_N_MODULES * N];
+  const uint32_t off = a.dev_module_cluster_start[e * VELO_N_MODULES];
+  const uint32_t ev_hits = a.dev_module_cluster_start[(e + 1) * VELO_N_MODULES] - off;
+  SlotView g;
+  g.y = reinterpret_cast<const float*>(a.dev_velo_cluster_container + off);
+  g.z = reinterpret_cast<const float*>(a.dev_velo_cluster_container + total + off);
+  g.id = a.dev_velo_cluster_container + 2 * (size_t) total + off;
+  g.x = reinterpret_cast<const float*>(a.dev_velo_cluster_container + 5 * (size_t) total + off);
+  const uint32_t cap = a.tuning.lds_tile;
+  if (n > 0 && ev_hits <= cap) {
+    for (uint32_t i = threadIdx.x; i < ev_hits; i += blockDim.x) {
+      s_hits[i] = __float_as_uint(g.x[i]);
+      s_hits[cap + i] = __float_as_uint(g.y[i]);
+      s_hits[2 * cap + i] = __float_as_uint(g.z[i]);
+      s_hits[3 * cap + i] = g.id[i];
+    }
+    __syncthreads();
+    SlotView l;
+    l.x = reinterpret_cast<const float*>(s_hits);
+    l.y = reinterpret_cast<const float*>(s_hits + cap);
+    l.z = reinterpret_cast<const float*>(s_hits + 2 * cap);
+    l.id = s_hits + 3 * cap;
+    consolidate_event(l, a, e, n, toff, total_tracks, total_hits);
+  }
+  else {
+    consolidate_event(g, a, e, n, toff, total_tracks, total_hits);
   }
 }
 
@@ -276,7 +314,10 @@ int velo_consolidate_velo_tracks_launch(const velo_consolidate_velo_tracks_args*
     return (int) hipErrorInvalidValue;
   if (a->n_events == 0) return 0;
   const uint32_t block = a->tuning.block_dim ? a->tuning.block_dim : 256;
-  hipLaunchKernelGGL(consolidate_kernel, dim3(a->n_events), dim3(block), 0, (hipStream_t) stream, *a);
+  velo_consolidate_velo_tracks_args args = *a;
+  if (!args.tuning.lds_tile) args.tuning.lds_tile = kDefaultConsolidateHits;
+  hipLaunchKernelGGL(consolidate_kernel, dim3(args.n_events), dim3(block), 4 * sizeof(uint32_t) * args.tuning.lds_tile,
+                     (hipStream_t) stream, args);
   return (int) hipGetLastError();
 }
 

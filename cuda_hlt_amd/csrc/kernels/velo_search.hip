@@ -9,8 +9,11 @@
 // output is deterministic and equal to the CPU oracle's sequential order.
 //
 // State: hit_used as a bitmask in LDS (events up to 16384 estimated hits;
-// larger events use the global byte array directly); the tracks-to-follow
-// ring lives in dev_tracks_to_follow (2000 entries of 8 bytes per event):
+// larger events use the global byte array directly); x, y, z, phi of the
+// nine modules a step can touch (f+3 .. f-5 for first module f) are copied
+// into an LDS window at the start of every step, other hits are read from
+// global memory; the tracks-to-follow ring lives in dev_tracks_to_follow
+// (2000 entries of 8 bytes per event):
 //   word0 = h[n-2] | h[n-1] << 16
 //   word1 = (tracklet ? first hit : track number) | n << 16 | skipped << 28
 //           | tracklet << 31
@@ -43,8 +46,76 @@ struct ModuleData {
   float z;
 };
 
+// Dynamic LDS of search_by_triplet: the hit window, four arrays of `cap`.
+extern __shared__ float s_window[];
+
+// Opaque copy of a register value.  Keeps the optimiser from turning
+// "LDS value, else global load" into one flat load through a selected
+// pointer (which gfx950 code generation rejects for this LDS symbol);
+// emits no instruction.
+__device__ inline float launder(float v)
+{
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
 struct EventHits {
-  const float *x, *y, *z, *phi;
+  const float *gx, *gy, *gz, *gphi; // event-relative global slots
+  uint32_t cap;                     // window capacity (hits)
+  uint32_t lo, n;                   // window = hits [lo, lo + n)
+  __device__ float x(uint32_t h) const
+  {
+    const uint32_t r = h - lo;
+    const uint32_t rr = r < n ? r : 0u;
+    float v = launder(s_window[rr]);
+    if (r >= n) v = gx[h];
+    return v;
+  }
+  __device__ float y(uint32_t h) const
+  {
+    const uint32_t r = h - lo;
+    const uint32_t rr = r < n ? r : 0u;
+    float v = launder(s_window[cap + rr]);
+    if (r >= n) v = gy[h];
+    return v;
+  }
+  __device__ float z(uint32_t h) const
+  {
+    const uint32_t r = h - lo;
+    const uint32_t rr = r < n ? r : 0u;
+    float v = launder(s_window[2 * cap + rr]);
+    if (r >= n) v = gz[h];
+    return v;
+  }
+  __device__ float phi(uint32_t h) const
+  {
+    const uint32_t r = h - lo;
+    const uint32_t rr = r < n ? r : 0u;
+    float v = launder(s_window[3 * cap + rr]);
+    if (r >= n) v = gphi[h];
+    return v;
+  }
+  // Copies hits [lo_, hi_) into the window when they fit (wave-wide).
+  __device__ void stage(uint32_t lo_, uint32_t hi_)
+  {
+    lo = lo_;
+    n = hi_ - lo_ <= cap ? hi_ - lo_ : 0u;
+    for (uint32_t i = lane_id(); i < n; i += kWave) {
+      s_window[i] = gx[lo + i];
+      s_window[cap + i] = gy[lo + i];
+      s_window[2 * cap + i] = gz[lo + i];
+      s_window[3 * cap + i] = gphi[lo + i];
+    }
+    wave_sync();
+  }
+};
+
+// Plain global view of an event's sorted x, y, z (weak_tracks_adder).
+struct GlobalHits {
+  const float *gx, *gy, *gz;
+  __device__ float x(uint32_t h) const { return gx[h]; }
+  __device__ float y(uint32_t h) const { return gy[h]; }
+  __device__ float z(uint32_t h) const { return gz[h]; }
 };
 
 __device__ inline void load_modules(
@@ -65,17 +136,17 @@ __device__ inline float seed_best(
   uint32_t h1, const EventHits& H, const int16_t* h0c, const int16_t* h2c, const UsedSet& used, uint32_t* b0,
   uint32_t* b2)
 {
-  const float h1x = H.x[h1], h1y = H.y[h1], h1z = H.z[h1];
+  const float h1x = H.x(h1), h1y = H.y(h1), h1z = H.z(h1);
   const int f0 = h0c[2 * h1], l0 = h0c[2 * h1 + 1];
   const int f2 = h2c[2 * h1], l2 = h2c[2 * h1 + 1];
   float best = kFltMax;
   uint32_t bb0 = 0, bb2 = 0;
   for (int h0 = f0; h0 < l0; ++h0) {
     if (used.test(h0)) continue;
-    const float h0x = H.x[h0], h0y = H.y[h0], h0z = H.z[h0];
+    const float h0x = H.x(h0), h0y = H.y(h0), h0z = H.z(h0);
     for (int h2 = f2; h2 < l2; ++h2) {
       if (used.test(h2)) continue;
-      const float h2x = H.x[h2], h2y = H.y[h2], h2z = H.z[h2];
+      const float h2x = H.x(h2), h2y = H.y(h2), h2z = H.z(h2);
       const float dmax = kMaxSlope * (h0z - h1z);
       const float dz21 = h2z - h1z;
       const float scatter_denom2 = 1.0f / (dz21 * dz21);
@@ -100,9 +171,16 @@ __device__ inline float seed_best(
   return best;
 }
 
+// phi of hit base + i through the window
+struct PhiAt {
+  const EventHits* H;
+  uint32_t base;
+  __device__ float operator()(int i) const { return H->phi(base + (uint32_t) i); }
+};
+
 // TrackForwarding.cuh:28-58
 __device__ inline void forward_window(
-  const ModuleData& md, float tx, float ty, const float* phis, float h0x, float h0y, float h0z, bool odd, int* f,
+  const ModuleData& md, float tx, float ty, const EventHits& H, float h0x, float h0y, float h0z, bool odd, int* f,
   int* l)
 {
   const float dz = md.z - h0z;
@@ -111,10 +189,11 @@ __device__ inline void forward_window(
   const float xp = h0x + predx;
   const float yp = h0y + predy;
   const float phi = odd ? phi_odd(xp, yp) : phi_even(xp, yp);
-  int first = bs_first_candidate(phis + md.start, md.num, phi, kForwardPhiTolerance);
+  int first = bs_first_candidate(PhiAt {&H, md.start}, md.num, phi, kForwardPhiTolerance);
   int last = -1;
   if (first != -1) {
-    last = bs_second_candidate(phis + md.start + first, md.num - (uint32_t) first, phi, kForwardPhiTolerance);
+    last = bs_second_candidate(PhiAt {&H, md.start + (uint32_t) first}, md.num - (uint32_t) first, phi,
+                               kForwardPhiTolerance);
     first += (int) md.start;
     last = last == 0 ? first + 1 : first + last;
   }
@@ -211,12 +290,14 @@ __device__ inline uint32_t seeding(
 {
   const uint32_t lane = lane_id();
   uint32_t staged = 0, tracklets = 0;
+#pragma unroll
   for (int side = 2; side <= 3; ++side) {
-    for (uint32_t base = 0; base < md[side].num; base += kWave) {
+    const uint32_t side_start = md[side].start, side_num = md[side].num;
+    for (uint32_t base = 0; base < side_num; base += kWave) {
       const uint32_t r = base + lane;
-      const uint32_t h1 = md[side].start + r;
+      const uint32_t h1 = side_start + r;
       bool cand = false;
-      if (r < md[side].num) cand = !used.test((int) h1) && h0c[2 * h1] != -1 && h2c[2 * h1] != -1;
+      if (r < side_num) cand = !used.test((int) h1) && h0c[2 * h1] != -1 && h2c[2 * h1] != -1;
       const uint64_t m = ballot(cand);
       if (cand) stage[staged + count_below(m)] = (uint16_t) h1;
       staged += (uint32_t) __popcll(m);
@@ -260,22 +341,22 @@ __device__ inline void forwarding(
     bool found = false;
     uint32_t best_h2 = 0;
     if (active) {
-      const float h0x = H.x[ha], h0y = H.y[ha], h0z = H.z[ha];
-      const float h1x = H.x[hb], h1y = H.y[hb], h1z = H.z[hb];
+      const float h0x = H.x(ha), h0y = H.y(ha), h0z = H.z(ha);
+      const float h1x = H.x(hb), h1y = H.y(hb), h1z = H.z(hb);
       const float td = 1.0f / (h1z - h0z);
       const float txn = h1x - h0x;
       const float tyn = h1y - h0y;
       const float tx = txn * td;
       const float ty = tyn * td;
       int of, ol, ef, el;
-      forward_window(md[4], tx, ty, H.phi, h0x, h0y, h0z, true, &of, &ol);
-      forward_window(md[5], tx, ty, H.phi, h0x, h0y, h0z, false, &ef, &el);
+      forward_window(md[4], tx, ty, H, h0x, h0y, h0z, true, &of, &ol);
+      forward_window(md[5], tx, ty, H, h0x, h0y, h0z, false, &ef, &el);
       const int n_odd = ol - of;
       const int n_all = n_odd + (el - ef);
       float best = kFltMax;
       for (int j = 0; j < n_all; ++j) {
         const int h2 = j < n_odd ? of + j : ef + j - n_odd;
-        const float h2x = H.x[h2], h2y = H.y[h2], h2z = H.z[h2];
+        const float h2x = H.x(h2), h2y = H.y(h2), h2z = H.z(h2);
         const float dz = h2z - h0z;
         const float predx = tx * dz;
         const float predy = ty * dz;
@@ -362,10 +443,13 @@ __device__ inline void forwarding(
   }
 }
 
+constexpr uint32_t kDefaultWindowHits = 1024;
+
 __global__ void __launch_bounds__(64) search_by_triplet_kernel(velo_search_by_triplet_args a)
 {
   __shared__ uint32_t used_bits[kUsedWords];
   __shared__ uint16_t stage[2 * kWave];
+
   const uint32_t N = a.n_events;
   const uint32_t e = blockIdx.x;
   const uint32_t lane = lane_id();
@@ -374,11 +458,21 @@ __global__ void __launch_bounds__(64) search_by_triplet_kernel(velo_search_by_tr
   const uint32_t* nums = a.dev_module_cluster_num + e * VELO_N_MODULES;
   const uint32_t off = starts[0];
   const uint32_t ev_hits = starts[VELO_N_MODULES] - off;
+  const uint32_t cap = a.tuning.lds_tile;
   EventHits H;
-  H.y = reinterpret_cast<const float*>(a.dev_velo_cluster_container + off);
-  H.z = reinterpret_cast<const float*>(a.dev_velo_cluster_container + total + off);
-  H.phi = reinterpret_cast<const float*>(a.dev_velo_cluster_container + 4 * (size_t) total + off);
-  H.x = reinterpret_cast<const float*>(a.dev_velo_cluster_container + 5 * (size_t) total + off);
+  H.gy = reinterpret_cast<const float*>(a.dev_velo_cluster_container + off);
+  H.gz = reinterpret_cast<const float*>(a.dev_velo_cluster_container + total + off);
+  H.gphi = reinterpret_cast<const float*>(a.dev_velo_cluster_container + 4 * (size_t) total + off);
+  H.gx = reinterpret_cast<const float*>(a.dev_velo_cluster_container + 5 * (size_t) total + off);
+  H.cap = cap;
+  H.lo = 0;
+  H.n = 0;
+  // window of modules first+3 .. first-5 (contiguous in the container)
+  const auto stage_window = [&](uint32_t first) {
+    const uint32_t lo_m = first - 5;
+    const uint32_t hi_m = first + 3 < VELO_N_MODULES ? first + 3 : VELO_N_MODULES - 1;
+    H.stage(starts[lo_m] - off, starts[hi_m + 1] - off);
+  };
   const int16_t* h0c = a.dev_h0_candidates + 2 * (size_t) off;
   const int16_t* h2c = a.dev_h2_candidates + 2 * (size_t) off;
   uint8_t* tracks = static_cast<uint8_t*>(a.dev_tracks) + (size_t) e * VELO_MAX_TRACKS * VELO_TRACK_HITS_BYTES;
@@ -398,11 +492,13 @@ __global__ void __launch_bounds__(64) search_by_triplet_kernel(velo_search_by_tr
   ModuleData md[6];
   uint32_t first = VELO_N_MODULES - 1;
   load_modules(md, first, starts, nums, off, a.dev_velo_module_zs);
+  stage_window(first);
   uint32_t prev_tracklets = seeding(md, stage, H, h0c, h2c, used, ring, st);
   uint32_t cur_begin = 0, cur_n = st.next_n;
   first -= 2;
   while (first >= 4) {
     load_modules(md, first, starts, nums, off, a.dev_velo_module_zs);
+    stage_window(first);
     st.next_begin = (cur_begin + cur_n) % VELO_TTF_MODULO;
     st.next_n = 0;
     st.cap = VELO_TTF_MODULO - cur_n;
@@ -442,7 +538,8 @@ __global__ void __launch_bounds__(64) search_by_triplet_kernel(velo_search_by_tr
 }
 
 // WeakTracksAdder.cu:8-86
-__device__ inline float weak_chi2(const EventHits& H, uint32_t a0, uint32_t a1, uint32_t a2)
+template<typename Hits>
+__device__ inline float weak_chi2(const Hits& H, uint32_t a0, uint32_t a1, uint32_t a2)
 {
   float s0 = 0.0f, sx = 0.0f, sz = 0.0f, sxz = 0.0f, sz2 = 0.0f;
   float u0 = 0.0f, uy = 0.0f, uz = 0.0f, uyz = 0.0f, uz2 = 0.0f;
@@ -450,7 +547,7 @@ __device__ inline float weak_chi2(const EventHits& H, uint32_t a0, uint32_t a1, 
 #pragma unroll
   for (int h = 0; h < 3; ++h) {
     const uint32_t i = hs[h];
-    const float x = H.x[i], y = H.y[i], z = H.z[i];
+    const float x = H.x(i), y = H.y(i), z = H.z(i);
     const float wx = kParamW;
     const float wx_t_x = wx * x;
     const float wx_t_z = wx * z;
@@ -479,11 +576,11 @@ __device__ inline float weak_chi2(const EventHits& H, uint32_t a0, uint32_t a1, 
 #pragma unroll
   for (int h = 0; h < 3; ++h) {
     const uint32_t i = hs[h];
-    const float z = H.z[i];
+    const float z = H.z(i);
     const float x = x0 + tx * z;
     const float y = y0 + ty * z;
-    const float dx = x - H.x[i];
-    const float dy = y - H.y[i];
+    const float dx = x - H.x(i);
+    const float dy = y - H.y(i);
     ch += dx * dx * kParamW + dy * dy * kParamW;
     ndof += 2;
   }
@@ -497,11 +594,10 @@ __global__ void __launch_bounds__(64) weak_tracks_adder_kernel(velo_weak_tracks_
   const uint32_t lane = lane_id();
   const uint32_t total = a.dev_module_cluster_start[VELO_N_MODULES * N];
   const uint32_t off = a.dev_module_cluster_start[e * VELO_N_MODULES];
-  EventHits H;
-  H.y = reinterpret_cast<const float*>(a.dev_velo_cluster_container + off);
-  H.z = reinterpret_cast<const float*>(a.dev_velo_cluster_container + total + off);
-  H.phi = nullptr;
-  H.x = reinterpret_cast<const float*>(a.dev_velo_cluster_container + 5 * (size_t) total + off);
+  GlobalHits H;
+  H.gy = reinterpret_cast<const float*>(a.dev_velo_cluster_container + off);
+  H.gz = reinterpret_cast<const float*>(a.dev_velo_cluster_container + total + off);
+  H.gx = reinterpret_cast<const float*>(a.dev_velo_cluster_container + 5 * (size_t) total + off);
   const uint8_t* used = a.dev_hit_used + off;
   uint8_t* tracks = static_cast<uint8_t*>(a.dev_tracks) + (size_t) e * VELO_MAX_TRACKS * VELO_TRACK_HITS_BYTES;
   const uint8_t* weak =
@@ -561,7 +657,10 @@ int velo_search_by_triplet_launch(const velo_search_by_triplet_args* a, void* st
       !a->dev_atomics_velo || !a->dev_h0_candidates || !a->dev_h2_candidates || !a->dev_velo_module_zs)
     return (int) hipErrorInvalidValue;
   if (a->n_events == 0) return 0;
-  hipLaunchKernelGGL(search_by_triplet_kernel, dim3(a->n_events), dim3(kWave), 0, (hipStream_t) stream, *a);
+  velo_search_by_triplet_args args = *a;
+  if (!args.tuning.lds_tile) args.tuning.lds_tile = kDefaultWindowHits;
+  hipLaunchKernelGGL(search_by_triplet_kernel, dim3(args.n_events), dim3(kWave), 4 * sizeof(float) * args.tuning.lds_tile,
+                     (hipStream_t) stream, args);
   return (int) hipGetLastError();
 }
 
