@@ -53,6 +53,37 @@ def algorithmic_bytes(S, H, C, T, Ht, N):
     }
 
 
+# kernels each step launches (names as rocprofv3 reports them)
+STEP_KERNELS = {
+    "velo_estimate_input_size_t": ["estimate_input_size_kernel"],
+    "velo_masked_clustering_t": ["masked_clustering_kernel"],
+    "velo_calculate_phi_and_sort_t": ["calculate_phi_and_sort_kernel"],
+    "velo_fill_candidates_t": ["fill_candidates_kernel"],
+    "velo_search_by_triplet_t": ["search_by_triplet_kernel"],
+    "velo_weak_tracks_adder_t": ["weak_tracks_adder_kernel"],
+    "consolidate_velo_tracks_t": ["consolidate_kernel"],
+}
+
+
+def measured_traffic(path, step, events_per_launch):
+    """HBM bytes per launch of the step's kernels from a rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), or None
+    when no summary for this launch size exists."""
+    try:
+        summary = json.loads(Path(path).read_text())
+    except (OSError, ValueError):
+        return None
+    if summary.get("events_per_launch") != events_per_launch:
+        return None
+    total = 0.0
+    for k in STEP_KERNELS.get(step, []):
+        rec = summary.get("kernels", {}).get(k)
+        if rec is None:
+            return None
+        total += rec["fetch_bytes"] + rec["write_bytes"]
+    return round(total) if total else None
+
+
 def cpu_baseline(ev, geometry, zs, budget_s=12.0):
     """The oracle (single thread, C) on the first events of the workload."""
     from oracle import pyoracle as po
@@ -69,6 +100,60 @@ def cpu_baseline(ev, geometry, zs, budget_s=12.0):
             "sample": f"first {sub.n_events} events of the batch, single-threaded C oracle, {dt:.1f} s"}
 
 
+def batched_run(args, geometry, zs):
+    """Uneven events (nu ~ U(2,11) per event) in one batch: the sequence
+    captured once in a hipGraph, persistent steps in largest-first order,
+    one graph launch per step; outputs are those of the plain run."""
+    import torch
+    from cuda_hlt_amd import autotune, gen
+    from cuda_hlt_amd.stream import VeloSequence
+    n = args.batched_events
+    t0 = time.perf_counter()
+    ev = gen.uneven(n, seed=args.seed + 1)
+    log(f"[bench] batched: generated {n} uneven events in {time.perf_counter() - t0:.1f} s")
+    raw = torch.from_numpy(ev.events).cuda()
+    off = torch.from_numpy(ev.offsets.view(np.int32)).cuda()
+    sizes = np.diff(ev.offsets.astype(np.int64))
+    order = torch.from_numpy(np.argsort(-sizes, kind="stable").astype(np.uint32).view(np.int32)).cuda()
+    tuning = autotune.load(args.batched_tuning) if args.batched_tuning else autotune.BATCHED_DEFAULT
+    need_mb = autotune.batched_reserve_mb(n, ev.events.nbytes)
+    seq = VeloSequence(geometry, zs, max_events=n, reserve_mb=need_mb)
+    seq.enable_step_timing(True)
+
+    def run():
+        return seq.run(n, dev_events=raw.data_ptr(), dev_offsets=off.data_ptr(), events_size=ev.events.nbytes,
+                       copy_outputs=False, batched=True, tuning=tuning, dev_work_order=order.data_ptr())
+
+    t0 = time.perf_counter()
+    run()  # capture + first replay
+    capture_s = time.perf_counter() - t0
+    run()
+    torch.cuda.synchronize()
+    lat = []
+    graph_ms = []
+    t_start = time.perf_counter()
+    for _ in range(args.batched_steps):
+        t1 = time.perf_counter()
+        o = run()
+        lat.append(time.perf_counter() - t1)
+        graph_ms.append(seq.step_times()[0][1])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    hits = int(o.total_estimated_hits)
+    seq.close()
+    return {"value": round(args.batched_steps * n / elapsed, 1), "unit": "events/s",
+            "events_per_launch": n, "steps": args.batched_steps,
+            "ms_per_step": round(1e3 * elapsed / args.batched_steps, 3),
+            "graph_ms_p50": round(float(np.percentile(graph_ms, 50)), 3),
+            "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 3),
+                           "p99": round(1e3 * float(np.percentile(lat, 99)), 3)},
+            "capture_and_first_run_s": round(capture_s, 3),
+            "workload": f"uneven nu~U(2,11), {n} events (seed {args.seed + 1}), sequence captured in one hipGraph, "
+                        f"persistent steps in largest-first order",
+            "tuning": tuning, "reserve_mb": need_mb, "tracks": int(o.total_tracks),
+            "raw_bytes": int(ev.events.nbytes), "estimated_clusters": hits}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +163,16 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tuning", default=None, help="launch shapes from cuda_hlt_amd.autotune (JSON)")
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="first run one int32 add-one kernel over this many MiB (reads and writes exactly that "
+                         "much): the known-size reference for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE")
+    ap.add_argument("--batched-events", type=int, default=50000,
+                    help="events of the batched run (uneven nu ~ U(2,11)); 0 skips it")
+    ap.add_argument("--batched-steps", type=int, default=5)
+    ap.add_argument("--batched-tuning", default=None,
+                    help="launch shapes of the batched run (cuda_hlt_amd.autotune JSON)")
+    ap.add_argument("--traffic", default=str(ROOT / "profiles" / "traffic_minbias_10k.json"),
+                    help="per-kernel HBM traffic measured by rocprofv3 --pmc (cuda_hlt_amd.profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -97,6 +192,15 @@ def main():
     geometry, zs = gen.geometry(), gen.module_zs()
 
     dev = torch.device("cuda:0")
+    if args.calibrate_copy_mb:
+        words = args.calibrate_copy_mb << 18
+        src = torch.ones(words, dtype=torch.int32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        torch.add(src, 1, out=dst)
+        torch.cuda.synchronize()
+        log(f"[bench] calibration copy: {4 * words} bytes read and written")
+        del src, dst
     raw = torch.from_numpy(ev.events).to(dev)
     batch_offsets = []
     for b in range(n_batches):
@@ -111,9 +215,9 @@ def main():
         from cuda_hlt_amd import autotune
         tuning = autotune.load(args.tuning)
 
-    def run(b):
+    def run(b, copy_outputs=False):
         base, off = batch_offsets[b]
-        return seq.run(B, dev_events=raw.data_ptr() + base, dev_offsets=off.data_ptr(), copy_outputs=False,
+        return seq.run(B, dev_events=raw.data_ptr() + base, dev_offsets=off.data_ptr(), copy_outputs=copy_outputs,
                        tuning=tuning)
 
     for w in range(args.warmup):
@@ -135,8 +239,10 @@ def main():
     value = args.steps * B / elapsed
 
     # per-batch sizes for the algorithmic bytes (batch 0 stands for all)
+    from cuda_hlt_amd import checker
     from cuda_hlt_amd import native as nat
-    o0 = run(0)
+    o0 = run(0, copy_outputs=True)
+    chk = checker.check(ev.subset(0, B), seq.host_tracks())
     host_nums = nat.d2h(o0.dev_module_cluster_num, 4 * B * 52).view(np.uint32)
     H = int(host_nums.sum())
     S = int(ev.sp_counts()[:B].sum())
@@ -152,7 +258,8 @@ def main():
     dom_ms = per_step[dominant]
     achieved = ab[dominant] / (dom_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None}
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": measured_traffic(args.traffic, dominant, B)}
 
     result = {
         "metric": METRIC, "value": round(value, 1), "unit": "events/s", "n_gpus": 1, "steps": args.steps,
@@ -164,9 +271,15 @@ def main():
         "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 3),
                        "p99": round(1e3 * float(np.percentile(lat, 99)), 3)},
         "roofline": roofline,
+        "efficiency": {"category": checker.HEADLINE, "value": round(chk.efficiency(), 5),
+                       "ghost_rate": round(chk.ghost_rate, 5), "events": B, "tracks": chk.tracks},
         "kernels": kernels,
         "batch": {"sp_words": S, "clusters": H, "tracks": o0.total_tracks, "track_hits": o0.total_track_hits},
     }
+    seq.close()
+    del raw, batch_offsets
+    if args.batched_events:
+        result["batched"] = batched_run(args, geometry, zs)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(ev.subset(0, B), geometry, zs)
     print(json.dumps(result), flush=True)

@@ -27,6 +27,30 @@ SPACE = {
     "search_by_triplet": [(64, t) for t in (1, 256, 512, 768, 1024, 2048)],
     "consolidate": [(b, t) for b in (64, 128, 256, 512) for t in (1, 1024, 2048, 3072)],
 }
+# Uneven batches: workgroup size x LDS tile first, then the persistent grid
+# (0 = one workgroup per event) at the best shape; events in largest-first
+# order.  The search kernel is one wave per event.
+BATCHED_BLOCKS = (64, 128, 256, 512)
+BATCHED_TILES = {
+    "estimate_input_size": (4096, 6144, 8192, 12288),
+    "masked_clustering": (4096, 6144, 8192, 12288),
+    "calculate_phi_and_sort": (0,),
+    "fill_candidates": (1024, 2048, 4096, 8192),
+    "search_by_triplet": (256, 512, 1024, 2048, 4096),
+    "consolidate": (256, 1024, 2048, 3072),
+}
+BATCHED_GRIDS = (0, 1024, 2048, 4096, 8192)
+
+# Launch shapes of the batched path when no tuning file is given
+BATCHED_DEFAULT = {
+    "estimate_input_size": (512, 8192, 0),
+    "masked_clustering": (512, 8192, 0),
+    "calculate_phi_and_sort": (512, 0, 0),
+    "fill_candidates": (512, 4096, 0),
+    "search_by_triplet": (64, 256, 0),
+    "consolidate": (512, 3072, 0),
+}
+
 STEP_OF = {
     "estimate_input_size": "velo_estimate_input_size_t",
     "masked_clustering": "velo_masked_clustering_t",
@@ -73,6 +97,55 @@ def tune(seq, n_events, dev_events, dev_offsets, repeats=3, log=print):
     return best, table
 
 
+def batched_reserve_mb(n_events, raw_bytes):
+    """Device arena a batched run needs: the bounds it sizes buffers by
+    (clusters <= raw bytes / 2, 1200 tracks and 1200 * 26 track hits per
+    event) plus the per-event arrays, with 10 % for alignment."""
+    clusters = raw_bytes // 2 + 1
+    tracks = n_events * 1200
+    per_cluster = 24 + 4 + 8 + 1              # container, permutation, windows, used flags
+    per_track = 54 + 4 + 26 * 16 + 49 + 1     # TrackHits, hit count, consolidated hits, state, accepted
+    per_event = 52 * 4 * 2 + 3000 * 4 + 4 * 4 + 2000 * 8 + 500 * 6 + 5 * 4
+    total = clusters * per_cluster + tracks * per_track + n_events * per_event
+    return int(total * 1.1 / 2 ** 20) + 64
+
+
+def tune_batched(seq, n_events, dev_events, dev_offsets, events_size, dev_order, repeats=3, log=print):
+    """Launch shapes for uneven batches: per step, workgroup size x LDS tile,
+    then the persistent grid, timed with HIP events in plain runs that use
+    the batched path's largest-first order; checks the batched (graph) run
+    with the result against the default output.  Returns (best, table)."""
+    seq.enable_step_timing(True)
+    kw = dict(dev_events=dev_events, dev_offsets=dev_offsets, events_size=events_size, copy_outputs=False,
+              dev_work_order=dev_order)
+    seq.run(n_events, **kw)
+    reference = _outputs(seq, n_events)
+    best = dict(BATCHED_DEFAULT)
+    table = {}
+
+    def time_cfg(alg, cfg):
+        ms = []
+        for _ in range(repeats):
+            seq.run(n_events, tuning={**best, alg: cfg}, **kw)
+            ms.append(dict(seq.step_times())[STEP_OF[alg]])
+        return float(np.median(ms))
+
+    for alg in STEP_OF:
+        blocks = (64,) if alg == "search_by_triplet" else BATCHED_BLOCKS
+        times = {(b, t, 0): time_cfg(alg, (b, t, 0)) for b in blocks for t in BATCHED_TILES[alg]}
+        b, t, _ = min(times, key=times.get)
+        for g in BATCHED_GRIDS[1:]:
+            times[(b, t, g)] = time_cfg(alg, (b, t, g))
+        cfg = min(times, key=times.get)
+        best[alg] = cfg
+        table[alg] = {f"{b}x{t}x{g}": round(v, 4) for (b, t, g), v in times.items()}
+        log(f"[autotune] {alg}: best block={cfg[0]} tile={cfg[1]} grid={cfg[2]} {times[cfg]:.3f} ms")
+    seq.run(n_events, tuning=best, batched=True, **kw)
+    if _outputs(seq, n_events) != reference:
+        raise RuntimeError("tuned batched run changed the output")
+    return best, table
+
+
 def load(path):
     with open(path) as f:
         d = json.load(f)
@@ -85,6 +158,8 @@ def main():
     ap.add_argument("--uneven", action="store_true", help="per-event nu ~ U(2, 11)")
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--out", default="tuning.json")
+    ap.add_argument("--batched", action="store_true",
+                    help="tune the batched path (persistent grids, largest-first order)")
     args = ap.parse_args()
     import torch
     from . import gen
@@ -92,9 +167,18 @@ def main():
     ev = gen.uneven(args.events, seed=args.seed) if args.uneven else gen.minbias(args.events, seed=args.seed)
     raw = torch.from_numpy(ev.events).cuda()
     off = torch.from_numpy(ev.offsets.view(np.int32)).cuda()
-    seq = VeloSequence(gen.geometry(), gen.module_zs(), max_events=args.events, reserve_mb=8192)
     t0 = time.time()
-    best, table = tune(seq, ev.n_events, raw.data_ptr(), off.data_ptr(), log=lambda *a: print(*a, file=sys.stderr))
+    say = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+    if args.batched:
+        sizes = np.diff(ev.offsets.astype(np.int64))
+        order = torch.from_numpy(np.argsort(-sizes, kind="stable").astype(np.uint32).view(np.int32)).cuda()
+        seq = VeloSequence(gen.geometry(), gen.module_zs(), max_events=args.events,
+                           reserve_mb=batched_reserve_mb(args.events, ev.events.nbytes))
+        best, table = tune_batched(seq, ev.n_events, raw.data_ptr(), off.data_ptr(), ev.events.nbytes,
+                                   order.data_ptr(), log=say)
+    else:
+        seq = VeloSequence(gen.geometry(), gen.module_zs(), max_events=args.events, reserve_mb=8192)
+        best, table = tune(seq, ev.n_events, raw.data_ptr(), off.data_ptr(), log=say)
     out = {"best": {k: list(v) for k, v in best.items()}, "ms": table,
            "workload": f"{'uneven nu~U(2,11)' if args.uneven else 'minbias nu=7.6'} {args.events} events seed {args.seed}",
            "seconds": round(time.time() - t0, 1)}

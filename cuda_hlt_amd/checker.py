@@ -1,0 +1,135 @@
+"""VELO track checker: efficiency per MC category, clone and ghost rates.
+
+Python side of velo_checker_run (include/velo_checker.h), the host C++
+restatement of the reference's VELO checker (MCAssociator.cpp:13-60,
+TrackChecker.cpp:93-146 and :349-405, CategoriesTrackChecker.cpp:5-230).
+It takes consolidated tracks in the reference layout (scanned track counts,
+scanned hit counts, SoA hits whose last quarter is the LHCbIDs), so the
+same call checks the GPU output, the oracle's and a filtered subset.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import native as nat
+from .gen import MCP_DTYPE, Events
+
+N_CATEGORIES = 40
+MIN_WEIGHT = 0.7
+HEADLINE = "Not electron Velo"
+
+
+class _Category(C.Structure):
+    _fields_ = [("accepted", C.c_uint64), ("found", C.c_uint64), ("clones", C.c_uint64),
+                ("eff_per_event", C.c_double), ("hit_eff", C.c_double), ("hit_purity", C.c_double)]
+
+
+class _Result(C.Structure):
+    _fields_ = [("events", C.c_uint64), ("tracks", C.c_uint64), ("ghosts", C.c_uint64),
+                ("ghost_per_event", C.c_double), ("categories", _Category * N_CATEGORIES)]
+
+
+_bound = False
+
+
+def _lib():
+    global _bound
+    L = nat.lib()
+    if not _bound:
+        L.velo_checker_category_name.argtypes = [C.c_uint32]
+        L.velo_checker_category_name.restype = C.c_char_p
+        L.velo_checker_run.argtypes = [C.c_uint32] + [C.c_void_p] * 4 + [C.c_void_p, C.c_size_t, C.c_void_p,
+                                                                         C.POINTER(_Result), C.c_void_p, C.c_void_p]
+        L.velo_checker_run.restype = C.c_int
+        _bound = True
+    return L
+
+
+def category_names() -> list[str]:
+    L = _lib()
+    return [L.velo_checker_category_name(i).decode() for i in range(N_CATEGORIES)]
+
+
+@dataclass
+class CheckerResult:
+    events: int
+    tracks: int
+    ghosts: int
+    ghost_per_event: float
+    categories: dict = field(default_factory=dict)
+    track_mcp: np.ndarray | None = None     # int32 per track: matched particle (index into mcps) or -1
+    track_weight: np.ndarray | None = None  # float32 per track: best weight (0 when unmatched)
+
+    @property
+    def ghost_rate(self) -> float:
+        return self.ghosts / self.tracks if self.tracks else 0.0
+
+    def efficiency(self, name: str = HEADLINE) -> float:
+        c = self.categories[name]
+        return c["found"] / c["accepted"] if c["accepted"] else 0.0
+
+    @property
+    def ghost_labels(self) -> np.ndarray:
+        """Per track: True when the track is a ghost (no particle holds at
+        least 70 % of its matched hits)."""
+        return self.track_mcp < 0
+
+    def report(self) -> str:
+        """The reference's printout (TrackChecker.cpp:24-30, :126-145)."""
+        lines = []
+        for name, c in self.categories.items():
+            if not c["accepted"]:
+                continue
+            lines.append(f"{name:<50s}: {c['found']:9d}/{c['accepted']:9d} {100 * self.efficiency(name):6.2f}% "
+                         f"({100 * c['eff_per_event']:6.2f}%), {c['clones']:9d} ({100 * c['clone_rate']:6.2f}%) "
+                         f"clones, hit eff {100 * c['hit_eff']:6.2f}% pur {100 * c['hit_purity']:6.2f}%")
+        lines.append(f"{'TrackChecker output':<50s}: {self.ghosts:9d}/{self.tracks:9d} "
+                     f"{100 * self.ghost_rate:6.2f}% ({100 * self.ghost_per_event:6.2f}%) ghosts")
+        return "\n".join(lines)
+
+
+def check_arrays(n_events: int, event_track_offsets, track_hit_offsets, hit_ids, mcps, mc_hits,
+                 accepted=None) -> CheckerResult:
+    """Check tracks given as raw arrays (see velo_checker_run)."""
+    L = _lib()
+    eto = np.ascontiguousarray(event_track_offsets, np.uint32)
+    tho = np.ascontiguousarray(track_hit_offsets, np.uint32)
+    ids = np.ascontiguousarray(hit_ids, np.uint32)
+    mcps = np.ascontiguousarray(mcps, MCP_DTYPE)
+    mch = np.ascontiguousarray(mc_hits, np.uint32)
+    T = int(eto[n_events])
+    acc = None if accepted is None else np.ascontiguousarray(accepted, np.uint8)
+    if acc is not None and len(acc) < T:
+        raise ValueError("accepted holds fewer entries than tracks")
+    track_mcp = np.empty(max(T, 1), np.int32)
+    track_weight = np.empty(max(T, 1), np.float32)
+    res = _Result()
+    rc = L.velo_checker_run(n_events, eto.ctypes.data, tho.ctypes.data, ids.ctypes.data if len(ids) else None,
+                            acc.ctypes.data if acc is not None else None,
+                            mcps.ctypes.data if len(mcps) else None, len(mcps),
+                            mch.ctypes.data if len(mch) else None, C.byref(res),
+                            track_mcp.ctypes.data, track_weight.ctypes.data)
+    if rc != 0:
+        raise ValueError("velo_checker_run rejected its input (null pointer or MC particles out of event order)")
+    cats = {}
+    for i, name in enumerate(category_names()):
+        c = res.categories[i]
+        cats[name] = {"accepted": int(c.accepted), "found": int(c.found), "clones": int(c.clones),
+                      "eff_per_event": c.eff_per_event, "hit_eff": c.hit_eff, "hit_purity": c.hit_purity,
+                      "clone_rate": c.clones / (2 * c.found) if c.found else 0.0}
+    return CheckerResult(int(res.events), int(res.tracks), int(res.ghosts), res.ghost_per_event, cats,
+                         track_mcp[:T], track_weight[:T])
+
+
+def check(ev: Events, out, accepted=None) -> CheckerResult:
+    """Check consolidated tracks `out` (anything with n_events, atomics,
+    hit_number and track_hits in the reference layout) against ev's MC truth."""
+    n = out.n_events
+    T = int(out.atomics[2 * n])
+    Ht = int(out.hit_number[T])
+    ids = np.asarray(out.track_hits)[12 * Ht: 16 * Ht].view(np.uint32)
+    return check_arrays(n, np.asarray(out.atomics)[n: 2 * n + 1], np.asarray(out.hit_number)[: T + 1], ids,
+                        ev.mcps, ev.mc_hits, accepted)

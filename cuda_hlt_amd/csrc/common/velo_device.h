@@ -248,4 +248,50 @@ __device__ inline void set_status(uint32_t* status, uint32_t e, uint32_t bits)
   if (status && bits) atomicOr(status + e, bits);
 }
 
+// Runs body(e) for the events of this workgroup.  One workgroup per event
+// (t.grid_dim == 0): e = blockIdx.x.  Persistent (t.grid_dim > 0): the
+// workgroup takes positions from t.dev_work_counter[0] until they run out,
+// e = t.dev_work_order[k]; the last workgroup to leave zeroes both counter
+// words (word 1 counts finished workgroups) so the next launch starts
+// from zero.  body may use __syncthreads and LDS: iterations are separated
+// by a workgroup barrier.
+template<typename F>
+__device__ inline void for_each_event(const velo_tuning& t, uint32_t n_events, F&& body)
+{
+  if (t.grid_dim == 0) {
+    body(blockIdx.x);
+    return;
+  }
+  __shared__ uint32_t s_position;
+  for (;;) {
+    if (threadIdx.x == 0) s_position = atomicAdd(t.dev_work_counter, 1u);
+    __syncthreads();
+    const uint32_t k = s_position;
+    __syncthreads();
+    if (k >= n_events) break;
+    body(t.dev_work_order ? t.dev_work_order[k] : k);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(t.dev_work_counter + 1, 1u) == gridDim.x - 1) {
+      atomicExch(t.dev_work_counter, 0u);
+      atomicExch(t.dev_work_counter + 1, 0u);
+    }
+  }
+}
+
+// Grid of a per-event launch: n_events workgroups, or the persistent
+// grid_dim (never more workgroups than events).
+inline uint32_t event_grid(const velo_tuning& t, uint32_t n_events)
+{
+  return t.grid_dim ? (t.grid_dim < n_events ? t.grid_dim : n_events) : n_events;
+}
+
+// Host-side check of the persistent fields.
+inline bool schedule_ok(const velo_tuning& t)
+{
+  return t.reserved == 0 && (t.grid_dim == 0 || t.dev_work_counter != nullptr);
+}
+
 } // namespace velo

@@ -1,6 +1,9 @@
 // Stream::initialize / run_sequence (Stream.cu:16-83).
 #include "sequence/Stream.h"
 
+#include <algorithm>
+#include <cstring>
+
 hipError_t Stream::initialize(
   unsigned max_number_of_events,
   bool print_memory_manager,
@@ -19,13 +22,148 @@ hipError_t Stream::initialize(
   reserved_bytes = reserve_mb * 1024 * 1024;
   hipCheck(hipMalloc((void**) &dev_base_pointer, reserved_bytes));
   scheduler.initialize(do_print_memory_manager, reserved_bytes, dev_base_pointer);
+  hipCheck(hipMalloc((void**) &dev_work_counters, VELO_WORK_COUNTER_WORDS * sizeof(unsigned)));
+  hipCheck(hipMemset(dev_work_counters, 0, VELO_WORK_COUNTER_WORDS * sizeof(unsigned)));
+  hipCheck(hipEventCreate(&graph_start));
+  hipCheck(hipEventCreate(&graph_end));
   return hipSuccess;
 }
 
-hipError_t Stream::run_sequence(const RuntimeOptions& runtime_options)
+bool Stream::GraphKey::operator==(const GraphKey& o) const
 {
-  if (runtime_options.number_of_events > host_buffers.host_max_number_of_events)
+  return dev_events == o.dev_events && dev_offsets == o.dev_offsets && dev_work_order == o.dev_work_order &&
+         events_size == o.events_size && n_events == o.n_events && dev_ghost_model == o.dev_ghost_model &&
+         std::memcmp(&tuning, &o.tuning, sizeof(tuning)) == 0;
+}
+
+void Stream::batched_capacities(const RuntimeOptions& ro, HostBuffers& hb)
+{
+  const size_t n = ro.number_of_events;
+  // Every SP word yields at most two clusters (estimate_input_size counts at
+  // most two per SP), and the input holds fewer SP words than bytes / 4.
+  const size_t clusters = ro.host_velopix_events_size / 2 + 1;
+  const size_t tracks = n * VELO_MAX_TRACKS;
+  const size_t track_hits = tracks * VELO_MAX_TRACK_SIZE;
+  if (clusters > 0xFFFFFFFFull || track_hits > 0xFFFFFFFFull)
+    throw std::invalid_argument("batched run: batch too large for 32-bit hit indices");
+  hb.host_number_of_selected_events[0] = (unsigned) n;
+  hb.host_total_number_of_velo_clusters[0] = (unsigned) clusters;
+  hb.host_number_of_reconstructed_velo_tracks[0] = (unsigned) tracks;
+  hb.host_accumulated_number_of_hits_in_velo_tracks[0] = (unsigned) track_hits;
+}
+
+void Stream::drop_graphs()
+{
+  for (auto& g : graphs) {
+    if (g.exec) (void) hipGraphExecDestroy(g.exec);
+    if (g.graph) (void) hipGraphDestroy(g.graph);
+  }
+  graphs.clear();
+}
+
+Stream::GraphEntry& Stream::capture(const RuntimeOptions& ro, const GraphKey& key)
+{
+  if (graphs.size() >= max_graphs) { // evict the least recently used
+    auto lru = std::min_element(graphs.begin(), graphs.end(),
+                                [](const GraphEntry& a, const GraphEntry& b) { return a.last_use < b.last_use; });
+    if (lru->exec) (void) hipGraphExecDestroy(lru->exec);
+    if (lru->graph) (void) hipGraphDestroy(lru->graph);
+    graphs.erase(lru);
+  }
+  batched_capacities(ro, host_buffers);
+  scheduler.reset();
+  hipCheck(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+  constexpr size_t n_steps = std::tuple_size_v<configured_sequence_t>;
+  step_names.assign(n_steps + 1, "");
+  auto hook = [&](size_t i, const char* name) { step_names[i] = name; };
+  GraphEntry entry;
+  entry.key = key;
+  try {
+    Sch::run_sequence(
+      scheduler,
+      sequence_visitor,
+      hook,
+      std::forward_as_tuple(ro, static_cast<const Constants&>(constants), static_cast<const HostBuffers&>(host_buffers)),
+      std::forward_as_tuple(ro, static_cast<const Constants&>(constants), host_buffers, stream, event));
+  } catch (...) {
+    hipGraph_t partial = nullptr;
+    (void) hipStreamEndCapture(stream, &partial);
+    if (partial) (void) hipGraphDestroy(partial);
+    throw;
+  }
+  hipCheck(hipStreamEndCapture(stream, &entry.graph));
+  const hipError_t rc = hipGraphInstantiate(&entry.exec, entry.graph, nullptr, nullptr, 0);
+  if (rc != hipSuccess) {
+    (void) hipGraphDestroy(entry.graph);
+    hipCheck(rc);
+  }
+  entry.layout = scheduler.argument_manager.arguments_tuple;
+  entry.memory_required = scheduler.memory_manager.total_memory_required;
+  graphs.push_back(entry);
+  return graphs.back();
+}
+
+hipError_t Stream::run_batched(const RuntimeOptions& ro)
+{
+  if (!ro.dev_velopix_events || !ro.dev_velopix_event_offsets)
+    throw std::invalid_argument("batched run: the input must be device-resident");
+  GraphKey key;
+  key.dev_events = ro.dev_velopix_events;
+  key.dev_offsets = ro.dev_velopix_event_offsets;
+  key.dev_work_order = ro.dev_work_order;
+  key.dev_ghost_model = ro.dev_ghost_model;
+  key.events_size = ro.host_velopix_events_size;
+  key.n_events = ro.number_of_events;
+  key.tuning = ro.tuning;
+  auto it = std::find_if(graphs.begin(), graphs.end(), [&](const GraphEntry& g) { return g.key == key; });
+  GraphEntry& g = it != graphs.end() ? *it : capture(ro, key);
+  g.last_use = ++graph_clock;
+  scheduler.argument_manager.arguments_tuple = g.layout;
+  const unsigned n = ro.number_of_events;
+  hipCheck(hipEventRecord(graph_start, stream));
+  for (unsigned r = 0; r < ro.number_of_repetitions; ++r) hipCheck(hipGraphLaunch(g.exec, stream));
+  hipCheck(hipEventRecord(graph_end, stream));
+  hipCheck(hipEventSynchronize(graph_end));
+  hipCheck(hipEventElapsedTime(&graph_ms, graph_start, graph_end));
+  if (time_steps) {
+    step_ms.assign(1, graph_ms / (float) ro.number_of_repetitions);
+    step_names.assign(2, "");
+    step_names[0] = "velo_batched_graph";
+    step_names[1] = "end";
+  }
+
+  // totals (the plain sequence reads them between steps)
+  unsigned* hb = host_buffers.host_number_of_selected_events;
+  hb[0] = n;
+  hipCheck(hipMemcpy(hb + 1, output<dev_estimated_input_size>() + (size_t) n * VELO_N_MODULES, sizeof(unsigned),
+                     hipMemcpyDeviceToHost));
+  hipCheck(hipMemcpy(hb + 2, reinterpret_cast<unsigned*>(output<dev_atomics_velo>()) + 2 * (size_t) n,
+                     sizeof(unsigned), hipMemcpyDeviceToHost));
+  hipCheck(hipMemcpy(hb + 3, output<dev_velo_track_hit_number>() + hb[2], sizeof(unsigned), hipMemcpyDeviceToHost));
+  if (ro.copy_outputs_to_host) {
+    const size_t t = hb[2], h = hb[3];
+    host_buffers.ensure_outputs(t, h);
+    hipCheck(hipMemcpy(host_buffers.host_atomics_velo, output<dev_atomics_velo>(), (2 * (size_t) n + 1) * sizeof(unsigned),
+                       hipMemcpyDeviceToHost));
+    hipCheck(hipMemcpy(host_buffers.host_velo_track_hit_number, output<dev_velo_track_hit_number>(),
+                       (t + 1) * sizeof(unsigned), hipMemcpyDeviceToHost));
+    hipCheck(hipMemcpy(host_buffers.host_velo_track_hits, output<dev_velo_track_hits>(), h * 16, hipMemcpyDeviceToHost));
+    hipCheck(hipMemcpy(host_buffers.host_velo_states, output<dev_velo_states>(), t * VELO_STATE_BYTES,
+                       hipMemcpyDeviceToHost));
+    hipCheck(hipMemcpy(host_buffers.host_accepted_velo_tracks, output<dev_accepted_velo_tracks>(), t,
+                       hipMemcpyDeviceToHost));
+  }
+  return hipSuccess;
+}
+
+hipError_t Stream::run_sequence(const RuntimeOptions& options)
+{
+  if (options.number_of_events > host_buffers.host_max_number_of_events)
     throw std::invalid_argument("run_sequence: more events than the stream was initialised for");
+  RuntimeOptions runtime_options = options;
+  runtime_options.dev_work_counters = dev_work_counters;
+  if (runtime_options.batched) return run_batched(runtime_options);
+  // the graphs stay valid: each restores its own layout of the arena
   for (unsigned repetition = 0; repetition < runtime_options.number_of_repetitions; ++repetition) {
     host_buffers.host_number_of_selected_events[0] = runtime_options.number_of_events;
     scheduler.reset();
@@ -58,6 +196,12 @@ hipError_t Stream::run_sequence(const RuntimeOptions& runtime_options)
 
 void Stream::release()
 {
+  drop_graphs();
+  if (dev_work_counters) (void) hipFree(dev_work_counters);
+  dev_work_counters = nullptr;
+  if (graph_start) (void) hipEventDestroy(graph_start);
+  if (graph_end) (void) hipEventDestroy(graph_end);
+  graph_start = graph_end = nullptr;
   for (auto e : step_events)
     if (e) (void) hipEventDestroy(e);
   step_events.clear();

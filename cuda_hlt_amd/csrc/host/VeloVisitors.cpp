@@ -7,11 +7,38 @@
 #include "sequence/SequenceVisitor.h"
 
 namespace {
+// Counter pairs of the persistent steps (two words each, see velo_tuning).
+enum WorkSlot : unsigned {
+  kSlotEstimate,
+  kSlotClustering,
+  kSlotSort,
+  kSlotFill,
+  kSlotSearch,
+  kSlotConsolidate,
+  kNumWorkSlots
+};
+
 template<typename S>
-void tune(S& state, const velo_tuning& t)
+void tune(S& state, const velo_tuning& t, const RuntimeOptions& ro, WorkSlot slot)
 {
-  state.handler.set_tuning(t);
+  velo_tuning x = t;
+  if (x.grid_dim) {
+    if (!ro.dev_work_counters) throw std::invalid_argument("persistent launch without work counters");
+    x.dev_work_counter = ro.dev_work_counters + 2 * slot;
+    if (!x.dev_work_order) x.dev_work_order = ro.dev_work_order;
+  }
+  state.handler.set_tuning(x);
 }
+
+// Host read of one total between steps (the plain sequence's three
+// synchronisation points); batched runs size by bounds instead.
+void read_total(unsigned* host, const unsigned* dev, hipStream_t& stream, hipEvent_t& event)
+{
+  hipCheck(hipMemcpyAsync(host, dev, sizeof(unsigned), hipMemcpyDeviceToHost, stream));
+  hipCheck(hipEventRecord(event, stream));
+  hipCheck(hipEventSynchronize(event));
+}
+static_assert(2 * kNumWorkSlots <= VELO_WORK_COUNTER_WORDS, "work counter words");
 } // namespace
 
 // ---- init_event_list_t ------------------------------------------------------
@@ -89,7 +116,7 @@ void SequenceVisitor::visit<velo_estimate_input_size_t>(
   hipStream_t& stream,
   hipEvent_t&)
 {
-  state.set_opts(dim3(host_buffers.host_number_of_selected_events[0]), dim3(256), stream);
+  state.set_opts(dim3(host_buffers.host_number_of_selected_events[0]), dim3(512), stream);
   state.set_arguments(
     arguments.offset<dev_velo_raw_input>(),
     arguments.offset<dev_velo_raw_input_offsets>(),
@@ -101,7 +128,7 @@ void SequenceVisitor::visit<velo_estimate_input_size_t>(
     arguments.offset<dev_event_order>(),
     constants.dev_velo_candidate_ks,
     arguments.offset<dev_velo_event_status>());
-  tune(state, runtime_options.tuning.estimate_input_size);
+  tune(state, runtime_options.tuning.estimate_input_size, runtime_options, kSlotEstimate);
   state.invoke();
 }
 
@@ -121,7 +148,7 @@ template<>
 void SequenceVisitor::visit<prefix_sum_velo_clusters_t>(
   prefix_sum_velo_clusters_t& state,
   const prefix_sum_velo_clusters_t::arguments_t& arguments,
-  const RuntimeOptions&,
+  const RuntimeOptions& runtime_options,
   const Constants&,
   HostBuffers& host_buffers,
   hipStream_t& stream,
@@ -132,11 +159,9 @@ void SequenceVisitor::visit<prefix_sum_velo_clusters_t>(
   state.set_opts(stream);
   state.set_arguments(arguments.offset<dev_estimated_input_size>(), arguments.offset<dev_cluster_offset>());
   state.invoke();
-  hipCheck(hipMemcpyAsync(
-    host_buffers.host_total_number_of_velo_clusters, arguments.offset<dev_estimated_input_size>() + n * VELO_N_MODULES,
-    sizeof(unsigned), hipMemcpyDeviceToHost, stream));
-  hipCheck(hipEventRecord(event, stream));
-  hipCheck(hipEventSynchronize(event));
+  if (!runtime_options.batched)
+    read_total(host_buffers.host_total_number_of_velo_clusters,
+               arguments.offset<dev_estimated_input_size>() + n * VELO_N_MODULES, stream, event);
 }
 
 // ---- velo_masked_clustering_t -------------------------------------------------
@@ -160,7 +185,7 @@ void SequenceVisitor::visit<velo_masked_clustering_t>(
   hipStream_t& stream,
   hipEvent_t&)
 {
-  state.set_opts(dim3(host_buffers.host_number_of_selected_events[0]), dim3(256), stream);
+  state.set_opts(dim3(host_buffers.host_number_of_selected_events[0]), dim3(512), stream);
   state.set_arguments(
     arguments.offset<dev_velo_raw_input>(),
     arguments.offset<dev_velo_raw_input_offsets>(),
@@ -176,7 +201,7 @@ void SequenceVisitor::visit<velo_masked_clustering_t>(
     constants.dev_velo_sp_fx,
     constants.dev_velo_sp_fy,
     arguments.offset<dev_velo_event_status>());
-  tune(state, runtime_options.tuning.masked_clustering);
+  tune(state, runtime_options.tuning.masked_clustering, runtime_options, kSlotClustering);
   state.invoke();
 }
 
@@ -207,7 +232,7 @@ void SequenceVisitor::visit<velo_calculate_phi_and_sort_t>(
     arguments.offset<dev_module_cluster_num>(),
     arguments.offset<dev_velo_cluster_container>(),
     arguments.offset<dev_hit_permutation>());
-  tune(state, runtime_options.tuning.calculate_phi_and_sort);
+  tune(state, runtime_options.tuning.calculate_phi_and_sort, runtime_options, kSlotSort);
   state.invoke();
 }
 
@@ -233,14 +258,14 @@ void SequenceVisitor::visit<velo_fill_candidates_t>(
   hipStream_t& stream,
   hipEvent_t&)
 {
-  state.set_opts(dim3(host_buffers.host_number_of_selected_events[0]), dim3(256), stream);
+  state.set_opts(dim3(host_buffers.host_number_of_selected_events[0]), dim3(512), stream);
   state.set_arguments(
     arguments.offset<dev_velo_cluster_container>(),
     arguments.offset<dev_estimated_input_size>(),
     arguments.offset<dev_module_cluster_num>(),
     arguments.offset<dev_h0_candidates>(),
     arguments.offset<dev_h2_candidates>());
-  tune(state, runtime_options.tuning.fill_candidates);
+  tune(state, runtime_options.tuning.fill_candidates, runtime_options, kSlotFill);
   state.invoke();
 }
 
@@ -288,7 +313,7 @@ void SequenceVisitor::visit<velo_search_by_triplet_t>(
     arguments.offset<dev_rel_indices>(),
     constants.dev_velo_module_zs,
     arguments.offset<dev_velo_event_status>());
-  tune(state, runtime_options.tuning.search_by_triplet);
+  tune(state, runtime_options.tuning.search_by_triplet, runtime_options, kSlotSearch);
   state.invoke();
 }
 
@@ -324,7 +349,7 @@ template<>
 void SequenceVisitor::visit<copy_and_prefix_sum_single_block_velo_t>(
   copy_and_prefix_sum_single_block_velo_t& state,
   const copy_and_prefix_sum_single_block_velo_t::arguments_t& arguments,
-  const RuntimeOptions&,
+  const RuntimeOptions& runtime_options,
   const Constants&,
   HostBuffers& host_buffers,
   hipStream_t& stream,
@@ -335,11 +360,8 @@ void SequenceVisitor::visit<copy_and_prefix_sum_single_block_velo_t>(
   state.set_opts(dim3(1), dim3(1024), stream);
   state.set_arguments(atomics + 2 * n, atomics, atomics + n, n);
   state.invoke();
-  hipCheck(hipMemcpyAsync(
-    host_buffers.host_number_of_reconstructed_velo_tracks, atomics + 2 * n, sizeof(unsigned), hipMemcpyDeviceToHost,
-    stream));
-  hipCheck(hipEventRecord(event, stream));
-  hipCheck(hipEventSynchronize(event));
+  if (!runtime_options.batched)
+    read_total(host_buffers.host_number_of_reconstructed_velo_tracks, atomics + 2 * n, stream, event);
 }
 
 // ---- copy_velo_track_hit_number_t ------------------------------------------------
@@ -385,7 +407,7 @@ template<>
 void SequenceVisitor::visit<prefix_sum_velo_track_hit_number_t>(
   prefix_sum_velo_track_hit_number_t& state,
   const prefix_sum_velo_track_hit_number_t::arguments_t& arguments,
-  const RuntimeOptions&,
+  const RuntimeOptions& runtime_options,
   const Constants&,
   HostBuffers& host_buffers,
   hipStream_t& stream,
@@ -395,12 +417,15 @@ void SequenceVisitor::visit<prefix_sum_velo_track_hit_number_t>(
   state.set_size(t);
   state.set_opts(stream);
   state.set_arguments(arguments.offset<dev_velo_track_hit_number>(), arguments.offset<dev_prefix_sum_auxiliary_array_2>());
+  if (runtime_options.batched) {
+    // t is the capacity; the track total sits at dev_atomics_velo[2N]
+    const unsigned n = host_buffers.host_number_of_selected_events[0];
+    state.set_device_count(reinterpret_cast<const uint32_t*>(arguments.offset<dev_atomics_velo>()) + 2 * n);
+  }
   state.invoke();
-  hipCheck(hipMemcpyAsync(
-    host_buffers.host_accumulated_number_of_hits_in_velo_tracks, arguments.offset<dev_velo_track_hit_number>() + t,
-    sizeof(unsigned), hipMemcpyDeviceToHost, stream));
-  hipCheck(hipEventRecord(event, stream));
-  hipCheck(hipEventSynchronize(event));
+  if (!runtime_options.batched)
+    read_total(host_buffers.host_accumulated_number_of_hits_in_velo_tracks,
+               arguments.offset<dev_velo_track_hit_number>() + t, stream, event);
 }
 
 // ---- consolidate_velo_tracks_t ------------------------------------------------------
@@ -428,7 +453,7 @@ void SequenceVisitor::visit<consolidate_velo_tracks_t>(
   hipEvent_t&)
 {
   const unsigned n = host_buffers.host_number_of_selected_events[0];
-  state.set_opts(dim3(n), dim3(256), stream);
+  state.set_opts(dim3(n), dim3(512), stream);
   state.set_arguments(
     arguments.offset<dev_atomics_velo>(),
     arguments.offset<dev_tracks>(),
@@ -438,10 +463,12 @@ void SequenceVisitor::visit<consolidate_velo_tracks_t>(
     arguments.offset<dev_module_cluster_num>(),
     arguments.offset<dev_velo_track_hits>(),
     arguments.offset<dev_velo_states>(),
-    arguments.offset<dev_accepted_velo_tracks>());
-  tune(state, runtime_options.tuning.consolidate);
+    arguments.offset<dev_accepted_velo_tracks>(),
+    runtime_options.dev_ghost_model,
+    static_cast<float*>(nullptr));
+  tune(state, runtime_options.tuning.consolidate, runtime_options, kSlotConsolidate);
   state.invoke();
-  if (!runtime_options.copy_outputs_to_host) return;
+  if (!runtime_options.copy_outputs_to_host || runtime_options.batched) return; // batched: copied after the graph
   // ConsolidateVeloVisitor.cu:47-66: device to host
   const size_t t = host_buffers.host_number_of_reconstructed_velo_tracks[0];
   const size_t h = host_buffers.host_accumulated_number_of_hits_in_velo_tracks[0];
@@ -453,5 +480,7 @@ void SequenceVisitor::visit<consolidate_velo_tracks_t>(
   hipCheck(hipMemcpyAsync(host_buffers.host_velo_track_hits, arguments.offset<dev_velo_track_hits>(), h * sizeof(Velo::Hit),
                           hipMemcpyDeviceToHost, stream));
   hipCheck(hipMemcpyAsync(host_buffers.host_velo_states, arguments.offset<dev_velo_states>(), t * VELO_STATE_BYTES,
+                          hipMemcpyDeviceToHost, stream));
+  hipCheck(hipMemcpyAsync(host_buffers.host_accepted_velo_tracks, arguments.offset<dev_accepted_velo_tracks>(), t,
                           hipMemcpyDeviceToHost, stream));
 }

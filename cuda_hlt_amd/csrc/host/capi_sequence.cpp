@@ -78,6 +78,9 @@ int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, char* err
   ro.tuning.fill_candidates = in->tuning[3];
   ro.tuning.search_by_triplet = in->tuning[4];
   ro.tuning.consolidate = in->tuning[5];
+  ro.batched = in->batched != 0;
+  ro.dev_work_order = in->dev_work_order;
+  ro.dev_ghost_model = in->dev_ghost_model;
   try {
     s->stream->run_sequence(ro);
   }
@@ -104,6 +107,7 @@ int velo_sequence_output_get(const velo_sequence* s, velo_sequence_output* out)
     out->host_track_hit_number = hb.host_velo_track_hit_number;
     out->host_track_hits = hb.host_velo_track_hits;
     out->host_states = hb.host_velo_states;
+    out->host_accepted = hb.host_accepted_velo_tracks;
   }
   out->dev_atomics_velo = st.output<dev_atomics_velo>();
   out->dev_track_hit_number = st.output<dev_velo_track_hit_number>();

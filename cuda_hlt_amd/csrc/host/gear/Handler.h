@@ -45,6 +45,10 @@ struct Handler {
   {
     if (t.block_dim) args.tuning.block_dim = t.block_dim;
     if (t.lds_tile) args.tuning.lds_tile = t.lds_tile;
+    args.tuning.grid_dim = t.grid_dim;
+    args.tuning.dev_work_counter = t.dev_work_counter;
+    args.tuning.dev_work_order = t.dev_work_order;
+    args.tuning.dev_count = t.dev_count;
   }
 
   template<typename... T>
@@ -114,6 +118,8 @@ Handler<ArgsT> make_handler(int (*f)(const ArgsT*, void*))
       auxiliary_array_size = aux_array_size(n);                                                     \
     }                                                                                               \
     void set_opts(hipStream_t& s) { stream = &s; }                                                  \
+    /* element count read on the device (array_size is then the capacity) */                      \
+    void set_device_count(const uint32_t* c) { args.tuning.dev_count = c; }                        \
     template<typename T>                                                                            \
     void set_arguments(T array, T auxiliary_array)                                                  \
     {                                                                                               \

@@ -21,7 +21,8 @@ struct HostBuffers {
   unsigned* host_velo_track_hit_number = nullptr;
   char* host_velo_track_hits = nullptr;
   char* host_velo_states = nullptr;
-  size_t cap_track_hit_number = 0, cap_track_hits = 0, cap_states = 0;
+  uint8_t* host_accepted_velo_tracks = nullptr;
+  size_t cap_track_hit_number = 0, cap_track_hits = 0, cap_states = 0, cap_accepted = 0;
 
   void reserve(unsigned max_number_of_events)
   {
@@ -52,6 +53,7 @@ struct HostBuffers {
     grow(reinterpret_cast<char**>(&host_velo_track_hit_number), &cap_track_hit_number, (tracks + 1) * sizeof(unsigned));
     grow(&host_velo_track_hits, &cap_track_hits, hits * 16 + 16);
     grow(&host_velo_states, &cap_states, tracks * VELO_STATE_BYTES + 64);
+    grow(reinterpret_cast<char**>(&host_accepted_velo_tracks), &cap_accepted, tracks + 64);
   }
 
   size_t velo_track_hit_number_size() const { return host_number_of_reconstructed_velo_tracks[0] + 1; }
@@ -60,7 +62,7 @@ struct HostBuffers {
   {
     for (void* p : {(void*) host_number_of_selected_events, (void*) host_event_list, (void*) host_atomics_velo,
                     (void*) host_velo_event_status, (void*) host_velo_track_hit_number, (void*) host_velo_track_hits,
-                    (void*) host_velo_states})
+                    (void*) host_velo_states, (void*) host_accepted_velo_tracks})
       if (p) (void) hipHostFree(p);
     host_number_of_selected_events = nullptr;
     host_event_list = nullptr;
@@ -69,6 +71,7 @@ struct HostBuffers {
     host_velo_track_hit_number = nullptr;
     host_velo_track_hits = nullptr;
     host_velo_states = nullptr;
-    cap_track_hit_number = cap_track_hits = cap_states = 0;
+    host_accepted_velo_tracks = nullptr;
+    cap_track_hit_number = cap_track_hits = cap_states = cap_accepted = 0;
   }
 };

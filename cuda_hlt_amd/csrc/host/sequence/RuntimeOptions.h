@@ -9,6 +9,9 @@
 
 #include "velo_hlt.h"
 
+// Device words the Stream keeps for the persistent steps' work counters.
+constexpr unsigned VELO_WORK_COUNTER_WORDS = 16;
+
 struct VeloTuningOptions {
   velo_tuning estimate_input_size {};
   velo_tuning masked_clustering {};
@@ -29,4 +32,17 @@ struct RuntimeOptions {
   unsigned number_of_repetitions = 1;
   bool copy_outputs_to_host = true;
   VeloTuningOptions tuning {};
+  // Batched runs: no host reads between steps (buffers are sized by upper
+  // bounds from the input size and the per-event capacities), the whole
+  // sequence is captured once in a hipGraph and replayed; needs
+  // device-resident input.
+  bool batched = false;
+  // Persistent launches (tuning.grid_dim > 0): processing order of the
+  // events (largest first balances uneven events), NULL = identity.
+  const unsigned* dev_work_order = nullptr;
+  // Ghost filter run inline by consolidation (a device velo_ghost_model),
+  // NULL = every track accepted.
+  const void* dev_ghost_model = nullptr;
+  // Set by the Stream: two counter words per persistent step.
+  unsigned* dev_work_counters = nullptr;
 };

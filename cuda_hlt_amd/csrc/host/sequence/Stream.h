@@ -26,6 +26,39 @@ struct Stream {
   Constants constants;
   SequenceVisitor sequence_visitor;
 
+  // Work counters of persistent launches (zeroed once; every persistent
+  // launch leaves its pair at zero).
+  unsigned* dev_work_counters = nullptr;
+
+  // Batched runs: the captured sequence and what it was captured for.
+  struct GraphKey {
+    const char* dev_events = nullptr;
+    const unsigned* dev_offsets = nullptr;
+    const unsigned* dev_work_order = nullptr;
+    const void* dev_ghost_model = nullptr;
+    size_t events_size = 0;
+    unsigned n_events = 0;
+    VeloTuningOptions tuning {};
+    bool operator==(const GraphKey& o) const;
+  };
+  // One captured graph per key (batch shape), with the argument layout it
+  // was captured on; replaying a graph restores its layout so output<A>()
+  // points at that run's buffers.  All graphs share the arena: one runs at
+  // a time on this stream.
+  struct GraphEntry {
+    GraphKey key;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    typename scheduler_t::arguments_tuple_t layout;
+    size_t memory_required = 0;
+    unsigned long long last_use = 0;
+  };
+  static constexpr size_t max_graphs = 16;
+  std::vector<GraphEntry> graphs;
+  unsigned long long graph_clock = 0;
+  hipEvent_t graph_start = nullptr, graph_end = nullptr;
+  float graph_ms = 0.f;
+
   // Optional per-step timing with HIP events recorded on `stream` before
   // every step (for the benchmark and profiling); times of the last run.
   bool time_steps = false;
@@ -47,6 +80,9 @@ struct Stream {
 
   hipError_t run_sequence(const RuntimeOptions& runtime_options);
 
+  // Capacities a batched run sizes its buffers by (no host reads).
+  static void batched_capacities(const RuntimeOptions& ro, HostBuffers& hb);
+
   template<typename A>
   typename A::type* output() const
   {
@@ -54,4 +90,9 @@ struct Stream {
   }
 
   void release();
+
+private:
+  hipError_t run_batched(const RuntimeOptions& runtime_options);
+  GraphEntry& capture(const RuntimeOptions& runtime_options, const GraphKey& key);
+  void drop_graphs();
 };

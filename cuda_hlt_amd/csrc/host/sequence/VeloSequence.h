@@ -104,7 +104,8 @@ ALGORITHM(
 
 PREFIX_SUM_ALGORITHM(
   prefix_sum_velo_track_hit_number_t,
-  ARGUMENTS(dev_velo_track_hit_number, dev_prefix_sum_auxiliary_array_2))
+  // dev_atomics_velo: batched runs read the track total on the device
+  ARGUMENTS(dev_velo_track_hit_number, dev_prefix_sum_auxiliary_array_2, dev_atomics_velo))
 
 ALGORITHM(
   velo_consolidate_velo_tracks_launch,

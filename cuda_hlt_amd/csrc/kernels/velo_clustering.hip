@@ -80,6 +80,88 @@ __device__ inline void load_window(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Neighbour lookups in a bank whose SP addresses are strictly increasing
+// (checked at run time; other banks use the linear scans above).  They
+// return exactly what the linear scans return for such a bank: with unique
+// addresses every neighbour position matches at most one SP.
+// ---------------------------------------------------------------------------
+__device__ inline uint32_t sp_address(uint32_t w) { return (w & 0x007FFF00u) >> 8; }
+
+template<typename P>
+__device__ inline uint32_t lower_bound_address(P words, uint32_t n, uint32_t key)
+{
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (sp_address(words[mid]) < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// neighbour_pixels for the SP at index i of a sorted bank
+template<typename P>
+__device__ inline uint32_t neighbour_pixels_sorted(uint32_t w, uint32_t i, P words, uint32_t n)
+{
+  const uint32_t addr = sp_address(w);
+  const uint32_t sp = w & 0xFFu;
+  uint32_t pixels = (sp & 0x0Fu) | ((sp & 0xF0u) << 2);
+  const uint32_t row = addr & 0x3Fu;
+  if (i > 0 && row > 0) { // bottom: (col, row - 1)
+    const uint32_t o = words[i - 1];
+    if (sp_address(o) == addr - 1u) pixels |= ((o & 0xFFu) & 0x80u) >> 2;
+  }
+  if (i + 1 < n && row < 63) { // top: (col, row + 1)
+    const uint32_t o = words[i + 1];
+    if (sp_address(o) == addr + 1u) {
+      const uint32_t osp = o & 0xFFu;
+      pixels |= ((osp & 0x01u) | ((osp & 0x10u) << 2)) << 4;
+    }
+  }
+  // right column: right-bottom (+63), right (+64), top-right (+65)
+  for (uint32_t j = lower_bound_address(words, n, addr + 63u); j < n; ++j) {
+    const uint32_t o = words[j];
+    const uint32_t oa = sp_address(o);
+    if (oa > addr + 65u) break;
+    const uint32_t osp = o & 0xFFu;
+    if (oa == addr + 63u && row > 0) pixels |= (osp & 0x08u) << 8;
+    if (oa == addr + 64u) pixels |= (osp & 0x0Fu) << 12;
+    if (oa == addr + 65u && row < 63) pixels |= (osp & 0x01u) << 16;
+  }
+  return pixels;
+}
+
+// load_window for a sorted bank: three address ranges, one per SP column
+template<typename P>
+__device__ inline void load_window_sorted(int32_t sp_row, int32_t sp_col, P words, uint32_t n, uint32_t pa[3])
+{
+  pa[0] = pa[1] = pa[2] = 0;
+  const int32_t row_lo = sp_row - 2;
+  const int32_t r0 = row_lo < 0 ? 0 : row_lo;
+  const int32_t r1 = sp_row + 1 > 63 ? 63 : sp_row + 1;
+#pragma unroll
+  for (int32_t rc = 0; rc < 3; ++rc) {
+    const int32_t col = sp_col - 1 + rc;
+    if (col < 0 || col > 511) continue;
+    const uint32_t key_lo = ((uint32_t) col << 6) | (uint32_t) r0;
+    const uint32_t key_hi = ((uint32_t) col << 6) | (uint32_t) r1;
+    uint32_t bits_acc = 0;
+    for (uint32_t j = lower_bound_address(words, n, key_lo); j < n; ++j) {
+      const uint32_t o = words[j];
+      const uint32_t oa = sp_address(o);
+      if (oa > key_hi) break;
+      if (o & 0x80000000u) continue;
+      const int32_t rr = (int32_t) (oa & 0x3Fu) - row_lo;
+      const uint32_t osp = o & 0xFFu;
+      bits_acc |= ((osp & 0x0Fu) << (4 * rr)) | ((osp & 0xF0u) << (12 + 4 * rr));
+    }
+    if (rc == 0) pa[0] = bits_acc;
+    else if (rc == 1) pa[1] = bits_acc;
+    else pa[2] = bits_acc;
+  }
+}
+
 struct Hit {
   float x, y, z;
   uint32_t id;
@@ -205,6 +287,7 @@ struct ModuleBanks {
   uint32_t n0, n1, n2, n3; // SP counts (0 for a missing or mislabelled bank)
   uint32_t p1, p2, p3, total;
   uint32_t status;
+  uint32_t sorted; // bit k: bank 4m+k has strictly increasing SP addresses
 };
 
 template<typename P>
@@ -238,7 +321,30 @@ __device__ inline ModuleBanks module_banks(P ev, uint32_t n_banks, uint32_t m)
   mb.p2 = mb.p1 + mb.n1;
   mb.p3 = mb.p2 + mb.n2;
   mb.total = mb.p3 + mb.n3;
+  mb.sorted = 0xFu;
   return mb;
+}
+
+__device__ inline uint32_t bank_slot(const ModuleBanks& mb, uint32_t j, uint32_t* k);
+
+// Clears mb.sorted bit k for every bank whose SP addresses do not strictly
+// increase (wave-wide; uniform result).
+template<typename P>
+__device__ inline void check_sorted(P ev, ModuleBanks& mb)
+{
+  uint32_t bad = 0;
+  for (uint32_t jb = 0; jb < mb.total; jb += kWave) {
+    const uint32_t j = jb + lane_id();
+    if (j < mb.total) {
+      uint32_t k;
+      const uint32_t i = bank_slot(mb, j, &k);
+      const uint32_t base = sel4(k, mb.b0, mb.b1, mb.b2, mb.b3);
+      const uint32_t n = sel4(k, mb.n0, mb.n1, mb.n2, mb.n3);
+      if (i + 1 < n && sp_address(ev[base + i]) >= sp_address(ev[base + i + 1])) bad |= 1u << k;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) bad |= __shfl_xor(bad, o, 64);
+  mb.sorted &= ~bad;
 }
 
 // Position j of the module's SP list -> (bank k, index in bank).
@@ -291,7 +397,8 @@ __device__ __forceinline__ void estimate_event(
 
   // pass 1: estimates and candidate counts per module
   for (uint32_t m = wave; m < VELO_N_MODULES; m += waves) {
-    const ModuleBanks mb = module_banks(ev, n_banks, m);
+    ModuleBanks mb = module_banks(ev, n_banks, m);
+    check_sorted(ev, mb);
     status |= mb.status;
     uint32_t est = 0, ncand = 0;
     for (uint32_t j = lane; j < mb.total; j += kWave) {
@@ -303,7 +410,9 @@ __device__ __forceinline__ void estimate_event(
         est += isolated_cluster_count(w & 0xFFu);
       }
       else {
-        const uint32_t c = candidates_from_pixels(neighbour_pixels(w, ev + base, sel4(k, mb.n0, mb.n1, mb.n2, mb.n3)));
+        const uint32_t n = sel4(k, mb.n0, mb.n1, mb.n2, mb.n3);
+        const uint32_t c = candidates_from_pixels(((mb.sorted >> k) & 1u) ? neighbour_pixels_sorted(w, i, ev + base, n)
+                                                                          : neighbour_pixels(w, ev + base, n));
         const uint32_t f = ((c & 0x0Fu) ? 1u : 0u) + ((c & 0xF0u) ? 1u : 0u);
         est += f;
         ncand += f;
@@ -330,7 +439,8 @@ __device__ __forceinline__ void estimate_event(
   uint32_t* cands = a.dev_cluster_candidates + (size_t) e * VELO_MAX_CANDIDATES_EVENT;
   for (uint32_t m = wave; m < VELO_N_MODULES; m += waves) {
     if (module_cands[m] == 0) continue;
-    const ModuleBanks mb = module_banks(ev, n_banks, m);
+    ModuleBanks mb = module_banks(ev, n_banks, m);
+    check_sorted(ev, mb);
     uint32_t pos = module_cand_offset[m];
     for (uint32_t jb = 0; jb < mb.total; jb += kWave) {
       const uint32_t j = jb + lane;
@@ -339,8 +449,10 @@ __device__ __forceinline__ void estimate_event(
         i = bank_slot(mb, j, &k);
         const uint32_t base = sel4(k, mb.b0, mb.b1, mb.b2, mb.b3);
         const uint32_t w = ev[base + i];
+        const uint32_t n = sel4(k, mb.n0, mb.n1, mb.n2, mb.n3);
         if (!(w & 0x80000000u))
-          c = candidates_from_pixels(neighbour_pixels(w, ev + base, sel4(k, mb.n0, mb.n1, mb.n2, mb.n3)));
+          c = candidates_from_pixels(((mb.sorted >> k) & 1u) ? neighbour_pixels_sorted(w, i, ev + base, n)
+                                                             : neighbour_pixels(w, ev + base, n));
       }
       const uint32_t cnt = ((c & 0x0Fu) ? 1u : 0u) + ((c & 0xF0u) ? 1u : 0u);
       uint32_t total;
@@ -374,21 +486,22 @@ __global__ void __launch_bounds__(1024) estimate_input_size_kernel(velo_estimate
   __shared__ uint32_t module_cands[VELO_N_MODULES];
   __shared__ uint32_t module_cand_offset[VELO_N_MODULES + 1];
   __shared__ uint32_t block_status;
-  const uint32_t e = blockIdx.x;
-  const uint32_t ei = event_index(a.dev_event_list, e);
-  const uint32_t off = a.dev_raw_input_offsets[ei];
-  const uint32_t nwords = (a.dev_raw_input_offsets[ei + 1] - off) / 4u;
-  const uint32_t* gev = reinterpret_cast<const uint32_t*>(a.dev_raw_input + off);
-  if (threadIdx.x == 0) block_status = 0;
-  if (nwords <= a.tuning.lds_tile) {
-    stage_words(lds, gev, nwords);
-    __syncthreads();
-    estimate_event(static_cast<const uint32_t*>(lds), a, e, module_cands, module_cand_offset, &block_status);
-  }
-  else {
-    __syncthreads();
-    estimate_event(gev, a, e, module_cands, module_cand_offset, &block_status);
-  }
+  for_each_event(a.tuning, a.n_events, [&](const uint32_t e) {
+    const uint32_t ei = event_index(a.dev_event_list, e);
+    const uint32_t off = a.dev_raw_input_offsets[ei];
+    const uint32_t nwords = (a.dev_raw_input_offsets[ei + 1] - off) / 4u;
+    const uint32_t* gev = reinterpret_cast<const uint32_t*>(a.dev_raw_input + off);
+    if (threadIdx.x == 0) block_status = 0;
+    if (nwords <= a.tuning.lds_tile) {
+      stage_words(lds, gev, nwords);
+      __syncthreads();
+      estimate_event(static_cast<const uint32_t*>(lds), a, e, module_cands, module_cand_offset, &block_status);
+    }
+    else {
+      __syncthreads();
+      estimate_event(gev, a, e, module_cands, module_cand_offset, &block_status);
+    }
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -411,7 +524,8 @@ __device__ __forceinline__ void cluster_event(P ev, const velo_masked_clustering
   uint32_t event_hits = 0;
 
   for (uint32_t m = wave; m < VELO_N_MODULES; m += waves) {
-    const ModuleBanks mb = module_banks(ev, n_banks, m);
+    ModuleBanks mb = module_banks(ev, n_banks, m);
+    check_sorted(ev, mb);
     const uint32_t mstart = starts[m];
     uint32_t pos = 0;
     for (uint32_t jb = 0; jb < mb.total; jb += kWave) {
@@ -441,7 +555,9 @@ __device__ __forceinline__ void cluster_event(P ev, const velo_masked_clustering
           }
         }
         else {
-          const uint32_t c = candidates_from_pixels(neighbour_pixels(w, ev + base, n));
+          const bool sorted = (mb.sorted >> k) & 1u;
+          const uint32_t c = candidates_from_pixels(sorted ? neighbour_pixels_sorted(w, i, ev + base, n)
+                                                           : neighbour_pixels(w, ev + base, n));
           if (c) {
             // the reference decodes the candidate's SP index as uint8_t (:179)
             const uint32_t ci = i & 0xFFu;
@@ -449,7 +565,8 @@ __device__ __forceinline__ void cluster_event(P ev, const velo_masked_clustering
             const uint32_t caddr = (cw & 0x007FFF00u) >> 8;
             const int32_t sp_row = (int32_t) (caddr & 0x3Fu), sp_col = (int32_t) (caddr >> 6);
             uint32_t pa[3];
-            load_window(sp_row, sp_col, ev + base, n, pa);
+            if (sorted) load_window_sorted(sp_row, sp_col, ev + base, n, pa);
+            else load_window(sp_row, sp_col, ev + base, n, pa);
             Hit t;
             if ((c & 0x0Fu) && cluster_candidate(pa[0], pa[1], pa[2], sp_row, sp_col, c_candidate_ks[c & 0x0Fu],
                                                  a.dev_velo_geometry, sensor, &t)) {
@@ -500,21 +617,22 @@ __global__ void __launch_bounds__(1024) masked_clustering_kernel(velo_masked_clu
 {
   extern __shared__ uint32_t lds[];
   __shared__ uint32_t block_words[2];
-  const uint32_t e = blockIdx.x;
-  const uint32_t ei = event_index(a.dev_event_list, e);
-  const uint32_t off = a.dev_raw_input_offsets[ei];
-  const uint32_t nwords = (a.dev_raw_input_offsets[ei + 1] - off) / 4u;
-  const uint32_t* gev = reinterpret_cast<const uint32_t*>(a.dev_raw_input + off);
-  if (threadIdx.x < 2) block_words[threadIdx.x] = 0;
-  if (nwords <= a.tuning.lds_tile) {
-    stage_words(lds, gev, nwords);
-    __syncthreads();
-    cluster_event(static_cast<const uint32_t*>(lds), a, e, block_words);
-  }
-  else {
-    __syncthreads();
-    cluster_event(gev, a, e, block_words);
-  }
+  for_each_event(a.tuning, a.n_events, [&](const uint32_t e) {
+    const uint32_t ei = event_index(a.dev_event_list, e);
+    const uint32_t off = a.dev_raw_input_offsets[ei];
+    const uint32_t nwords = (a.dev_raw_input_offsets[ei + 1] - off) / 4u;
+    const uint32_t* gev = reinterpret_cast<const uint32_t*>(a.dev_raw_input + off);
+    if (threadIdx.x < 2) block_words[threadIdx.x] = 0;
+    if (nwords <= a.tuning.lds_tile) {
+      stage_words(lds, gev, nwords);
+      __syncthreads();
+      cluster_event(static_cast<const uint32_t*>(lds), a, e, block_words);
+    }
+    else {
+      __syncthreads();
+      cluster_event(gev, a, e, block_words);
+    }
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -653,12 +771,13 @@ int velo_estimate_input_size_launch(const velo_estimate_input_size_args* in, voi
       !in->dev_module_cluster_num || !in->dev_event_candidate_num || !in->dev_cluster_candidates)
     return (int) hipErrorInvalidValue;
   if (in->n_events == 0) return 0;
+  if (!schedule_ok(in->tuning)) return (int) hipErrorInvalidValue;
   velo_estimate_input_size_args a = *in;
   const uint32_t block = block_of(a.tuning);
   if (!block) return (int) hipErrorInvalidValue;
   if (!a.tuning.lds_tile) a.tuning.lds_tile = kDefaultEventWords;
   const size_t shmem = (size_t) a.tuning.lds_tile * sizeof(uint32_t);
-  hipLaunchKernelGGL(estimate_input_size_kernel, dim3(a.n_events), dim3(block), shmem, (hipStream_t) stream, a);
+  hipLaunchKernelGGL(estimate_input_size_kernel, dim3(event_grid(a.tuning, a.n_events)), dim3(block), shmem, (hipStream_t) stream, a);
   return (int) hipGetLastError();
 }
 
@@ -669,12 +788,13 @@ int velo_masked_clustering_launch(const velo_masked_clustering_args* in, void* s
       !in->dev_velo_sp_patterns || !in->dev_velo_sp_fx || !in->dev_velo_sp_fy)
     return (int) hipErrorInvalidValue;
   if (in->n_events == 0) return 0;
+  if (!schedule_ok(in->tuning)) return (int) hipErrorInvalidValue;
   velo_masked_clustering_args a = *in;
   const uint32_t block = block_of(a.tuning);
   if (!block) return (int) hipErrorInvalidValue;
   if (!a.tuning.lds_tile) a.tuning.lds_tile = kDefaultEventWords;
   const size_t shmem = (size_t) a.tuning.lds_tile * sizeof(uint32_t);
-  hipLaunchKernelGGL(masked_clustering_kernel, dim3(a.n_events), dim3(block), shmem, (hipStream_t) stream, a);
+  hipLaunchKernelGGL(masked_clustering_kernel, dim3(event_grid(a.tuning, a.n_events)), dim3(block), shmem, (hipStream_t) stream, a);
   return (int) hipGetLastError();
 }
 

@@ -22,101 +22,102 @@ __device__ inline float module_phi(uint32_t m, float x, float y) { return (m & 1
 __global__ void __launch_bounds__(1024) calculate_phi_and_sort_kernel(velo_calculate_phi_and_sort_args a)
 {
   extern __shared__ uint32_t lds[];
-  const uint32_t e = blockIdx.x;
-  const uint32_t waves = blockDim.x / kWave;
-  const uint32_t wave = threadIdx.x / kWave;
-  const uint32_t lane = lane_id();
-  const uint32_t total = a.dev_module_cluster_start[VELO_N_MODULES * a.n_events];
-  uint32_t* X = a.dev_velo_cluster_container;
-  uint32_t* Y = X + total;
-  uint32_t* Z = Y + total;
-  uint32_t* ID = Z + total;
-  uint32_t* PHI = ID + total;
-  uint32_t* TMP = PHI + total;
-  float* phi = reinterpret_cast<float*>(lds) + wave * kSortCap;
-  uint16_t* perm = reinterpret_cast<uint16_t*>(lds + waves * kSortCap) + wave * kSortCap;
+  for_each_event(a.tuning, a.n_events, [&](const uint32_t e) {
+    const uint32_t waves = blockDim.x / kWave;
+    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t lane = lane_id();
+    const uint32_t total = a.dev_module_cluster_start[VELO_N_MODULES * a.n_events];
+    uint32_t* X = a.dev_velo_cluster_container;
+    uint32_t* Y = X + total;
+    uint32_t* Z = Y + total;
+    uint32_t* ID = Z + total;
+    uint32_t* PHI = ID + total;
+    uint32_t* TMP = PHI + total;
+    float* phi = reinterpret_cast<float*>(lds) + wave * kSortCap;
+    uint16_t* perm = reinterpret_cast<uint16_t*>(lds + waves * kSortCap) + wave * kSortCap;
 
-  for (uint32_t m = wave; m < VELO_N_MODULES; m += waves) {
-    const uint32_t s = a.dev_module_cluster_start[e * VELO_N_MODULES + m];
-    const uint32_t n = a.dev_module_cluster_num[e * VELO_N_MODULES + m];
-    // Slots estimated but not filled ("holes") read as NaN in every sorted
-    // slot, so no search can ever select them (the reference leaves them
-    // uninitialised).
-    const uint32_t hole_end = a.dev_module_cluster_start[e * VELO_N_MODULES + m + 1];
-    for (uint32_t p = s + n + lane; p < hole_end; p += kWave) {
-      X[p] = kHoleBits;
-      Y[p] = kHoleBits;
-      Z[p] = kHoleBits;
-      PHI[p] = kHoleBits;
-      TMP[p] = kHoleBits;
+    for (uint32_t m = wave; m < VELO_N_MODULES; m += waves) {
+      const uint32_t s = a.dev_module_cluster_start[e * VELO_N_MODULES + m];
+      const uint32_t n = a.dev_module_cluster_num[e * VELO_N_MODULES + m];
+      // Slots estimated but not filled ("holes") read as NaN in every sorted
+      // slot, so no search can ever select them (the reference leaves them
+      // uninitialised).
+      const uint32_t hole_end = a.dev_module_cluster_start[e * VELO_N_MODULES + m + 1];
+      for (uint32_t p = s + n + lane; p < hole_end; p += kWave) {
+        X[p] = kHoleBits;
+        Y[p] = kHoleBits;
+        Z[p] = kHoleBits;
+        PHI[p] = kHoleBits;
+        TMP[p] = kHoleBits;
+      }
+      if (n == 0) continue;
+      if (n <= kSortCap) {
+        for (uint32_t i = lane; i < n; i += kWave) phi[i] = module_phi(m, __uint_as_float(X[s + i]), __uint_as_float(Y[s + i]));
+        wave_sync();
+        for (uint32_t i = lane; i < n; i += kWave) {
+          const float p = phi[i];
+          uint32_t pos = 0;
+          for (uint32_t j = 0; j < n; ++j) {
+            const float q = phi[j];
+            pos += (p > q) || (p == q && i > j);
+          }
+          perm[pos] = (uint16_t) i;
+          a.dev_hit_permutation[s + pos] = s + i;
+          PHI[s + pos] = __float_as_uint(p);
+        }
+        wave_sync();
+        uint32_t vx[kSortChunks], vy[kSortChunks], vz[kSortChunks], vid[kSortChunks];
+  #pragma unroll
+        for (uint32_t c = 0; c < kSortChunks; ++c) {
+          const uint32_t p = c * kWave + lane;
+          if (p < n) {
+            const uint32_t i = s + perm[p];
+            vx[c] = X[i];
+            vy[c] = Y[i];
+            vz[c] = Z[i];
+            vid[c] = ID[i];
+          }
+        }
+        wave_sync(); // every read of the module is done before slots 0-2 are overwritten
+  #pragma unroll
+        for (uint32_t c = 0; c < kSortChunks; ++c) {
+          const uint32_t p = c * kWave + lane;
+          if (p < n) {
+            TMP[s + p] = vx[c];
+            X[s + p] = vy[c];
+            Y[s + p] = vz[c];
+            Z[s + p] = vid[c];
+          }
+        }
+      }
+      else {
+        // phis staged in slot 5 (it receives X only at the end)
+        for (uint32_t i = lane; i < n; i += kWave)
+          TMP[s + i] = __float_as_uint(module_phi(m, __uint_as_float(X[s + i]), __uint_as_float(Y[s + i])));
+        wave_sync();
+        for (uint32_t i = lane; i < n; i += kWave) {
+          const float p = __uint_as_float(TMP[s + i]);
+          uint32_t pos = 0;
+          for (uint32_t j = 0; j < n; ++j) {
+            const float q = __uint_as_float(TMP[s + j]);
+            pos += (p > q) || (p == q && i > j);
+          }
+          a.dev_hit_permutation[s + pos] = s + i;
+          PHI[s + pos] = __float_as_uint(p);
+        }
+        wave_sync();
+        // SortByPhi.cu:21-33 order: X->5, Y->0, Z->1, ID->2
+        for (uint32_t p = lane; p < n; p += kWave) TMP[s + p] = X[a.dev_hit_permutation[s + p]];
+        wave_sync();
+        for (uint32_t p = lane; p < n; p += kWave) X[s + p] = Y[a.dev_hit_permutation[s + p]];
+        wave_sync();
+        for (uint32_t p = lane; p < n; p += kWave) Y[s + p] = Z[a.dev_hit_permutation[s + p]];
+        wave_sync();
+        for (uint32_t p = lane; p < n; p += kWave) Z[s + p] = ID[a.dev_hit_permutation[s + p]];
+      }
+      wave_sync();
     }
-    if (n == 0) continue;
-    if (n <= kSortCap) {
-      for (uint32_t i = lane; i < n; i += kWave) phi[i] = module_phi(m, __uint_as_float(X[s + i]), __uint_as_float(Y[s + i]));
-      wave_sync();
-      for (uint32_t i = lane; i < n; i += kWave) {
-        const float p = phi[i];
-        uint32_t pos = 0;
-        for (uint32_t j = 0; j < n; ++j) {
-          const float q = phi[j];
-          pos += (p > q) || (p == q && i > j);
-        }
-        perm[pos] = (uint16_t) i;
-        a.dev_hit_permutation[s + pos] = s + i;
-        PHI[s + pos] = __float_as_uint(p);
-      }
-      wave_sync();
-      uint32_t vx[kSortChunks], vy[kSortChunks], vz[kSortChunks], vid[kSortChunks];
-#pragma unroll
-      for (uint32_t c = 0; c < kSortChunks; ++c) {
-        const uint32_t p = c * kWave + lane;
-        if (p < n) {
-          const uint32_t i = s + perm[p];
-          vx[c] = X[i];
-          vy[c] = Y[i];
-          vz[c] = Z[i];
-          vid[c] = ID[i];
-        }
-      }
-      wave_sync(); // every read of the module is done before slots 0-2 are overwritten
-#pragma unroll
-      for (uint32_t c = 0; c < kSortChunks; ++c) {
-        const uint32_t p = c * kWave + lane;
-        if (p < n) {
-          TMP[s + p] = vx[c];
-          X[s + p] = vy[c];
-          Y[s + p] = vz[c];
-          Z[s + p] = vid[c];
-        }
-      }
-    }
-    else {
-      // phis staged in slot 5 (it receives X only at the end)
-      for (uint32_t i = lane; i < n; i += kWave)
-        TMP[s + i] = __float_as_uint(module_phi(m, __uint_as_float(X[s + i]), __uint_as_float(Y[s + i])));
-      wave_sync();
-      for (uint32_t i = lane; i < n; i += kWave) {
-        const float p = __uint_as_float(TMP[s + i]);
-        uint32_t pos = 0;
-        for (uint32_t j = 0; j < n; ++j) {
-          const float q = __uint_as_float(TMP[s + j]);
-          pos += (p > q) || (p == q && i > j);
-        }
-        a.dev_hit_permutation[s + pos] = s + i;
-        PHI[s + pos] = __float_as_uint(p);
-      }
-      wave_sync();
-      // SortByPhi.cu:21-33 order: X->5, Y->0, Z->1, ID->2
-      for (uint32_t p = lane; p < n; p += kWave) TMP[s + p] = X[a.dev_hit_permutation[s + p]];
-      wave_sync();
-      for (uint32_t p = lane; p < n; p += kWave) X[s + p] = Y[a.dev_hit_permutation[s + p]];
-      wave_sync();
-      for (uint32_t p = lane; p < n; p += kWave) Y[s + p] = Z[a.dev_hit_permutation[s + p]];
-      wave_sync();
-      for (uint32_t p = lane; p < n; p += kWave) Z[s + p] = ID[a.dev_hit_permutation[s + p]];
-    }
-    wave_sync();
-  }
+  });
 }
 
 // fill_candidates: the event's phi slot is copied to LDS when it fits
@@ -171,24 +172,25 @@ __device__ __forceinline__ void fill_event(
 __global__ void __launch_bounds__(1024) fill_candidates_kernel(velo_fill_candidates_args a)
 {
   extern __shared__ float s_phi[];
-  const uint32_t e = blockIdx.x;
-  const uint32_t total = a.dev_module_cluster_start[VELO_N_MODULES * a.n_events];
-  const uint32_t* starts = a.dev_module_cluster_start + e * VELO_N_MODULES;
-  const uint32_t* nums = a.dev_module_cluster_num + e * VELO_N_MODULES;
-  const uint32_t off = starts[0];
-  const uint32_t ev_hits = starts[VELO_N_MODULES] - off;
-  const float* phis = reinterpret_cast<const float*>(a.dev_velo_cluster_container + 4 * (size_t) total + off);
-  int16_t* h0c = a.dev_h0_candidates + 2 * (size_t) off;
-  int16_t* h2c = a.dev_h2_candidates + 2 * (size_t) off;
-  if (ev_hits <= a.tuning.lds_tile) {
-    for (uint32_t i = threadIdx.x; i < ev_hits; i += blockDim.x) s_phi[i] = phis[i];
-    __syncthreads();
-    const auto acc = [&](int i) { return s_phi[i]; };
-    fill_event(acc, a, starts, nums, off, h0c, h2c);
-  }
-  else {
-    fill_event(ArrayAt {phis}, a, starts, nums, off, h0c, h2c);
-  }
+  for_each_event(a.tuning, a.n_events, [&](const uint32_t e) {
+    const uint32_t total = a.dev_module_cluster_start[VELO_N_MODULES * a.n_events];
+    const uint32_t* starts = a.dev_module_cluster_start + e * VELO_N_MODULES;
+    const uint32_t* nums = a.dev_module_cluster_num + e * VELO_N_MODULES;
+    const uint32_t off = starts[0];
+    const uint32_t ev_hits = starts[VELO_N_MODULES] - off;
+    const float* phis = reinterpret_cast<const float*>(a.dev_velo_cluster_container + 4 * (size_t) total + off);
+    int16_t* h0c = a.dev_h0_candidates + 2 * (size_t) off;
+    int16_t* h2c = a.dev_h2_candidates + 2 * (size_t) off;
+    if (ev_hits <= a.tuning.lds_tile) {
+      for (uint32_t i = threadIdx.x; i < ev_hits; i += blockDim.x) s_phi[i] = phis[i];
+      __syncthreads();
+      const auto acc = [&](int i) { return s_phi[i]; };
+      fill_event(acc, a, starts, nums, off, h0c, h2c);
+    }
+    else {
+      fill_event(ArrayAt {phis}, a, starts, nums, off, h0c, h2c);
+    }
+  });
 }
 
 } // namespace velo
@@ -203,10 +205,11 @@ int velo_calculate_phi_and_sort_launch(const velo_calculate_phi_and_sort_args* a
       !a->dev_hit_permutation)
     return (int) hipErrorInvalidValue;
   if (a->n_events == 0) return 0;
+  if (!schedule_ok(a->tuning)) return (int) hipErrorInvalidValue;
   const uint32_t block = a->tuning.block_dim ? a->tuning.block_dim : kSortBlock;
   if (block % kWave || block > 1024) return (int) hipErrorInvalidValue;
   const size_t shmem = (size_t) (block / kWave) * kSortCap * (sizeof(float) + sizeof(uint16_t));
-  hipLaunchKernelGGL(calculate_phi_and_sort_kernel, dim3(a->n_events), dim3(block), shmem, (hipStream_t) stream, *a);
+  hipLaunchKernelGGL(calculate_phi_and_sort_kernel, dim3(event_grid(a->tuning, a->n_events)), dim3(block), shmem, (hipStream_t) stream, *a);
   return (int) hipGetLastError();
 }
 
@@ -216,11 +219,12 @@ int velo_fill_candidates_launch(const velo_fill_candidates_args* a, void* stream
       !a->dev_h0_candidates || !a->dev_h2_candidates)
     return (int) hipErrorInvalidValue;
   if (a->n_events == 0) return 0;
+  if (!schedule_ok(a->tuning)) return (int) hipErrorInvalidValue;
   const uint32_t block = a->tuning.block_dim ? a->tuning.block_dim : 256;
   if (block % kWave || block > 1024) return (int) hipErrorInvalidValue;
   velo_fill_candidates_args args = *a;
   if (!args.tuning.lds_tile) args.tuning.lds_tile = kDefaultPhiTile;
-  hipLaunchKernelGGL(fill_candidates_kernel, dim3(args.n_events), dim3(block), args.tuning.lds_tile * sizeof(float),
+  hipLaunchKernelGGL(fill_candidates_kernel, dim3(event_grid(args.tuning, args.n_events)), dim3(block), args.tuning.lds_tile * sizeof(float),
                      (hipStream_t) stream, args);
   return (int) hipGetLastError();
 }

@@ -6,7 +6,10 @@
 // threads x 8 elements), one workgroup scanning the tile sums, then each
 // tile scanned in LDS and offset.  HBM traffic: 4 B read + 4 B read + 4 B
 // written per element.
+#include <hip/hip_fp16.h>
+
 #include "velo_device.h"
+#include "velo_ghost_device.h"
 
 namespace velo {
 
@@ -34,9 +37,19 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* total, uin
   return before + inc - v;
 }
 
-__global__ void __launch_bounds__(kScanThreads) scan_reduce_kernel(const uint32_t* a, uint32_t* aux, uint32_t n)
+// Element count of a scan: the capacity, or the count the device holds.
+__device__ inline uint32_t scan_count(const uint32_t* count, uint32_t capacity)
+{
+  if (!count) return capacity;
+  const uint32_t c = *count;
+  return c < capacity ? c : capacity;
+}
+
+__global__ void __launch_bounds__(kScanThreads)
+  scan_reduce_kernel(const uint32_t* a, uint32_t* aux, uint32_t capacity, const uint32_t* count)
 {
   __shared__ uint32_t wave_sums[kScanThreads / kWave];
+  const uint32_t n = scan_count(count, capacity);
   const uint32_t base = blockIdx.x * kScanTile;
   uint32_t s = 0;
   for (uint32_t k = 0; k < kScanItems; ++k) {
@@ -71,11 +84,16 @@ __global__ void __launch_bounds__(1024)
   if (threadIdx.x == 0 && total_out) *total_out = carry;
 }
 
+// Scans each tile and adds its offset; the tile holding the last element
+// (tile 0 when the scan is empty) also writes the total to a[n].
 __global__ void __launch_bounds__(kScanThreads)
-  scan_tiles_kernel(uint32_t* a, const uint32_t* aux, uint32_t n, uint32_t n_tiles)
+  scan_tiles_kernel(uint32_t* a, const uint32_t* aux, uint32_t capacity, const uint32_t* count)
 {
   __shared__ uint32_t tile[kScanTile];
   __shared__ uint32_t wave_sums[kScanThreads / kWave];
+  const uint32_t n = scan_count(count, capacity);
+  const uint32_t last_tile = n ? (n - 1) / kScanTile : 0u;
+  if (blockIdx.x > last_tile) return;
   const uint32_t base = blockIdx.x * kScanTile;
   for (uint32_t k = 0; k < kScanItems; ++k) {
     const uint32_t j = k * kScanThreads + threadIdx.x;
@@ -99,7 +117,7 @@ __global__ void __launch_bounds__(kScanThreads)
     const uint32_t j = k * kScanThreads + threadIdx.x;
     if (base + j < n) a[base + j] = tile[j];
   }
-  if (blockIdx.x == n_tiles - 1 && threadIdx.x == 0) a[n] = aux[blockIdx.x] + tot;
+  if (blockIdx.x == last_tile && threadIdx.x == 0) a[n] = aux[blockIdx.x] + tot;
 }
 
 __global__ void store_zero_kernel(uint32_t* p) { *p = 0u; }
@@ -107,28 +125,46 @@ __global__ void store_zero_kernel(uint32_t* p) { *p = 0u; }
 // K11: one workgroup per event (PrefixSumSingleBlock.cu:153-171).
 __global__ void copy_track_hit_number_kernel(velo_copy_velo_track_hit_number_args a)
 {
-  const uint32_t e = blockIdx.x;
-  const int32_t acc = a.dev_atomics_storage[a.n_events + e];
-  const int32_t n = a.dev_atomics_storage[e];
-  const uint8_t* tracks = static_cast<const uint8_t*>(a.dev_tracks) + (size_t) e * VELO_MAX_TRACKS * VELO_TRACK_HITS_BYTES;
-  for (int32_t i = threadIdx.x; i < n; i += blockDim.x)
-    a.dev_velo_track_hit_number[acc + i] = *reinterpret_cast<const uint16_t*>(tracks + (size_t) i * VELO_TRACK_HITS_BYTES);
+  for_each_event(a.tuning, a.n_events, [&](const uint32_t e) {
+    const int32_t acc = a.dev_atomics_storage[a.n_events + e];
+    const int32_t n = a.dev_atomics_storage[e];
+    const uint8_t* tracks = static_cast<const uint8_t*>(a.dev_tracks) + (size_t) e * VELO_MAX_TRACKS * VELO_TRACK_HITS_BYTES;
+    for (int32_t i = threadIdx.x; i < n; i += blockDim.x)
+      a.dev_velo_track_hit_number[acc + i] = *reinterpret_cast<const uint16_t*>(tracks + (size_t) i * VELO_TRACK_HITS_BYTES);
+  });
 }
 
 // K15: one workgroup per event, one lane per track.  The event's sorted x,
 // y, z, id are copied to LDS when they fit (tuning.lds_tile hits) so the
 // per-track gathers hit LDS.
-constexpr uint32_t kDefaultConsolidateHits = 2048;
+constexpr uint32_t kDefaultConsolidateHits = 3072;
+constexpr uint32_t kMaxConsolidateHits = 3584;
 
 struct SlotView {
   const float *x, *y, *z;
   const uint32_t* id;
 };
 
+// Hits of one track (TrackHits indices) read through a slot view.
 template<typename V>
+struct TrackHitsOf {
+  const V& hv;
+  const uint16_t* t;
+  __device__ void operator()(uint32_t h, float* x, float* y, float* z, uint32_t* id) const
+  {
+    const uint32_t hi = t[1 + h];
+    *x = hv.x[hi];
+    *y = hv.y[hi];
+    *z = hv.z[hi];
+    *id = hv.id[hi];
+  }
+};
+
+// Filter: score tracks with the ghost model whose FP32 parameters are at gw
+template<bool Filter, typename V>
 __device__ __forceinline__ void consolidate_event(
   const V& hv, const velo_consolidate_velo_tracks_args& a, uint32_t e, uint32_t n, uint32_t toff,
-  uint32_t total_tracks, uint32_t total_hits)
+  uint32_t total_tracks, uint32_t total_hits, const float* gw)
 {
   float* cx = reinterpret_cast<float*>(a.dev_velo_track_hits);
   float* cy = cx + total_hits;
@@ -215,46 +251,72 @@ __device__ __forceinline__ void consolidate_event(
     S[10 * T + ti] = chi2;
     S[11 * T + ti] = zb;
     backward[ti] = bwd ? 1 : 0;
-    if (a.dev_accepted_velo_tracks) a.dev_accepted_velo_tracks[ti] = 1;
+    uint8_t accept = 1;
+    if constexpr (Filter) {
+      const velo_ghost_model* model = static_cast<const velo_ghost_model*>(a.dev_ghost_model);
+      float f[VELO_GHOST_FEATURES];
+      track_features(TrackHitsOf<V> {hv, t}, nh, x0, y0, tx, ty, zb, chi2, f);
+      for (uint32_t k = 0; k < VELO_GHOST_FEATURES; ++k) f[k] = (f[k] - model->mean[k]) * model->scale[k];
+      float h1[VELO_GHOST_HIDDEN], h2[VELO_GHOST_HIDDEN];
+      const float p = ghost_sigmoid(ghost_logit(gw, f, h1, h2));
+      accept = p < model->threshold ? 1 : 0;
+      if (a.dev_ghost_score) a.dev_ghost_score[ti] = p;
+    }
+    if (a.dev_accepted_velo_tracks) a.dev_accepted_velo_tracks[ti] = accept;
   }
 }
 
-__global__ void __launch_bounds__(1024) consolidate_kernel(velo_consolidate_velo_tracks_args a)
+// The filtering instance keeps the network's activations in registers:
+// at most kFilterBlock threads per workgroup.
+constexpr uint32_t kFilterBlock = 256;
+
+template<bool Filter>
+__global__ void __launch_bounds__(Filter ? kFilterBlock : 1024) consolidate_kernel(velo_consolidate_velo_tracks_args a)
 {
   extern __shared__ uint32_t s_hits[];
   const uint32_t N = a.n_events;
-  const uint32_t e = blockIdx.x;
-  const uint32_t total_tracks = (uint32_t) a.dev_atomics_velo[2 * N];
-  const uint32_t total_hits = a.dev_velo_track_hit_number[total_tracks];
-  const uint32_t n = (uint32_t) a.dev_atomics_velo[e];
-  const uint32_t toff = (uint32_t) a.dev_atomics_velo[N + e];
-  const uint32_t total = a.dev_module_cluster_start[VELO_N_MODULES * N];
-  const uint32_t off = a.dev_module_cluster_start[e * VELO_N_MODULES];
-  const uint32_t ev_hits = a.dev_module_cluster_start[(e + 1) * VELO_N_MODULES] - off;
-  SlotView g;
-  g.y = reinterpret_cast<const float*>(a.dev_velo_cluster_container + off);
-  g.z = reinterpret_cast<const float*>(a.dev_velo_cluster_container + total + off);
-  g.id = a.dev_velo_cluster_container + 2 * (size_t) total + off;
-  g.x = reinterpret_cast<const float*>(a.dev_velo_cluster_container + 5 * (size_t) total + off);
-  const uint32_t cap = a.tuning.lds_tile;
-  if (n > 0 && ev_hits <= cap) {
-    for (uint32_t i = threadIdx.x; i < ev_hits; i += blockDim.x) {
-      s_hits[i] = __float_as_uint(g.x[i]);
-      s_hits[cap + i] = __float_as_uint(g.y[i]);
-      s_hits[2 * cap + i] = __float_as_uint(g.z[i]);
-      s_hits[3 * cap + i] = g.id[i];
-    }
+  const float* gw = nullptr;
+  if constexpr (Filter) {
+    __shared__ float s_ghost_w[VELO_GHOST_PARAMS];
+    const velo_ghost_model* model = static_cast<const velo_ghost_model*>(a.dev_ghost_model);
+    for (uint32_t i = threadIdx.x; i < VELO_GHOST_PARAMS; i += blockDim.x)
+      s_ghost_w[i] = __half2float(__ushort_as_half(model->weights[i]));
     __syncthreads();
-    SlotView l;
-    l.x = reinterpret_cast<const float*>(s_hits);
-    l.y = reinterpret_cast<const float*>(s_hits + cap);
-    l.z = reinterpret_cast<const float*>(s_hits + 2 * cap);
-    l.id = s_hits + 3 * cap;
-    consolidate_event(l, a, e, n, toff, total_tracks, total_hits);
+    gw = s_ghost_w;
   }
-  else {
-    consolidate_event(g, a, e, n, toff, total_tracks, total_hits);
-  }
+  for_each_event(a.tuning, a.n_events, [&](const uint32_t e) {
+    const uint32_t total_tracks = (uint32_t) a.dev_atomics_velo[2 * N];
+    const uint32_t total_hits = a.dev_velo_track_hit_number[total_tracks];
+    const uint32_t n = (uint32_t) a.dev_atomics_velo[e];
+    const uint32_t toff = (uint32_t) a.dev_atomics_velo[N + e];
+    const uint32_t total = a.dev_module_cluster_start[VELO_N_MODULES * N];
+    const uint32_t off = a.dev_module_cluster_start[e * VELO_N_MODULES];
+    const uint32_t ev_hits = a.dev_module_cluster_start[(e + 1) * VELO_N_MODULES] - off;
+    SlotView g;
+    g.y = reinterpret_cast<const float*>(a.dev_velo_cluster_container + off);
+    g.z = reinterpret_cast<const float*>(a.dev_velo_cluster_container + total + off);
+    g.id = a.dev_velo_cluster_container + 2 * (size_t) total + off;
+    g.x = reinterpret_cast<const float*>(a.dev_velo_cluster_container + 5 * (size_t) total + off);
+    const uint32_t cap = a.tuning.lds_tile;
+    if (n > 0 && ev_hits <= cap) {
+      for (uint32_t i = threadIdx.x; i < ev_hits; i += blockDim.x) {
+        s_hits[i] = __float_as_uint(g.x[i]);
+        s_hits[cap + i] = __float_as_uint(g.y[i]);
+        s_hits[2 * cap + i] = __float_as_uint(g.z[i]);
+        s_hits[3 * cap + i] = g.id[i];
+      }
+      __syncthreads();
+      SlotView l;
+      l.x = reinterpret_cast<const float*>(s_hits);
+      l.y = reinterpret_cast<const float*>(s_hits + cap);
+      l.z = reinterpret_cast<const float*>(s_hits + 2 * cap);
+      l.id = s_hits + 3 * cap;
+      consolidate_event<Filter>(l, a, e, n, toff, total_tracks, total_hits, gw);
+    }
+    else {
+      consolidate_event<Filter>(g, a, e, n, toff, total_tracks, total_hits, gw);
+    }
+  });
 }
 
 } // namespace velo
@@ -279,11 +341,13 @@ int velo_prefix_sum_launch(const velo_prefix_sum_args* a, void* stream)
     return (int) hipGetLastError();
   }
   const uint32_t tiles = velo_prefix_sum_aux_size(n);
-  hipLaunchKernelGGL(scan_reduce_kernel, dim3(tiles), dim3(kScanThreads), 0, s, a->dev_array, a->dev_auxiliary_array, n);
+  const uint32_t* count = a->tuning.dev_count;
+  hipLaunchKernelGGL(scan_reduce_kernel, dim3(tiles), dim3(kScanThreads), 0, s, a->dev_array, a->dev_auxiliary_array,
+                     n, count);
   hipLaunchKernelGGL(scan_single_block_kernel, dim3(1), dim3(1024), 0, s, (uint32_t*) nullptr,
                      a->dev_auxiliary_array, a->dev_auxiliary_array, tiles);
   hipLaunchKernelGGL(scan_tiles_kernel, dim3(tiles), dim3(kScanThreads), 0, s, a->dev_array, a->dev_auxiliary_array,
-                     n, tiles);
+                     n, count);
   return (int) hipGetLastError();
 }
 
@@ -301,8 +365,9 @@ int velo_copy_velo_track_hit_number_launch(const velo_copy_velo_track_hit_number
   if (!a || !a->dev_tracks || !a->dev_atomics_storage || !a->dev_velo_track_hit_number)
     return (int) hipErrorInvalidValue;
   if (a->n_events == 0) return 0;
+  if (!schedule_ok(a->tuning)) return (int) hipErrorInvalidValue;
   const uint32_t block = a->tuning.block_dim ? a->tuning.block_dim : 256;
-  hipLaunchKernelGGL(copy_track_hit_number_kernel, dim3(a->n_events), dim3(block), 0, (hipStream_t) stream, *a);
+  hipLaunchKernelGGL(copy_track_hit_number_kernel, dim3(event_grid(a->tuning, a->n_events)), dim3(block), 0, (hipStream_t) stream, *a);
   return (int) hipGetLastError();
 }
 
@@ -313,11 +378,20 @@ int velo_consolidate_velo_tracks_launch(const velo_consolidate_velo_tracks_args*
       !a->dev_velo_states)
     return (int) hipErrorInvalidValue;
   if (a->n_events == 0) return 0;
+  if (!schedule_ok(a->tuning)) return (int) hipErrorInvalidValue;
   const uint32_t block = a->tuning.block_dim ? a->tuning.block_dim : 256;
+  if (block % kWave || block > 1024) return (int) hipErrorInvalidValue;
   velo_consolidate_velo_tracks_args args = *a;
   if (!args.tuning.lds_tile) args.tuning.lds_tile = kDefaultConsolidateHits;
-  hipLaunchKernelGGL(consolidate_kernel, dim3(args.n_events), dim3(block), 4 * sizeof(uint32_t) * args.tuning.lds_tile,
-                     (hipStream_t) stream, args);
+  // staged hits + the ghost model stay within 64 KB of LDS per workgroup
+  if (args.tuning.lds_tile > kMaxConsolidateHits) return (int) hipErrorInvalidValue;
+  const dim3 grid(event_grid(args.tuning, args.n_events));
+  const size_t shmem = 4 * sizeof(uint32_t) * args.tuning.lds_tile;
+  if (args.dev_ghost_model)
+    hipLaunchKernelGGL(consolidate_kernel<true>, grid, dim3(block < kFilterBlock ? block : kFilterBlock), shmem,
+                       (hipStream_t) stream, args);
+  else
+    hipLaunchKernelGGL(consolidate_kernel<false>, grid, dim3(block), shmem, (hipStream_t) stream, args);
   return (int) hipGetLastError();
 }
 

@@ -443,7 +443,7 @@ __device__ inline void forwarding(
   }
 }
 
-constexpr uint32_t kDefaultWindowHits = 1024;
+constexpr uint32_t kDefaultWindowHits = 256;
 
 __global__ void __launch_bounds__(64) search_by_triplet_kernel(velo_search_by_triplet_args a)
 {
@@ -451,90 +451,91 @@ __global__ void __launch_bounds__(64) search_by_triplet_kernel(velo_search_by_tr
   __shared__ uint16_t stage[2 * kWave];
 
   const uint32_t N = a.n_events;
-  const uint32_t e = blockIdx.x;
-  const uint32_t lane = lane_id();
-  const uint32_t total = a.dev_module_cluster_start[VELO_N_MODULES * N];
-  const uint32_t* starts = a.dev_module_cluster_start + e * VELO_N_MODULES;
-  const uint32_t* nums = a.dev_module_cluster_num + e * VELO_N_MODULES;
-  const uint32_t off = starts[0];
-  const uint32_t ev_hits = starts[VELO_N_MODULES] - off;
-  const uint32_t cap = a.tuning.lds_tile;
-  EventHits H;
-  H.gy = reinterpret_cast<const float*>(a.dev_velo_cluster_container + off);
-  H.gz = reinterpret_cast<const float*>(a.dev_velo_cluster_container + total + off);
-  H.gphi = reinterpret_cast<const float*>(a.dev_velo_cluster_container + 4 * (size_t) total + off);
-  H.gx = reinterpret_cast<const float*>(a.dev_velo_cluster_container + 5 * (size_t) total + off);
-  H.cap = cap;
-  H.lo = 0;
-  H.n = 0;
-  // window of modules first+3 .. first-5 (contiguous in the container)
-  const auto stage_window = [&](uint32_t first) {
-    const uint32_t lo_m = first - 5;
-    const uint32_t hi_m = first + 3 < VELO_N_MODULES ? first + 3 : VELO_N_MODULES - 1;
-    H.stage(starts[lo_m] - off, starts[hi_m + 1] - off);
-  };
-  const int16_t* h0c = a.dev_h0_candidates + 2 * (size_t) off;
-  const int16_t* h2c = a.dev_h2_candidates + 2 * (size_t) off;
-  uint8_t* tracks = static_cast<uint8_t*>(a.dev_tracks) + (size_t) e * VELO_MAX_TRACKS * VELO_TRACK_HITS_BYTES;
-  uint8_t* weak = static_cast<uint8_t*>(a.dev_weak_tracks) + (size_t) e * VELO_MAX_WEAK_TRACKS * VELO_TRACKLET_BYTES;
-  Ring ring {a.dev_tracks_to_follow + (size_t) e * VELO_TTF_MODULO * VELO_TTF_ENTRY_WORDS};
+  for_each_event(a.tuning, a.n_events, [&](const uint32_t e) {
+    const uint32_t lane = lane_id();
+    const uint32_t total = a.dev_module_cluster_start[VELO_N_MODULES * N];
+    const uint32_t* starts = a.dev_module_cluster_start + e * VELO_N_MODULES;
+    const uint32_t* nums = a.dev_module_cluster_num + e * VELO_N_MODULES;
+    const uint32_t off = starts[0];
+    const uint32_t ev_hits = starts[VELO_N_MODULES] - off;
+    const uint32_t cap = a.tuning.lds_tile;
+    EventHits H;
+    H.gy = reinterpret_cast<const float*>(a.dev_velo_cluster_container + off);
+    H.gz = reinterpret_cast<const float*>(a.dev_velo_cluster_container + total + off);
+    H.gphi = reinterpret_cast<const float*>(a.dev_velo_cluster_container + 4 * (size_t) total + off);
+    H.gx = reinterpret_cast<const float*>(a.dev_velo_cluster_container + 5 * (size_t) total + off);
+    H.cap = cap;
+    H.lo = 0;
+    H.n = 0;
+    // window of modules first+3 .. first-5 (contiguous in the container)
+    const auto stage_window = [&](uint32_t first) {
+      const uint32_t lo_m = first - 5;
+      const uint32_t hi_m = first + 3 < VELO_N_MODULES ? first + 3 : VELO_N_MODULES - 1;
+      H.stage(starts[lo_m] - off, starts[hi_m + 1] - off);
+    };
+    const int16_t* h0c = a.dev_h0_candidates + 2 * (size_t) off;
+    const int16_t* h2c = a.dev_h2_candidates + 2 * (size_t) off;
+    uint8_t* tracks = static_cast<uint8_t*>(a.dev_tracks) + (size_t) e * VELO_MAX_TRACKS * VELO_TRACK_HITS_BYTES;
+    uint8_t* weak = static_cast<uint8_t*>(a.dev_weak_tracks) + (size_t) e * VELO_MAX_WEAK_TRACKS * VELO_TRACKLET_BYTES;
+    Ring ring {a.dev_tracks_to_follow + (size_t) e * VELO_TTF_MODULO * VELO_TTF_ENTRY_WORDS};
 
-  UsedSet used {used_bits, a.dev_hit_used + off, ev_hits <= kUsedBits};
-  if (used.in_lds) {
-    for (uint32_t w = lane; w < kUsedWords; w += kWave) used_bits[w] = 0;
-  }
-  else {
-    for (uint32_t i = lane; i < ev_hits; i += kWave) used.bytes[i] = 0;
-  }
-  wave_sync();
+    UsedSet used {used_bits, a.dev_hit_used + off, ev_hits <= kUsedBits};
+    if (used.in_lds) {
+      for (uint32_t w = lane; w < kUsedWords; w += kWave) used_bits[w] = 0;
+    }
+    else {
+      for (uint32_t i = lane; i < ev_hits; i += kWave) used.bytes[i] = 0;
+    }
+    wave_sync();
 
-  SearchState st {0, 0, 0, 0, 0, 0, VELO_TTF_MODULO};
-  ModuleData md[6];
-  uint32_t first = VELO_N_MODULES - 1;
-  load_modules(md, first, starts, nums, off, a.dev_velo_module_zs);
-  stage_window(first);
-  uint32_t prev_tracklets = seeding(md, stage, H, h0c, h2c, used, ring, st);
-  uint32_t cur_begin = 0, cur_n = st.next_n;
-  first -= 2;
-  while (first >= 4) {
+    SearchState st {0, 0, 0, 0, 0, 0, VELO_TTF_MODULO};
+    ModuleData md[6];
+    uint32_t first = VELO_N_MODULES - 1;
     load_modules(md, first, starts, nums, off, a.dev_velo_module_zs);
     stage_window(first);
-    st.next_begin = (cur_begin + cur_n) % VELO_TTF_MODULO;
-    st.next_n = 0;
-    st.cap = VELO_TTF_MODULO - cur_n;
-    forwarding(md, cur_begin, cur_n, H, used, ring, tracks, weak, st);
-    wave_sync(); // hit_used marks of forwarding are read by seeding
-    const uint32_t tl = seeding(md, stage, H, h0c, h2c, used, ring, st);
-    if (tl + prev_tracklets > VELO_TTF_MODULO) st.status |= VELO_STATUS_TRACKLETS_OVERFLOW;
-    prev_tracklets = tl;
-    cur_begin = st.next_begin;
-    cur_n = st.next_n;
+    uint32_t prev_tracklets = seeding(md, stage, H, h0c, h2c, used, ring, st);
+    uint32_t cur_begin = 0, cur_n = st.next_n;
     first -= 2;
-    wave_sync();
-  }
+    while (first >= 4) {
+      load_modules(md, first, starts, nums, off, a.dev_velo_module_zs);
+      stage_window(first);
+      st.next_begin = (cur_begin + cur_n) % VELO_TTF_MODULO;
+      st.next_n = 0;
+      st.cap = VELO_TTF_MODULO - cur_n;
+      forwarding(md, cur_begin, cur_n, H, used, ring, tracks, weak, st);
+      wave_sync(); // hit_used marks of forwarding are read by seeding
+      const uint32_t tl = seeding(md, stage, H, h0c, h2c, used, ring, st);
+      if (tl + prev_tracklets > VELO_TTF_MODULO) st.status |= VELO_STATUS_TRACKLETS_OVERFLOW;
+      prev_tracklets = tl;
+      cur_begin = st.next_begin;
+      cur_n = st.next_n;
+      first -= 2;
+      wave_sync();
+    }
 
-  // ProcessModules.cu:141-162: tracklets still followed become weak tracks
-  for (uint32_t base = 0; base < cur_n; base += kWave) {
-    const uint32_t idx = base + lane;
-    uint2 ent = make_uint2(0, 0);
-    if (idx < cur_n) ent = ring.get(cur_begin + idx);
-    const bool tl = idx < cur_n && (ent.y & kTtfTrackletFlag);
-    append_weak(tl, ent.y & 0xFFFFu, ent.x & 0xFFFFu, ent.x >> 16, weak, st);
-  }
+    // ProcessModules.cu:141-162: tracklets still followed become weak tracks
+    for (uint32_t base = 0; base < cur_n; base += kWave) {
+      const uint32_t idx = base + lane;
+      uint2 ent = make_uint2(0, 0);
+      if (idx < cur_n) ent = ring.get(cur_begin + idx);
+      const bool tl = idx < cur_n && (ent.y & kTtfTrackletFlag);
+      append_weak(tl, ent.y & 0xFFFFu, ent.x & 0xFFFFu, ent.x >> 16, weak, st);
+    }
 
-  if (lane == 0) {
-    a.dev_atomics_velo[e] = (int32_t) st.n_tracks;
-    const uint32_t ip = N + e * (VELO_NUM_ATOMICS - 1u);
-    a.dev_atomics_velo[ip + 0] = (int32_t) st.n_weak;
-    a.dev_atomics_velo[ip + 1] = (int32_t) st.n_tracklets;
-    a.dev_atomics_velo[ip + 2] = 0;
-    a.dev_atomics_velo[ip + 3] = 0;
-    if (a.dev_velo_event_status && st.status) atomicOr(a.dev_velo_event_status + e, st.status);
-  }
-  if (used.in_lds) {
-    wave_sync();
-    for (uint32_t i = lane; i < ev_hits; i += kWave) used.bytes[i] = (uint8_t) used.test((int) i);
-  }
+    if (lane == 0) {
+      a.dev_atomics_velo[e] = (int32_t) st.n_tracks;
+      const uint32_t ip = N + e * (VELO_NUM_ATOMICS - 1u);
+      a.dev_atomics_velo[ip + 0] = (int32_t) st.n_weak;
+      a.dev_atomics_velo[ip + 1] = (int32_t) st.n_tracklets;
+      a.dev_atomics_velo[ip + 2] = 0;
+      a.dev_atomics_velo[ip + 3] = 0;
+      if (a.dev_velo_event_status && st.status) atomicOr(a.dev_velo_event_status + e, st.status);
+    }
+    if (used.in_lds) {
+      wave_sync();
+      for (uint32_t i = lane; i < ev_hits; i += kWave) used.bytes[i] = (uint8_t) used.test((int) i);
+    }
+  });
 }
 
 // WeakTracksAdder.cu:8-86
@@ -590,58 +591,59 @@ __device__ inline float weak_chi2(const Hits& H, uint32_t a0, uint32_t a1, uint3
 __global__ void __launch_bounds__(64) weak_tracks_adder_kernel(velo_weak_tracks_adder_args a)
 {
   const uint32_t N = a.n_events;
-  const uint32_t e = blockIdx.x;
-  const uint32_t lane = lane_id();
-  const uint32_t total = a.dev_module_cluster_start[VELO_N_MODULES * N];
-  const uint32_t off = a.dev_module_cluster_start[e * VELO_N_MODULES];
-  GlobalHits H;
-  H.gy = reinterpret_cast<const float*>(a.dev_velo_cluster_container + off);
-  H.gz = reinterpret_cast<const float*>(a.dev_velo_cluster_container + total + off);
-  H.gx = reinterpret_cast<const float*>(a.dev_velo_cluster_container + 5 * (size_t) total + off);
-  const uint8_t* used = a.dev_hit_used + off;
-  uint8_t* tracks = static_cast<uint8_t*>(a.dev_tracks) + (size_t) e * VELO_MAX_TRACKS * VELO_TRACK_HITS_BYTES;
-  const uint8_t* weak =
-    static_cast<const uint8_t*>(a.dev_weak_tracks) + (size_t) e * VELO_MAX_WEAK_TRACKS * VELO_TRACKLET_BYTES;
-  const uint32_t ip = N + e * (VELO_NUM_ATOMICS - 1u);
-  uint32_t n_weak = (uint32_t) a.dev_atomics_velo[ip];
-  if (n_weak > VELO_MAX_WEAK_TRACKS) n_weak = VELO_MAX_WEAK_TRACKS;
-  uint32_t n_tracks = (uint32_t) a.dev_atomics_velo[e];
-  uint32_t status = 0;
-  for (uint32_t base = 0; base < n_weak; base += kWave) {
-    const uint32_t w = base + lane;
-    bool accept = false;
-    uint32_t t0 = 0, t1 = 0, t2 = 0;
-    if (w < n_weak) {
-      const uint16_t* t = reinterpret_cast<const uint16_t*>(weak + (size_t) w * VELO_TRACKLET_BYTES);
-      t0 = t[0];
-      t1 = t[1];
-      t2 = t[2];
-      const bool any_used = used[t0] || used[t1] || used[t2];
-      const float chi2 = weak_chi2(H, t0, t1, t2);
-      accept = !any_used && chi2 < kMaxChi2;
+  for_each_event(a.tuning, a.n_events, [&](const uint32_t e) {
+    const uint32_t lane = lane_id();
+    const uint32_t total = a.dev_module_cluster_start[VELO_N_MODULES * N];
+    const uint32_t off = a.dev_module_cluster_start[e * VELO_N_MODULES];
+    GlobalHits H;
+    H.gy = reinterpret_cast<const float*>(a.dev_velo_cluster_container + off);
+    H.gz = reinterpret_cast<const float*>(a.dev_velo_cluster_container + total + off);
+    H.gx = reinterpret_cast<const float*>(a.dev_velo_cluster_container + 5 * (size_t) total + off);
+    const uint8_t* used = a.dev_hit_used + off;
+    uint8_t* tracks = static_cast<uint8_t*>(a.dev_tracks) + (size_t) e * VELO_MAX_TRACKS * VELO_TRACK_HITS_BYTES;
+    const uint8_t* weak =
+      static_cast<const uint8_t*>(a.dev_weak_tracks) + (size_t) e * VELO_MAX_WEAK_TRACKS * VELO_TRACKLET_BYTES;
+    const uint32_t ip = N + e * (VELO_NUM_ATOMICS - 1u);
+    uint32_t n_weak = (uint32_t) a.dev_atomics_velo[ip];
+    if (n_weak > VELO_MAX_WEAK_TRACKS) n_weak = VELO_MAX_WEAK_TRACKS;
+    uint32_t n_tracks = (uint32_t) a.dev_atomics_velo[e];
+    uint32_t status = 0;
+    for (uint32_t base = 0; base < n_weak; base += kWave) {
+      const uint32_t w = base + lane;
+      bool accept = false;
+      uint32_t t0 = 0, t1 = 0, t2 = 0;
+      if (w < n_weak) {
+        const uint16_t* t = reinterpret_cast<const uint16_t*>(weak + (size_t) w * VELO_TRACKLET_BYTES);
+        t0 = t[0];
+        t1 = t[1];
+        t2 = t[2];
+        const bool any_used = used[t0] || used[t1] || used[t2];
+        const float chi2 = weak_chi2(H, t0, t1, t2);
+        accept = !any_used && chi2 < kMaxChi2;
+      }
+      const uint64_t m = ballot(accept);
+      const uint32_t slot = n_tracks + count_below(m);
+      if (accept && slot < VELO_MAX_TRACKS) {
+        uint16_t* d = reinterpret_cast<uint16_t*>(tracks + (size_t) slot * VELO_TRACK_HITS_BYTES);
+        d[0] = 3;
+        d[1] = (uint16_t) t0;
+        d[2] = (uint16_t) t1;
+        d[3] = (uint16_t) t2;
+      }
+      const uint32_t cnt = (uint32_t) __popcll(m);
+      if (n_tracks + cnt > VELO_MAX_TRACKS) {
+        status |= VELO_STATUS_TRACKS_OVERFLOW;
+        n_tracks = VELO_MAX_TRACKS;
+      }
+      else {
+        n_tracks += cnt;
+      }
     }
-    const uint64_t m = ballot(accept);
-    const uint32_t slot = n_tracks + count_below(m);
-    if (accept && slot < VELO_MAX_TRACKS) {
-      uint16_t* d = reinterpret_cast<uint16_t*>(tracks + (size_t) slot * VELO_TRACK_HITS_BYTES);
-      d[0] = 3;
-      d[1] = (uint16_t) t0;
-      d[2] = (uint16_t) t1;
-      d[3] = (uint16_t) t2;
+    if (lane == 0) {
+      a.dev_atomics_velo[e] = (int32_t) n_tracks;
+      if (a.dev_velo_event_status && status) atomicOr(a.dev_velo_event_status + e, status);
     }
-    const uint32_t cnt = (uint32_t) __popcll(m);
-    if (n_tracks + cnt > VELO_MAX_TRACKS) {
-      status |= VELO_STATUS_TRACKS_OVERFLOW;
-      n_tracks = VELO_MAX_TRACKS;
-    }
-    else {
-      n_tracks += cnt;
-    }
-  }
-  if (lane == 0) {
-    a.dev_atomics_velo[e] = (int32_t) n_tracks;
-    if (a.dev_velo_event_status && status) atomicOr(a.dev_velo_event_status + e, status);
-  }
+  });
 }
 
 } // namespace velo
@@ -657,9 +659,10 @@ int velo_search_by_triplet_launch(const velo_search_by_triplet_args* a, void* st
       !a->dev_atomics_velo || !a->dev_h0_candidates || !a->dev_h2_candidates || !a->dev_velo_module_zs)
     return (int) hipErrorInvalidValue;
   if (a->n_events == 0) return 0;
+  if (!schedule_ok(a->tuning)) return (int) hipErrorInvalidValue;
   velo_search_by_triplet_args args = *a;
   if (!args.tuning.lds_tile) args.tuning.lds_tile = kDefaultWindowHits;
-  hipLaunchKernelGGL(search_by_triplet_kernel, dim3(args.n_events), dim3(kWave), 4 * sizeof(float) * args.tuning.lds_tile,
+  hipLaunchKernelGGL(search_by_triplet_kernel, dim3(event_grid(args.tuning, args.n_events)), dim3(kWave), 4 * sizeof(float) * args.tuning.lds_tile,
                      (hipStream_t) stream, args);
   return (int) hipGetLastError();
 }
@@ -670,7 +673,8 @@ int velo_weak_tracks_adder_launch(const velo_weak_tracks_adder_args* a, void* st
       !a->dev_weak_tracks || !a->dev_hit_used || !a->dev_atomics_velo)
     return (int) hipErrorInvalidValue;
   if (a->n_events == 0) return 0;
-  hipLaunchKernelGGL(weak_tracks_adder_kernel, dim3(a->n_events), dim3(kWave), 0, (hipStream_t) stream, *a);
+  if (!schedule_ok(a->tuning)) return (int) hipErrorInvalidValue;
+  hipLaunchKernelGGL(weak_tracks_adder_kernel, dim3(event_grid(a->tuning, a->n_events)), dim3(kWave), 0, (hipStream_t) stream, *a);
   return (int) hipGetLastError();
 }
 

@@ -46,7 +46,11 @@ class VeloError(RuntimeError):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("block_dim", U32), ("lds_tile", U32)]
+    """velo_tuning: launch shape; grid_dim > 0 makes a per-event launch
+    persistent (workgroups take events from dev_work_counter, in
+    dev_work_order); dev_count gives a scan its element count on the device."""
+    _fields_ = [("block_dim", U32), ("lds_tile", U32), ("grid_dim", U32), ("reserved", U32),
+                ("dev_work_counter", P), ("dev_work_order", P), ("dev_count", P)]
 
 
 class Constants(C.Structure):
@@ -91,7 +95,7 @@ CopyTrackHitNumberArgs = _args("CopyTrackHitNumberArgs", [
 ConsolidateArgs = _args("ConsolidateArgs", [
     "dev_atomics_velo", "dev_tracks", "dev_velo_track_hit_number", "dev_velo_cluster_container",
     "dev_module_cluster_start", "dev_module_cluster_num", "dev_velo_track_hits", "dev_velo_states",
-    "dev_accepted_velo_tracks"])
+    "dev_accepted_velo_tracks", "dev_ghost_model", "dev_ghost_score"])
 
 LAUNCHERS = {
     "velo_init_event_list_launch": InitEventListArgs,

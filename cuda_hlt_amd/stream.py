@@ -19,13 +19,15 @@ class SequenceConfig(C.Structure):
 class SequenceInput(C.Structure):
     _fields_ = [("host_events", P), ("host_offsets", P), ("events_size", C.c_size_t), ("dev_events", P),
                 ("dev_offsets", P), ("n_events", C.c_uint32), ("repetitions", C.c_uint32),
-                ("copy_outputs_to_host", C.c_int), ("tuning", nat.Tuning * 6)]
+                ("copy_outputs_to_host", C.c_int), ("tuning", nat.Tuning * 6), ("batched", C.c_int),
+                ("dev_work_order", P), ("dev_ghost_model", P)]
 
 
 class SequenceOutput(C.Structure):
     _fields_ = [("n_events", C.c_uint32), ("total_estimated_hits", C.c_uint32), ("total_tracks", C.c_uint32),
                 ("total_track_hits", C.c_uint32), ("host_atomics_velo", P), ("host_track_hit_number", P),
-                ("host_track_hits", P), ("host_states", P), ("dev_atomics_velo", P), ("dev_track_hit_number", P),
+                ("host_track_hits", P), ("host_states", P), ("host_accepted", P), ("dev_atomics_velo", P),
+                ("dev_track_hit_number", P),
                 ("dev_track_hits", P), ("dev_states", P), ("dev_accepted", P), ("dev_event_status", P),
                 ("dev_module_cluster_start", P), ("dev_module_cluster_num", P), ("dev_cluster_container", P),
                 ("memory_required", C.c_size_t)]
@@ -66,6 +68,7 @@ class HostTracks:
     hit_number: np.ndarray    # uint32[T+1]
     track_hits: np.ndarray    # uint8[16 Ht]
     states: np.ndarray        # uint8[49 T]
+    accepted: np.ndarray      # uint8[T]: dev_accepted_velo_tracks
 
 
 class VeloSequence:
@@ -97,11 +100,21 @@ class VeloSequence:
         return _lib().velo_sequence_stream(self.h)
 
     def run(self, n_events, host_events=None, host_offsets=None, dev_events=None, dev_offsets=None,
-            copy_outputs=True, repetitions=1, tuning=None) -> SequenceOutput:
+            copy_outputs=True, repetitions=1, tuning=None, events_size=0, batched=False,
+            dev_work_order=None, dev_ghost_model=None) -> SequenceOutput:
+        """Run the sequence on host input (copied by init_event_list) or on
+        device-resident input (dev_events/dev_offsets; events_size = their
+        bytes, needed by batched runs to bound the buffers).  batched=True
+        captures the sequence in a hipGraph on the first call and replays it
+        while inputs, sizes and tuning stay the same.  tuning maps a step
+        name to (block_dim, lds_tile) or (block_dim, lds_tile, grid_dim);
+        grid_dim > 0 runs that step persistent in dev_work_order.
+        dev_ghost_model (a device velo_ghost_model) makes consolidation
+        score every track and clear the accepted flag of likely ghosts."""
         inp = SequenceInput()
         if dev_events is not None:
             inp.dev_events, inp.dev_offsets = int(dev_events), int(dev_offsets)
-            inp.events_size = 0
+            inp.events_size = int(events_size)
         else:
             self._keep = (np.ascontiguousarray(host_events), np.ascontiguousarray(host_offsets, np.uint32))
             inp.host_events = self._keep[0].ctypes.data
@@ -110,8 +123,11 @@ class VeloSequence:
         inp.n_events = n_events
         inp.repetitions = repetitions
         inp.copy_outputs_to_host = int(copy_outputs)
-        for k, (b, t) in (tuning or {}).items():
-            inp.tuning[TUNING_SLOTS.index(k)] = nat.Tuning(b, t)
+        for k, shape in (tuning or {}).items():
+            inp.tuning[TUNING_SLOTS.index(k)] = nat.Tuning(*shape)
+        inp.batched = int(batched)
+        inp.dev_work_order = None if dev_work_order is None else int(dev_work_order)
+        inp.dev_ghost_model = None if dev_ghost_model is None else int(dev_ghost_model)
         err = C.create_string_buffer(1024)
         rc = _lib().velo_sequence_run(self.h, C.byref(inp), err, 1024)
         if rc != 0:
@@ -144,4 +160,5 @@ class VeloSequence:
                                          ).view(dtype)[:count].copy()
         return HostTracks(n, arr(o.host_atomics_velo, 2 * n + 1, np.uint32),
                           arr(o.host_track_hit_number, T + 1, np.uint32),
-                          arr(o.host_track_hits, 16 * Ht, np.uint8), arr(o.host_states, 49 * T, np.uint8))
+                          arr(o.host_track_hits, 16 * Ht, np.uint8), arr(o.host_states, 49 * T, np.uint8),
+                          arr(o.host_accepted, T, np.uint8))

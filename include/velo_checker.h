@@ -1,0 +1,86 @@
+/* VELO track checker: efficiency per MC category, clones and ghosts.
+ *
+ * Host-only, no GPU.  Replaces the VELO part of the reference's checker:
+ *   prepareVeloTracks        checker/tracking/src/PrepareTracks.cpp:14-47
+ *   MCAssociator             checker/tracking/src/MCAssociator.cpp:13-60,
+ *                            checker/tracking/include/MCAssociator.h:166-182
+ *   TrackChecker::operator() checker/tracking/src/TrackChecker.cpp:349-405
+ *   TrackEffReport           checker/tracking/src/TrackChecker.cpp:93-146
+ *   TrackCheckerVelo categories
+ *                            checker/tracking/src/CategoriesTrackChecker.cpp:5-230
+ *
+ * Semantics kept from the reference:
+ *  - each LHCbID of a track is looked up in the event's (LHCbID -> MC
+ *    particle) map; for an LHCbID shared by several particles the particle
+ *    listed first wins (the reference sorts with std::sort, which leaves the
+ *    choice among equal keys unspecified);
+ *  - weight(p) = hits matched to p / hits matched to any particle; the best
+ *    weight decides (ties: lowest particle index); a track with no match or a
+ *    best weight below 0.7 is a ghost;
+ *  - a matched track counts as found for every category its particle is in,
+ *    the second and later tracks of one particle as clones;
+ *  - clone rate as printed by the reference = clones / (2 * found)
+ *    (TrackChecker.cpp:130, kept as is); hit efficiency and purity are the
+ *    running means of TrackChecker.cpp:110-117.
+ */
+#ifndef VELO_CHECKER_H
+#define VELO_CHECKER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "velo_gen.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VELO_CHECKER_N_CATEGORIES 40
+#define VELO_CHECKER_MIN_WEIGHT 0.7f
+
+/* MC particle flags beyond velo_gen.h's (never set by the generator, which
+ * has no decay chains; categories that need them accept no particle) */
+#define VELO_MCP_FROM_B 0x10u
+#define VELO_MCP_FROM_D 0x20u
+#define VELO_MCP_FROM_S 0x40u
+
+typedef struct {
+  uint64_t accepted; /* reconstructible particles in the category */
+  uint64_t found;    /* of which matched by at least one track */
+  uint64_t clones;   /* further tracks matched to a found particle */
+  double eff_per_event;
+  double hit_eff;
+  double hit_purity;
+} velo_checker_category;
+
+typedef struct {
+  uint64_t events;
+  uint64_t tracks;
+  uint64_t ghosts;
+  double ghost_per_event;
+  velo_checker_category categories[VELO_CHECKER_N_CATEGORIES];
+} velo_checker_result;
+
+/* Name of category i (the reference's report label), or NULL. */
+const char* velo_checker_category_name(uint32_t i);
+
+/* Check the consolidated tracks of n_events events.
+ *   event_track_offsets  n_events+1 (the scanned track counts, dev_atomics_velo + n_events)
+ *   track_hit_offsets    T+1 (the scanned hit counts)
+ *   hit_ids              LHCbIDs of the consolidated hits
+ *   accepted             per track, 0 = dropped from the check (NULL: all kept)
+ *   mcps, n_mcps         MC particles, non-decreasing in .event
+ *   mc_hits              LHCbIDs indexed by mcps[i].hit_offset
+ * Optional per-track outputs (NULL to skip): track_mcp = index into mcps of
+ * the best particle or -1, track_weight = its weight (0 when unmatched).
+ * Returns 0, or -1 on a null pointer or unsorted mcps. */
+int velo_checker_run(uint32_t n_events, const uint32_t* event_track_offsets, const uint32_t* track_hit_offsets,
+                     const uint32_t* hit_ids, const uint8_t* accepted, const velo_gen_mcp* mcps, size_t n_mcps,
+                     const uint32_t* mc_hits, velo_checker_result* result, int32_t* track_mcp,
+                     float* track_weight);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -59,10 +59,28 @@ extern "C" {
 #define VELO_STATUS_TRACKS_OVERFLOW 0x080u     /* >1200 tracks; extra dropped */
 #define VELO_STATUS_BAD_BANK 0x100u            /* bank index != sensor index; bank skipped */
 
-/* ---- tuning block ---- */
+/* ---- tuning block ----
+ * Launch shape of one launcher call.  With grid_dim = 0 a per-event kernel
+ * runs one workgroup per event (workgroup e handles event e).  With
+ * grid_dim > 0 the launch is persistent: grid_dim workgroups take event
+ * positions k = 0, 1, ... from dev_work_counter and handle event
+ * dev_work_order[k] (k itself when dev_work_order is NULL), so a
+ * largest-first order balances events of very uneven size.
+ * dev_work_counter points to two u32 that must be zero before the first
+ * launch; the last workgroup to finish sets them back to zero, so the same
+ * pair serves the next launch (also inside a captured graph) as long as
+ * launches that share a pair do not overlap.  Outputs do not depend on
+ * the launch shape.  Scans (velo_prefix_sum_launch) use dev_count: when
+ * non-NULL the element count is read there on the device and array_size
+ * is only the capacity. */
 typedef struct {
   uint32_t block_dim; /* threads per workgroup, multiple of 64; 0 = default */
   uint32_t lds_tile;  /* per-wave LDS staging capacity in elements; 0 = default */
+  uint32_t grid_dim;  /* 0: one workgroup per event; >0: persistent workgroups */
+  uint32_t reserved;  /* must be 0 */
+  uint32_t* dev_work_counter;     /* persistent launches: 2 u32, zero at the first launch */
+  const uint32_t* dev_work_order; /* persistent launches: position -> event, or NULL */
+  const uint32_t* dev_count;      /* scans: element count on the device, or NULL */
 } velo_tuning;
 
 /* ---- device constants (Constants.cu:22-53, :239-250) ---- */
@@ -240,7 +258,11 @@ int velo_copy_velo_track_hit_number_launch(const velo_copy_velo_track_hit_number
 
 /* ---- K15: consolidate_velo_tracks (ConsolidateVelo.cu:116-170) ----
  * Also sets dev_accepted_velo_tracks[0, total_tracks) to 1 (the memset of
- * ConsolidateVeloVisitor.cu:42-43, done by a fill kernel) when non-NULL. */
+ * ConsolidateVeloVisitor.cu:42-43) when non-NULL.  With dev_ghost_model
+ * (a velo_ghost_model, velo_ghost.h; no reference counterpart) the track
+ * is scored inline and accepted only when p(ghost) < the model's
+ * threshold; dev_ghost_score, when non-NULL, receives p(ghost) per track.
+ * Hits and states are the same with or without the model. */
 typedef struct {
   const int32_t* dev_atomics_velo;
   const void* dev_tracks;
@@ -251,6 +273,8 @@ typedef struct {
   char* dev_velo_track_hits;
   char* dev_velo_states;
   uint8_t* dev_accepted_velo_tracks;
+  const void* dev_ghost_model;
+  float* dev_ghost_score;
   uint32_t n_events;
   velo_tuning tuning;
 } velo_consolidate_velo_tracks_args;

@@ -40,6 +40,17 @@ typedef struct {
   uint32_t repetitions;        /* 0 = 1 */
   int copy_outputs_to_host;    /* copy the consolidated tracks back */
   velo_tuning tuning[6];       /* estimate, clustering, sort, fill, search, consolidate */
+  /* Batched run (no reference counterpart): no host reads between steps,
+   * buffers sized by bounds (clusters <= events_size / 2, tracks <= 1200 and
+   * track hits <= 1200 * 26 per event), the sequence captured once in a
+   * hipGraph and replayed while the input pointers, n_events, events_size,
+   * dev_work_order and tuning stay the same.  Needs device-resident input.
+   * Outputs are identical to the plain run's. */
+  int batched;
+  const uint32_t* dev_work_order; /* persistent steps: event order (largest first), NULL = identity */
+  /* Ghost filter run inline by consolidation (device velo_ghost_model,
+   * velo_ghost.h), NULL = every track accepted.  No reference counterpart. */
+  const void* dev_ghost_model;
 } velo_sequence_input;
 
 typedef struct {
@@ -52,6 +63,7 @@ typedef struct {
   const uint32_t* host_track_hit_number;   /* T+1 */
   const char* host_track_hits;             /* 16*Ht: SoA x, y, z, LHCbID */
   const char* host_states;                 /* 49*T: SoA 12 floats + bool */
+  const uint8_t* host_accepted;            /* T: dev_accepted_velo_tracks */
   /* device outputs (valid until the next run) */
   const int32_t* dev_atomics_velo;
   const uint32_t* dev_track_hit_number;

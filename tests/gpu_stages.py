@@ -36,6 +36,22 @@ def sync():
     torch.cuda.synchronize()
 
 
+_keep = []
+
+
+def tuning_of(t):
+    return t if isinstance(t, nat.Tuning) else nat.Tuning(*t)
+
+
+def persistent(n, grid, block=0, tile=0, seed=0):
+    """Persistent launch shape: `grid` workgroups take the n events in a
+    seeded shuffled order.  Returns (tuning, counter tensor)."""
+    counters = u32(2)
+    order = dev(np.random.default_rng(seed).permutation(n).astype(np.uint32).view(np.int32))
+    _keep.append((counters, order))
+    return nat.Tuning(block, tile, grid, 0, counters.data_ptr(), order.data_ptr(), None), counters
+
+
 def estimate(events, offsets, consts: nat.DeviceConstants, tuning=(0, 0)):
     n = len(offsets) - 1
     raw, off = dev(events), dev(offsets.view(np.int32))
@@ -43,19 +59,24 @@ def estimate(events, offsets, consts: nat.DeviceConstants, tuning=(0, 0)):
     cand, status = garbage_u32(n * nat.MAX_CANDIDATES_EVENT), u32(n)
     a = nat.EstimateInputSizeArgs(raw.data_ptr(), off.data_ptr(), est.data_ptr(), num.data_ptr(), cnum.data_ptr(),
                                   cand.data_ptr(), None, None, consts.c.dev_velo_candidate_ks, status.data_ptr(),
-                                  n, nat.Tuning(*tuning))
+                                  n, tuning_of(tuning))
     nat.launch("velo_estimate_input_size_launch", a)
     sync()
     return (host(est).view(np.uint32), host(num).view(np.uint32), host(cnum).view(np.uint32),
             host(cand).view(np.uint32), host(status).view(np.uint32))
 
 
-def prefix_sum(arr: np.ndarray) -> np.ndarray:
-    """Exclusive scan of arr[:-1]; total written at [-1]."""
+def prefix_sum(arr: np.ndarray, device_count=None) -> np.ndarray:
+    """Exclusive scan of arr[:-1]; total written at [-1].  With device_count
+    the launch gets capacity len(arr) - 1 and reads the count on the device."""
     n = len(arr) - 1
     d = dev(arr.view(np.int32))
     aux = garbage_u32(nat.prefix_sum_aux_size(n))
-    nat.launch("velo_prefix_sum_launch", nat.PrefixSumArgs(d.data_ptr(), aux.data_ptr(), n, nat.Tuning(0, 0)))
+    t = nat.Tuning(0, 0)
+    if device_count is not None:
+        cnt = dev(np.array([device_count], np.uint32).view(np.int32))
+        t.dev_count = cnt.data_ptr()
+    nat.launch("velo_prefix_sum_launch", nat.PrefixSumArgs(d.data_ptr(), aux.data_ptr(), n, t))
     sync()
     return host(d).view(np.uint32)
 
@@ -69,7 +90,7 @@ def clustering(events, offsets, starts, consts: nat.DeviceConstants, tuning=(0, 
     c = consts.c
     a = nat.MaskedClusteringArgs(raw.data_ptr(), off.data_ptr(), st.data_ptr(), num.data_ptr(), None, None,
                                  cont.data_ptr(), None, None, c.dev_velo_geometry, c.dev_velo_sp_patterns,
-                                 c.dev_velo_sp_fx, c.dev_velo_sp_fy, status.data_ptr(), n, nat.Tuning(*tuning))
+                                 c.dev_velo_sp_fx, c.dev_velo_sp_fy, status.data_ptr(), n, tuning_of(tuning))
     nat.launch("velo_masked_clustering_launch", a)
     sync()
     return host(num).view(np.uint32), host(cont).view(np.uint32), host(status).view(np.uint32)
@@ -80,25 +101,25 @@ def phi_sort(n, starts, nums, container, tuning=(0, 0)):
     st, nm, cont = dev(starts.view(np.int32)), dev(nums.view(np.int32)), dev(container.view(np.int32))
     perm = garbage_u32(total + 1)
     a = nat.CalculatePhiAndSortArgs(st.data_ptr(), nm.data_ptr(), cont.data_ptr(), perm.data_ptr(), n,
-                                    nat.Tuning(*tuning))
+                                    tuning_of(tuning))
     nat.launch("velo_calculate_phi_and_sort_launch", a)
     sync()
     return host(cont).view(np.uint32), host(perm).view(np.uint32)
 
 
-def fill_candidates(n, container, starts, nums):
+def fill_candidates(n, container, starts, nums, tuning=(0, 0)):
     total = int(starts[n * N_MODULES])
     st, nm, cont = dev(starts.view(np.int32)), dev(nums.view(np.int32)), dev(container.view(np.int32))
     h0 = torch.full((2 * total + 2,), -7, dtype=torch.int16, device="cuda")
     h2 = torch.full((2 * total + 2,), -7, dtype=torch.int16, device="cuda")
     a = nat.FillCandidatesArgs(cont.data_ptr(), st.data_ptr(), nm.data_ptr(), h0.data_ptr(), h2.data_ptr(), n,
-                               nat.Tuning(0, 0))
+                               tuning_of(tuning))
     nat.launch("velo_fill_candidates_launch", a)
     sync()
     return host(h0), host(h2)
 
 
-def search_by_triplet(n, container, starts, nums, h0, h2, consts: nat.DeviceConstants):
+def search_by_triplet(n, container, starts, nums, h0, h2, consts: nat.DeviceConstants, tuning=(0, 0)):
     total = int(starts[n * N_MODULES])
     st, nm, cont = dev(starts.view(np.int32)), dev(nums.view(np.int32)), dev(container.view(np.int32))
     d0, d2 = dev(h0), dev(h2)
@@ -110,18 +131,18 @@ def search_by_triplet(n, container, starts, nums, h0, h2, consts: nat.DeviceCons
     a = nat.SearchByTripletArgs(cont.data_ptr(), st.data_ptr(), nm.data_ptr(), tracks.data_ptr(), None,
                                 ttf.data_ptr(), weak.data_ptr(), used.data_ptr(), atomics.data_ptr(), d0.data_ptr(),
                                 d2.data_ptr(), None, consts.c.dev_velo_module_zs, status.data_ptr(), n,
-                                nat.Tuning(0, 0))
+                                tuning_of(tuning))
     nat.launch("velo_search_by_triplet_launch", a)
     sync()
     return (host(tracks), host(weak), host(used), host(atomics), host(status).view(np.uint32))
 
 
-def weak_tracks_adder(n, container, starts, tracks, weak, used, atomics):
+def weak_tracks_adder(n, container, starts, tracks, weak, used, atomics, tuning=(0, 0)):
     st, cont = dev(starts.view(np.int32)), dev(container.view(np.int32))
     dt, dw, du, da = dev(tracks), dev(weak), dev(used), dev(atomics.view(np.int32))
     status = u32(n)
     a = nat.WeakTracksAdderArgs(cont.data_ptr(), st.data_ptr(), dt.data_ptr(), dw.data_ptr(), du.data_ptr(),
-                                da.data_ptr(), status.data_ptr(), n, nat.Tuning(0, 0))
+                                da.data_ptr(), status.data_ptr(), n, tuning_of(tuning))
     nat.launch("velo_weak_tracks_adder_launch", a)
     sync()
     return host(dt), host(da), host(status).view(np.uint32)
@@ -136,17 +157,17 @@ def copy_and_prefix_sum(n, atomics):
     return host(da).view(np.uint32)
 
 
-def copy_track_hit_number(n, tracks, atomics):
+def copy_track_hit_number(n, tracks, atomics, tuning=(0, 0)):
     total = int(atomics[2 * n])
     dt, da = dev(tracks), dev(atomics.view(np.int32))
     hn = garbage_u32(total + 1)
-    a = nat.CopyTrackHitNumberArgs(dt.data_ptr(), da.data_ptr(), hn.data_ptr(), n, nat.Tuning(0, 0))
+    a = nat.CopyTrackHitNumberArgs(dt.data_ptr(), da.data_ptr(), hn.data_ptr(), n, tuning_of(tuning))
     nat.launch("velo_copy_velo_track_hit_number_launch", a)
     sync()
     return host(hn).view(np.uint32)[:total + 1]
 
 
-def consolidate(n, atomics, tracks, hit_number, container, starts, nums):
+def consolidate(n, atomics, tracks, hit_number, container, starts, nums, tuning=(0, 0)):
     total_tracks = int(atomics[2 * n])
     total_hits = int(hit_number[total_tracks])
     da, dt, dh = dev(atomics.view(np.int32)), dev(tracks), dev(hit_number.view(np.int32))
@@ -155,7 +176,8 @@ def consolidate(n, atomics, tracks, hit_number, container, starts, nums):
     states = torch.zeros(49 * total_tracks + 49, dtype=torch.uint8, device="cuda")
     acc = torch.zeros(total_tracks + 1, dtype=torch.uint8, device="cuda")
     a = nat.ConsolidateArgs(da.data_ptr(), dt.data_ptr(), dh.data_ptr(), cont.data_ptr(), st.data_ptr(),
-                            nm.data_ptr(), hits.data_ptr(), states.data_ptr(), acc.data_ptr(), n, nat.Tuning(0, 0))
+                            nm.data_ptr(), hits.data_ptr(), states.data_ptr(), acc.data_ptr(), None, None, n,
+                            tuning_of(tuning))
     nat.launch("velo_consolidate_velo_tracks_launch", a)
     sync()
     return host(hits), host(states), host(acc)

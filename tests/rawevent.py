@@ -79,3 +79,35 @@ def line_pixels(geometry: bytes, x0, y0, z0, tx, ty, forward=True, rmin=5.1, rma
         row = int(ly / 0.055)
         out.append((s, col, row))
     return out
+
+
+def scramble_banks(events: np.ndarray, offsets: np.ndarray, seed: int = 0) -> np.ndarray:
+    """Copy of the raw events with the SP words of some banks out of order.
+
+    Per bank, at random: unchanged, shuffled, reversed, or one SP word copied
+    over its successor (a duplicate address).  Bank sizes and offsets stay
+    the same, so the result is a valid raw event of the same layout whose
+    banks are no longer strictly increasing in SP address.
+    """
+    rng = np.random.default_rng(seed)
+    out = events.copy()
+    for e in range(len(offsets) - 1):
+        lo, hi = int(offsets[e]), int(offsets[e + 1])
+        w = out[lo:hi].view(np.uint32)
+        n_banks = int(w[0])
+        base = n_banks + 2
+        for b in range(n_banks):
+            s = base + int(w[1 + b]) // 4
+            n = int(w[s + 1])
+            if n < 2:
+                continue
+            sps = w[s + 2: s + 2 + n]
+            mode = rng.integers(0, 4)
+            if mode == 1:
+                sps[:] = sps[rng.permutation(n)]
+            elif mode == 2:
+                sps[:] = sps[::-1].copy()
+            elif mode == 3:
+                j = int(rng.integers(0, n - 1))
+                sps[j + 1] = sps[j]
+    return out

@@ -75,6 +75,17 @@ def chain_uneven(geometry, module_zs):
     return Chain(gen.uneven(300, seed=11), geometry, module_zs)
 
 
+@pytest.fixture(scope="module")
+def chain_scrambled(minbias_200, geometry, module_zs):
+    """Banks shuffled, reversed or holding a duplicate SP address: the
+    decoder's linear-scan path for banks that are not strictly increasing."""
+    from cuda_hlt_amd.gen import Events
+    from tests.rawevent import scramble_banks
+    ev = minbias_200
+    scrambled = Events(scramble_banks(ev.events, ev.offsets, seed=5), ev.offsets, ev.mcps, ev.mc_hits)
+    return Chain(scrambled, geometry, module_zs)
+
+
 def tracks_view(raw_bytes, n):
     return np.frombuffer(raw_bytes.tobytes(), po.TRACK_DTYPE)[: n * po.MAX_TRACKS].reshape(n, po.MAX_TRACKS)
 
@@ -88,8 +99,8 @@ def assert_tracks_equal(a, b, counts):
             assert np.array_equal(a[e, t]["hits"][:k], b[e, t]["hits"][:k]), f"event {e} track {t}"
 
 
-def _check_estimate(stages, consts, c: Chain):
-    est, num, cnum, cand, st = stages.estimate(c.ev.events, c.ev.offsets, consts)
+def _check_estimate(stages, consts, c: Chain, tuning=(0, 0)):
+    est, num, cnum, cand, st = stages.estimate(c.ev.events, c.ev.offsets, consts, tuning)
     n = c.n
     np.testing.assert_array_equal(est[: n * N_MODULES], c.est[: n * N_MODULES])
     assert not num.any()
@@ -106,6 +117,10 @@ def test_estimate_input_size(stages, consts, chain):
 
 def test_estimate_input_size_uneven(stages, consts, chain_uneven):
     _check_estimate(stages, consts, chain_uneven)
+
+
+def test_estimate_input_size_unsorted_banks(stages, consts, chain_scrambled):
+    _check_estimate(stages, consts, chain_scrambled)
 
 
 @pytest.mark.parametrize("block,tile", [(64, 64), (128, 256), (512, 1024)])
@@ -148,6 +163,10 @@ def test_masked_clustering_uneven(stages, consts, chain_uneven):
     _check_clustering(stages, consts, chain_uneven)
 
 
+def test_masked_clustering_unsorted_banks(stages, consts, chain_scrambled):
+    _check_clustering(stages, consts, chain_scrambled)
+
+
 @pytest.mark.parametrize("block,tile", [(64, 32), (512, 512)])
 def test_masked_clustering_launch_shapes(stages, consts, chain, block, tile):
     _check_clustering(stages, consts, chain, (block, tile))
@@ -171,8 +190,8 @@ def test_calculate_phi_and_sort_uneven(stages, chain_uneven):
     _check_sort(stages, chain_uneven, (512, 0))
 
 
-def _check_fill(stages, c: Chain):
-    h0, h2 = stages.fill_candidates(c.n, c.cont, c.starts, c.nums)
+def _check_fill(stages, c: Chain, tuning=(0, 0)):
+    h0, h2 = stages.fill_candidates(c.n, c.cont, c.starts, c.nums, tuning)
     total = int(c.starts[c.n * N_MODULES])
     sel = np.zeros(total, bool)
     for e in range(c.n):
@@ -193,9 +212,10 @@ def test_fill_candidates_uneven(stages, chain_uneven):
     _check_fill(stages, chain_uneven)
 
 
-def _check_search(stages, consts, c: Chain):
+def _check_search(stages, consts, c: Chain, tuning=(0, 0)):
     n = c.n
-    tracks, weak, used, atomics, st = stages.search_by_triplet(n, c.cont, c.starts, c.nums, c.h0, c.h2, consts)
+    tracks, weak, used, atomics, st = stages.search_by_triplet(n, c.cont, c.starts, c.nums, c.h0, c.h2, consts,
+                                                               tuning)
     atomics = atomics.view(np.int32)
     np.testing.assert_array_equal(atomics[:n], c.atomics_s[:n])
     assert_tracks_equal(tracks_view(tracks, n), c.tracks_s.reshape(n, po.MAX_TRACKS), c.atomics_s[:n])
@@ -265,6 +285,51 @@ def test_coordinates_within_tolerance(stages, consts, chain):
         g = cont[s * total: (s + 1) * total][chain.mask].view(np.float32)
         o = chain.cont_raw[s * total: (s + 1) * total][chain.mask].view(np.float32)
         assert np.max(np.abs(g.astype(np.float64) - o)) <= 1e-5
+
+
+@pytest.mark.parametrize("count,capacity", [(0, 5), (1, 5), (2048, 5000), (2049, 5000), (4096, 4097), (9000, 9000)])
+def test_prefix_sum_device_count(stages, count, capacity):
+    """The scan reads its element count on the device; elements past it are
+    neither read nor written, and the total lands at [count]."""
+    rng = np.random.default_rng(count)
+    a = rng.integers(0, 1000, capacity + 1).astype(np.uint32)
+    got = stages.prefix_sum(a, device_count=count)
+    want = a.copy()
+    want[0] = 0
+    want[1: count + 1] = np.cumsum(a[:count].astype(np.uint64)).astype(np.uint32)
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("grid", [1, 7, 256])
+def test_stages_persistent_shuffled_order(stages, consts, chain_uneven, grid):
+    """Persistent launches (grid workgroups taking events from a counter in
+    a shuffled order) give the one-workgroup-per-event outputs, and every
+    launch leaves the shared counter pair at zero for the next."""
+    c = chain_uneven
+    n = c.n
+    t, counters = stages.persistent(n, grid, seed=grid)
+    _check_estimate(stages, consts, c, t)
+    _check_clustering(stages, consts, c, t)
+    _check_sort(stages, c, t)
+    _check_fill(stages, c, t)
+    _check_search(stages, consts, c, t)
+    tracks, atomics, st = stages.weak_tracks_adder(n, c.cont, c.starts, c.tracks_s.view(np.uint8), c.weak.view(np.uint8),
+                                                   c.used, c.atomics_s, t)
+    np.testing.assert_array_equal(atomics.view(np.int32)[:n], c.atomics_w[:n])
+    T = int(c.atomics[2 * n])
+    hn = stages.copy_track_hit_number(n, c.tracks.view(np.uint8), c.atomics, t)
+    np.testing.assert_array_equal(hn[:T], c.hn_raw[:T])
+    hits, states, acc = stages.consolidate(n, c.atomics, c.tracks.view(np.uint8), c.hn, c.cont, c.starts, c.nums, t)
+    Ht = int(c.hn[T])
+    np.testing.assert_array_equal(hits[: 16 * Ht], c.hits[: 16 * Ht])
+    np.testing.assert_array_equal(states[: 49 * T], c.states[: 49 * T])
+    assert not stages.host(counters).any()
+
+
+def test_persistent_launch_without_counters_is_rejected(stages, consts, chain):
+    from cuda_hlt_amd import native as nat
+    with pytest.raises(nat.VeloError):
+        stages.estimate(chain.ev.events, chain.ev.offsets, consts, tuning=(0, 0, 64))
 
 
 def test_empty_batch_is_a_no_op(stages, consts):
