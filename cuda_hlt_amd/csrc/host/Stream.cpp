@@ -12,7 +12,9 @@ hipError_t Stream::initialize(
   unsigned param_stream_number,
   const Constants& param_constants)
 {
-  hipCheck(hipStreamCreate(&stream)); // default priority
+  // Default priority; non-blocking so that synchronous copies other threads
+  // issue on the legacy stream never wait on (or join a capture of) it.
+  hipCheck(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   hipCheck(hipEventCreateWithFlags(&event, hipEventDisableTiming));
   stream_number = param_stream_number;
   do_print_memory_manager = print_memory_manager;
@@ -132,26 +134,30 @@ hipError_t Stream::run_batched(const RuntimeOptions& ro)
     step_names[1] = "end";
   }
 
-  // totals (the plain sequence reads them between steps)
+  // totals (the plain sequence reads them between steps), on this stream
   unsigned* hb = host_buffers.host_number_of_selected_events;
   hb[0] = n;
-  hipCheck(hipMemcpy(hb + 1, output<dev_estimated_input_size>() + (size_t) n * VELO_N_MODULES, sizeof(unsigned),
-                     hipMemcpyDeviceToHost));
-  hipCheck(hipMemcpy(hb + 2, reinterpret_cast<unsigned*>(output<dev_atomics_velo>()) + 2 * (size_t) n,
-                     sizeof(unsigned), hipMemcpyDeviceToHost));
-  hipCheck(hipMemcpy(hb + 3, output<dev_velo_track_hit_number>() + hb[2], sizeof(unsigned), hipMemcpyDeviceToHost));
+  auto d2h = [&](void* dst, const void* src, size_t bytes) {
+    hipCheck(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
+  };
+  auto wait = [&]() {
+    hipCheck(hipEventRecord(event, stream));
+    hipCheck(hipEventSynchronize(event));
+  };
+  d2h(hb + 1, output<dev_estimated_input_size>() + (size_t) n * VELO_N_MODULES, sizeof(unsigned));
+  d2h(hb + 2, reinterpret_cast<unsigned*>(output<dev_atomics_velo>()) + 2 * (size_t) n, sizeof(unsigned));
+  wait();
+  d2h(hb + 3, output<dev_velo_track_hit_number>() + hb[2], sizeof(unsigned));
+  wait();
   if (ro.copy_outputs_to_host) {
     const size_t t = hb[2], h = hb[3];
     host_buffers.ensure_outputs(t, h);
-    hipCheck(hipMemcpy(host_buffers.host_atomics_velo, output<dev_atomics_velo>(), (2 * (size_t) n + 1) * sizeof(unsigned),
-                       hipMemcpyDeviceToHost));
-    hipCheck(hipMemcpy(host_buffers.host_velo_track_hit_number, output<dev_velo_track_hit_number>(),
-                       (t + 1) * sizeof(unsigned), hipMemcpyDeviceToHost));
-    hipCheck(hipMemcpy(host_buffers.host_velo_track_hits, output<dev_velo_track_hits>(), h * 16, hipMemcpyDeviceToHost));
-    hipCheck(hipMemcpy(host_buffers.host_velo_states, output<dev_velo_states>(), t * VELO_STATE_BYTES,
-                       hipMemcpyDeviceToHost));
-    hipCheck(hipMemcpy(host_buffers.host_accepted_velo_tracks, output<dev_accepted_velo_tracks>(), t,
-                       hipMemcpyDeviceToHost));
+    d2h(host_buffers.host_atomics_velo, output<dev_atomics_velo>(), (2 * (size_t) n + 1) * sizeof(unsigned));
+    d2h(host_buffers.host_velo_track_hit_number, output<dev_velo_track_hit_number>(), (t + 1) * sizeof(unsigned));
+    d2h(host_buffers.host_velo_track_hits, output<dev_velo_track_hits>(), h * 16);
+    d2h(host_buffers.host_velo_states, output<dev_velo_states>(), t * VELO_STATE_BYTES);
+    d2h(host_buffers.host_accepted_velo_tracks, output<dev_accepted_velo_tracks>(), t);
+    wait();
   }
   return hipSuccess;
 }

@@ -141,25 +141,30 @@ __device__ inline float seed_best(
   const int f2 = h2c[2 * h1], l2 = h2c[2 * h1 + 1];
   float best = kFltMax;
   uint32_t bb0 = 0, bb2 = 0;
+  // The reference evaluates every condition for every (h0, h2); the slope
+  // cut depends on h0 alone and the scatter only matters once both
+  // tolerance cuts pass, so they are tested first.  Same arithmetic, same
+  // iteration order, same result.
   for (int h0 = f0; h0 < l0; ++h0) {
     if (used.test(h0)) continue;
     const float h0x = H.x(h0), h0y = H.y(h0), h0z = H.z(h0);
+    const float dmax = kMaxSlope * (h0z - h1z);
+    if (!(fabsf(h1x - h0x) < dmax && fabsf(h1y - h0y) < dmax)) continue;
+    const float dz10 = h1z - h0z;
     for (int h2 = f2; h2 < l2; ++h2) {
       if (used.test(h2)) continue;
       const float h2x = H.x(h2), h2y = H.y(h2), h2z = H.z(h2);
-      const float dmax = kMaxSlope * (h0z - h1z);
-      const float dz21 = h2z - h1z;
-      const float scatter_denom2 = 1.0f / (dz21 * dz21);
-      const float z2_tz = (h2z - h0z) / (h1z - h0z);
+      const float z2_tz = (h2z - h0z) / dz10;
       const float x = h0x + (h1x - h0x) * z2_tz;
       const float y = h0y + (h1y - h0y) * z2_tz;
       const float dx = x - h2x;
       const float dy = y - h2y;
+      if (!(fabsf(dx) < kTolerance && fabsf(dy) < kTolerance)) continue;
+      const float dz21 = h2z - h1z;
+      const float scatter_denom2 = 1.0f / (dz21 * dz21);
       const float scatter_num = (dx * dx) + (dy * dy);
       const float scatter = scatter_num * scatter_denom2;
-      const bool cond = fabsf(h1x - h0x) < dmax && fabsf(h1y - h0y) < dmax && fabsf(dx) < kTolerance &&
-                        fabsf(dy) < kTolerance && scatter < kMaxScatterSeeding && scatter < best;
-      if (cond) {
+      if (scatter < kMaxScatterSeeding && scatter < best) {
         best = scatter;
         bb0 = (uint32_t) h0;
         bb2 = (uint32_t) h2;
@@ -360,18 +365,18 @@ __device__ inline void forwarding(
         const float dz = h2z - h0z;
         const float predx = tx * dz;
         const float predy = ty * dz;
-        const float dz21 = h2z - h1z;
-        const float scatter_denom2 = 1.0f / (dz21 * dz21);
         const float xp = h0x + predx;
         const float dx = fabsf(xp - h2x);
         const float yp = h0y + predy;
         const float dy = fabsf(yp - h2y);
+        // the scatter (a division) only matters inside both tolerances
+        if (!(dx < kTolerance && dy < kTolerance)) continue;
+        const float dz21 = h2z - h1z;
+        const float scatter_denom2 = 1.0f / (dz21 * dz21);
         const float scatter_num = (dx * dx) + (dy * dy);
         const float scatter = scatter_num * scatter_denom2;
-        const bool cond = dx < kTolerance && dy < kTolerance && scatter < kMaxScatterForwarding;
-        const float fit = cond ? scatter : kFltMax;
-        if (fit < best) {
-          best = fit;
+        if (scatter < kMaxScatterForwarding && scatter < best) {
+          best = scatter;
           best_h2 = (uint32_t) h2;
         }
       }

@@ -129,7 +129,7 @@ class PlainSequence:
         accepted = torch.empty(max(n_tracks, 1), dtype=torch.uint8, device=self.device)
         nat.launch("velo_consolidate_velo_tracks_launch", nat.ConsolidateArgs(
             atomics.data_ptr(), tracks.data_ptr(), hn.data_ptr(), cont.data_ptr(), est.data_ptr(), num.data_ptr(),
-            track_hits.data_ptr(), states.data_ptr(), accepted.data_ptr(), N, self._t("consolidate")), s)
+            track_hits.data_ptr(), states.data_ptr(), accepted.data_ptr(), None, None, N, self._t("consolidate")), s)
         return VeloOutput(N, atomics[: 2 * N + 1], hn, track_hits, states, accepted, status, est, num, cont,
                           n_tracks, n_hits)
 

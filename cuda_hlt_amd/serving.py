@@ -98,7 +98,7 @@ def cut_results(host, n_events: int, requests, first_event: list):
         out.append({"tracks_per_event": host.atomics[e0: e0 + r.n].copy(),
                     "hit_number": (host.hit_number[t0: t1 + 1] - h0).astype(np.uint32),
                     "hits": hits, "states": states,
-                    "accepted": None if host.accepted is None else host.accepted[t0:t1].copy()})
+                    "accepted": None if getattr(host, "accepted", None) is None else host.accepted[t0:t1].copy()})
     return out
 
 
@@ -163,6 +163,15 @@ class Slot:
         return cut_results(self.seq.host_tracks(), bucket, requests, first), bucket
 
     def close(self):
+        """Frees the buffers before the sequence's stream: the pinned staging
+        buffers were copied from on that stream, and freeing them records an
+        event on it."""
+        import torch
+        torch.cuda.synchronize()
+        self.host_events = self.host_offsets = self.host_order = None
+        self.dev_events = self.dev_offsets = self.dev_order = None
+        torch.cuda.synchronize()
+        self.stream = None
         self.seq.close()
 
 
