@@ -22,7 +22,7 @@ import numpy as np
 SPACE = {
     "estimate_input_size": [(b, t) for b in (128, 256, 512, 1024) for t in (2048, 3072, 4096, 6144, 8192)],
     "masked_clustering": [(b, t) for b in (128, 256, 512, 1024) for t in (2048, 3072, 4096, 6144, 8192)],
-    "calculate_phi_and_sort": [(b, 0) for b in (64, 128, 256, 512, 1024)],
+    "calculate_phi_and_sort": [(b, t) for b in (64, 128, 256, 512) for t in (1, 1024, 2048, 3072)],
     "fill_candidates": [(b, t) for b in (64, 128, 256, 512) for t in (1024, 2048, 4096)],
     "search_by_triplet": [(64, t) for t in (1, 256, 512, 768, 1024, 2048)],
     "consolidate": [(b, t) for b in (64, 128, 256, 512) for t in (1, 1024, 2048, 3072)],
@@ -34,20 +34,22 @@ BATCHED_BLOCKS = (64, 128, 256, 512)
 BATCHED_TILES = {
     "estimate_input_size": (4096, 6144, 8192, 12288),
     "masked_clustering": (4096, 6144, 8192, 12288),
-    "calculate_phi_and_sort": (0,),
+    "calculate_phi_and_sort": (1, 1024, 2048, 3072),
     "fill_candidates": (1024, 2048, 4096, 8192),
     "search_by_triplet": (256, 512, 1024, 2048, 4096),
     "consolidate": (256, 1024, 2048, 3072),
 }
 BATCHED_GRIDS = (0, 1024, 2048, 4096, 8192)
 
-# Launch shapes of the batched path when no tuning file is given
+# Launch shapes of the batched path when no tuning file is given: the
+# autotuner's result on MI355X for 20 000 uneven events
+# (cuda_hlt_amd/tuning/mi355x_uneven_batched.json)
 BATCHED_DEFAULT = {
-    "estimate_input_size": (512, 8192, 0),
-    "masked_clustering": (512, 8192, 0),
-    "calculate_phi_and_sort": (512, 0, 0),
-    "fill_candidates": (512, 4096, 0),
-    "search_by_triplet": (64, 256, 0),
+    "estimate_input_size": (256, 4096, 2048),
+    "masked_clustering": (256, 6144, 2048),
+    "calculate_phi_and_sort": (256, 0, 2048),
+    "fill_candidates": (512, 8192, 0),
+    "search_by_triplet": (64, 256, 8192),
     "consolidate": (512, 3072, 0),
 }
 

@@ -1,13 +1,16 @@
 // calculate_phi_and_sort (CalculatePhiAndSort.cu:6-52) and fill_candidates
 // (FillCandidates.cu:8-97) for gfx950.
 //
-// One workgroup per event, wave w owns modules w, w+W, ...  Phi sort: the
-// wave computes the module's phis into its LDS slice, ranks every hit
-// against all others (stable: ties broken by slot, CalculatePhi.cu:47-53),
-// then gathers x,y,z,id by the permutation into registers and, after a wave
-// barrier, writes them to their sorted positions.  Modules larger than the
-// register budget (512 hits; the reference caps a module at 500) fall back
-// to the reference's slot-by-slot permutation through global memory.
+// One workgroup per event, wave w owns modules w, w+W, ...  Phi sort: when
+// the event fits (tuning.lds_tile hits; off by default) the workgroup first
+// copies its x, y, z, id slots to LDS in one coalesced pass; the wave then
+// computes the module's phis into its LDS slice, ranks every hit against
+// all others (stable: ties broken by slot, CalculatePhi.cu:47-53) and
+// writes x, y, z, id from LDS to their sorted positions.  Larger events
+// gather by the permutation from global memory into registers instead, and
+// modules larger than the register budget (512 hits; the reference caps a
+// module at 500) fall back to the reference's slot-by-slot permutation
+// through global memory.
 #include "velo_device.h"
 
 namespace velo {
@@ -16,6 +19,8 @@ constexpr uint32_t kSortCap = 512;
 constexpr uint32_t kSortChunks = kSortCap / kWave;
 constexpr uint32_t kSortBlock = 256;
 constexpr uint32_t kHoleBits = 0x7FC00000u; // quiet NaN
+constexpr uint32_t kDefaultSortStage = 0;  // hits of an event staged in LDS (0: never)
+constexpr size_t kMaxSortLds = 65536;
 
 __device__ inline float module_phi(uint32_t m, float x, float y) { return (m & 1u) ? phi_odd(x, y) : phi_even(x, y); }
 
@@ -35,6 +40,21 @@ __global__ void __launch_bounds__(1024) calculate_phi_and_sort_kernel(velo_calcu
     uint32_t* TMP = PHI + total;
     float* phi = reinterpret_cast<float*>(lds) + wave * kSortCap;
     uint16_t* perm = reinterpret_cast<uint16_t*>(lds + waves * kSortCap) + wave * kSortCap;
+    // the event's x, y, z, id (slots 0-3), when they fit
+    uint32_t* ev = lds + waves * kSortCap + waves * kSortCap / 2;
+    const uint32_t cap = a.tuning.lds_tile;
+    const uint32_t off = a.dev_module_cluster_start[e * VELO_N_MODULES];
+    const uint32_t ev_hits = a.dev_module_cluster_start[(e + 1) * VELO_N_MODULES] - off;
+    const bool staged = ev_hits <= cap;
+    if (staged) {
+      for (uint32_t i = threadIdx.x; i < ev_hits; i += blockDim.x) {
+        ev[i] = X[off + i];
+        ev[cap + i] = Y[off + i];
+        ev[2 * cap + i] = Z[off + i];
+        ev[3 * cap + i] = ID[off + i];
+      }
+      __syncthreads();
+    }
 
     for (uint32_t m = wave; m < VELO_N_MODULES; m += waves) {
       const uint32_t s = a.dev_module_cluster_start[e * VELO_N_MODULES + m];
@@ -51,7 +71,33 @@ __global__ void __launch_bounds__(1024) calculate_phi_and_sort_kernel(velo_calcu
         TMP[p] = kHoleBits;
       }
       if (n == 0) continue;
-      if (n <= kSortCap) {
+      if (staged && n <= kSortCap) {
+        const uint32_t r = s - off;
+        for (uint32_t i = lane; i < n; i += kWave)
+          phi[i] = module_phi(m, __uint_as_float(ev[r + i]), __uint_as_float(ev[cap + r + i]));
+        wave_sync();
+        for (uint32_t i = lane; i < n; i += kWave) {
+          const float p = phi[i];
+          uint32_t pos = 0;
+          for (uint32_t j = 0; j < n; ++j) {
+            const float q = phi[j];
+            pos += (p > q) || (p == q && i > j);
+          }
+          perm[pos] = (uint16_t) i;
+          a.dev_hit_permutation[s + pos] = s + i;
+          PHI[s + pos] = __float_as_uint(p);
+        }
+        wave_sync();
+        // SortByPhi.cu:21-33 order: X->5, Y->0, Z->1, ID->2; every read is from LDS
+        for (uint32_t p = lane; p < n; p += kWave) {
+          const uint32_t q = r + perm[p];
+          TMP[s + p] = ev[q];
+          X[s + p] = ev[cap + q];
+          Y[s + p] = ev[2 * cap + q];
+          Z[s + p] = ev[3 * cap + q];
+        }
+      }
+      else if (n <= kSortCap) {
         for (uint32_t i = lane; i < n; i += kWave) phi[i] = module_phi(m, __uint_as_float(X[s + i]), __uint_as_float(Y[s + i]));
         wave_sync();
         for (uint32_t i = lane; i < n; i += kWave) {
@@ -208,8 +254,15 @@ int velo_calculate_phi_and_sort_launch(const velo_calculate_phi_and_sort_args* a
   if (!schedule_ok(a->tuning)) return (int) hipErrorInvalidValue;
   const uint32_t block = a->tuning.block_dim ? a->tuning.block_dim : kSortBlock;
   if (block % kWave || block > 1024) return (int) hipErrorInvalidValue;
-  const size_t shmem = (size_t) (block / kWave) * kSortCap * (sizeof(float) + sizeof(uint16_t));
-  hipLaunchKernelGGL(calculate_phi_and_sort_kernel, dim3(event_grid(a->tuning, a->n_events)), dim3(block), shmem, (hipStream_t) stream, *a);
+  velo_calculate_phi_and_sort_args args = *a;
+  const size_t sort_lds = (size_t) (block / kWave) * kSortCap * (sizeof(float) + sizeof(uint16_t));
+  // event staging: as requested, within 64 KB of LDS per workgroup
+  const size_t room = sort_lds < kMaxSortLds ? (kMaxSortLds - sort_lds) / (4 * sizeof(uint32_t)) : 0;
+  const size_t want = args.tuning.lds_tile ? args.tuning.lds_tile : kDefaultSortStage;
+  args.tuning.lds_tile = (uint32_t) (want < room ? want : room);
+  const size_t shmem = sort_lds + (size_t) args.tuning.lds_tile * 4 * sizeof(uint32_t);
+  hipLaunchKernelGGL(calculate_phi_and_sort_kernel, dim3(event_grid(args.tuning, args.n_events)), dim3(block), shmem,
+                     (hipStream_t) stream, args);
   return (int) hipGetLastError();
 }
 

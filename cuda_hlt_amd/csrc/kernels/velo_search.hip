@@ -270,18 +270,116 @@ __device__ inline void push_ttf(bool want, uint32_t w0, uint32_t w1, const Ring&
   }
 }
 
-// Seeds the first k (<= 64) staged h1 hits, one per lane.
+// LDS of one seeding batch: per staged h1 the first (h1, h0) item and the
+// best packed key found so far.
+struct SeedScratch {
+  uint32_t item_begin[kWave + 1];
+  uint64_t best[kWave];
+};
+
+// Items of one seeding batch must fit the key's 17-bit item field.
+constexpr uint32_t kMaxSeedItems = 1u << 17;
+
+// Best h2 for the item (h1, h0): the inner loop of seed_best for one h0.
+// Returns the packed key (scatter bits << 33 | item << 16 | h2), or ~0
+// when no h2 qualifies.  Scatters are non-negative (sign bit 0), so their
+// bit patterns order like the floats; among equal scatters the lower item
+// (smaller h0), then the first h2, wins, as in seed_best's loop order.
+__device__ inline uint64_t seed_item(
+  uint32_t h1, int h0, uint32_t item, const EventHits& H, const int16_t* h2c, const UsedSet& used)
+{
+  if (used.test(h0)) return ~0ull;
+  const float h1x = H.x(h1), h1y = H.y(h1), h1z = H.z(h1);
+  const float h0x = H.x(h0), h0y = H.y(h0), h0z = H.z(h0);
+  const float dmax = kMaxSlope * (h0z - h1z);
+  if (!(fabsf(h1x - h0x) < dmax && fabsf(h1y - h0y) < dmax)) return ~0ull;
+  const float dz10 = h1z - h0z;
+  const int f2 = h2c[2 * h1], l2 = h2c[2 * h1 + 1];
+  float best = kFltMax;
+  uint32_t bb2 = 0;
+  for (int h2 = f2; h2 < l2; ++h2) {
+    if (used.test(h2)) continue;
+    const float h2x = H.x(h2), h2y = H.y(h2), h2z = H.z(h2);
+    const float z2_tz = (h2z - h0z) / dz10;
+    const float x = h0x + (h1x - h0x) * z2_tz;
+    const float y = h0y + (h1y - h0y) * z2_tz;
+    const float dx = x - h2x;
+    const float dy = y - h2y;
+    if (!(fabsf(dx) < kTolerance && fabsf(dy) < kTolerance)) continue;
+    const float dz21 = h2z - h1z;
+    const float scatter_denom2 = 1.0f / (dz21 * dz21);
+    const float scatter_num = (dx * dx) + (dy * dy);
+    const float scatter = scatter_num * scatter_denom2;
+    if (scatter < kMaxScatterSeeding && scatter < best) {
+      best = scatter;
+      bb2 = (uint32_t) h2;
+    }
+  }
+  if (!(best < kFltMax)) return ~0ull;
+  return ((uint64_t) __float_as_uint(best) << 33) | ((uint64_t) item << 16) | bb2;
+}
+
+// Seeds the first k (<= 64) staged h1 hits.  The (h1, h0) pairs of the
+// batch are spread over the lanes (one pair per lane, its h2 window in a
+// loop); each pair's best is folded into its h1 with an LDS 64-bit minimum.
+// The result per h1 equals seed_best's, which runs instead (one h1 per
+// lane) for a batch with more pairs than the key can number.
 __device__ inline uint32_t seed_batch(
   const uint16_t* stage, uint32_t k, const EventHits& H, const int16_t* h0c, const int16_t* h2c, const UsedSet& used,
-  const Ring& ring, SearchState& st)
+  const Ring& ring, SearchState& st, SeedScratch& sc)
 {
   const uint32_t lane = lane_id();
-  bool emit = false;
-  uint32_t h1 = 0, b0 = 0, b2 = 0;
+  uint32_t h1 = 0, c = 0;
+  int f0 = 0;
   if (lane < k) {
     h1 = stage[lane];
-    emit = seed_best(h1, H, h0c, h2c, used, &b0, &b2) < kFltMax;
+    f0 = h0c[2 * h1];
+    const int l0 = h0c[2 * h1 + 1];
+    c = l0 > f0 ? (uint32_t) (l0 - f0) : 0u;
   }
+  const uint32_t inc = wave_inclusive_scan(c);
+  const uint32_t items = __shfl(inc, kWave - 1, kWave);
+  if (items >= kMaxSeedItems) {
+    bool emit = false;
+    uint32_t b0 = 0, b2 = 0;
+    if (lane < k) emit = seed_best(h1, H, h0c, h2c, used, &b0, &b2) < kFltMax;
+    const uint32_t cnt = (uint32_t) __popcll(ballot(emit));
+    st.n_tracklets += cnt;
+    push_ttf(emit, h1 | (b2 << 16), b0 | (3u << 16) | kTtfTrackletFlag, ring, st);
+    return cnt;
+  }
+  sc.item_begin[lane] = inc - c;
+  if (lane == kWave - 1) sc.item_begin[kWave] = inc;
+  sc.best[lane] = ~0ull;
+  wave_sync();
+  for (uint32_t base = 0; base < items; base += kWave) {
+    const uint32_t i = base + lane;
+    if (i < items) {
+      // the staged h1 whose items contain i: last j < k with item_begin[j] <= i
+      uint32_t lo = 0, hi = k - 1;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (sc.item_begin[mid] <= i) lo = mid;
+        else hi = mid - 1;
+      }
+      const uint32_t j = lo;
+      const uint32_t hj = stage[j];
+      const int h0 = h0c[2 * hj] + (int) (i - sc.item_begin[j]);
+      const uint64_t key = seed_item(hj, h0, i, H, h2c, used);
+      if (key != ~0ull) atomicMin(&sc.best[j], key);
+    }
+  }
+  wave_sync();
+  bool emit = false;
+  uint32_t b0 = 0, b2 = 0;
+  if (lane < k) {
+    const uint64_t key = sc.best[lane];
+    emit = key != ~0ull;
+    const uint32_t item = (uint32_t) (key >> 16) & 0xFFFFu;
+    b0 = (uint32_t) (f0 + (int) (item - sc.item_begin[lane]));
+    b2 = (uint32_t) key & 0xFFFFu;
+  }
+  wave_sync(); // the scratch is reused by the next batch
   const uint32_t cnt = (uint32_t) __popcll(ballot(emit));
   st.n_tracklets += cnt;
   push_ttf(emit, h1 | (b2 << 16), b0 | (3u << 16) | kTtfTrackletFlag, ring, st);
@@ -291,7 +389,7 @@ __device__ inline uint32_t seed_batch(
 // TrackSeeding.cu:7-363: h1 candidates of module_data[2] then [3].
 __device__ inline uint32_t seeding(
   const ModuleData* md, uint16_t* stage, const EventHits& H, const int16_t* h0c, const int16_t* h2c,
-  const UsedSet& used, const Ring& ring, SearchState& st)
+  const UsedSet& used, const Ring& ring, SearchState& st, SeedScratch& sc)
 {
   const uint32_t lane = lane_id();
   uint32_t staged = 0, tracklets = 0;
@@ -308,7 +406,7 @@ __device__ inline uint32_t seeding(
       staged += (uint32_t) __popcll(m);
       wave_sync();
       if (staged >= kWave) {
-        tracklets += seed_batch(stage, kWave, H, h0c, h2c, used, ring, st);
+        tracklets += seed_batch(stage, kWave, H, h0c, h2c, used, ring, st, sc);
         wave_sync();
         const uint32_t rest = staged - kWave;
         uint16_t carry = 0;
@@ -320,7 +418,7 @@ __device__ inline uint32_t seeding(
       }
     }
   }
-  if (staged) tracklets += seed_batch(stage, staged, H, h0c, h2c, used, ring, st);
+  if (staged) tracklets += seed_batch(stage, staged, H, h0c, h2c, used, ring, st, sc);
   wave_sync();
   return tracklets;
 }
@@ -454,6 +552,7 @@ __global__ void __launch_bounds__(64) search_by_triplet_kernel(velo_search_by_tr
 {
   __shared__ uint32_t used_bits[kUsedWords];
   __shared__ uint16_t stage[2 * kWave];
+  __shared__ SeedScratch seed_scratch;
 
   const uint32_t N = a.n_events;
   for_each_event(a.tuning, a.n_events, [&](const uint32_t e) {
@@ -498,7 +597,7 @@ __global__ void __launch_bounds__(64) search_by_triplet_kernel(velo_search_by_tr
     uint32_t first = VELO_N_MODULES - 1;
     load_modules(md, first, starts, nums, off, a.dev_velo_module_zs);
     stage_window(first);
-    uint32_t prev_tracklets = seeding(md, stage, H, h0c, h2c, used, ring, st);
+    uint32_t prev_tracklets = seeding(md, stage, H, h0c, h2c, used, ring, st, seed_scratch);
     uint32_t cur_begin = 0, cur_n = st.next_n;
     first -= 2;
     while (first >= 4) {
@@ -509,7 +608,7 @@ __global__ void __launch_bounds__(64) search_by_triplet_kernel(velo_search_by_tr
       st.cap = VELO_TTF_MODULO - cur_n;
       forwarding(md, cur_begin, cur_n, H, used, ring, tracks, weak, st);
       wave_sync(); // hit_used marks of forwarding are read by seeding
-      const uint32_t tl = seeding(md, stage, H, h0c, h2c, used, ring, st);
+      const uint32_t tl = seeding(md, stage, H, h0c, h2c, used, ring, st, seed_scratch);
       if (tl + prev_tracklets > VELO_TTF_MODULO) st.status |= VELO_STATUS_TRACKLETS_OVERFLOW;
       prev_tracklets = tl;
       cur_begin = st.next_begin;

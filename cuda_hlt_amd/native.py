@@ -120,6 +120,13 @@ def lib():
     if _lib is None:
         if not LIB_PATH.exists():
             raise VeloError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        # One HIP runtime per process: PyTorch ships its own libamdhip64 and
+        # loads it by file name, so it has to be loaded before this library
+        # (whose libamdhip64.so.7 dependency then resolves to the same copy).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(str(LIB_PATH))
         for name, st in LAUNCHERS.items():
             f = getattr(L, name)

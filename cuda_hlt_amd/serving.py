@@ -181,6 +181,8 @@ class Server:
     def __init__(self, geometry, zs, hbm_budget_mb=65536, max_slots=4, slot_events=4096, slot_bytes=None,
                  batched=True, tuning=None, ghost_model=None):
         slot_bytes = slot_bytes or slot_events * 40000
+        if tuning is None and batched:
+            tuning = dict(autotune.BATCHED_DEFAULT)
         per_slot = autotune.batched_reserve_mb(slot_events, slot_bytes)
         self.n_slots = max(1, min(max_slots, hbm_budget_mb // per_slot))
         self.slot_events, self.slot_bytes = slot_events, slot_bytes

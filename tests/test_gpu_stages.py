@@ -186,6 +186,13 @@ def test_calculate_phi_and_sort(stages, chain):
     _check_sort(stages, chain)
 
 
+@pytest.mark.parametrize("block,tile", [(64, 1), (256, 1024), (512, 4096), (1024, 0)])
+def test_calculate_phi_and_sort_launch_shapes(stages, chain_uneven, block, tile):
+    """Events staged in LDS (up to `tile` hits) and events sorted from
+    global memory give the same result."""
+    _check_sort(stages, chain_uneven, (block, tile))
+
+
 def test_calculate_phi_and_sort_uneven(stages, chain_uneven):
     _check_sort(stages, chain_uneven, (512, 0))
 
