@@ -61,7 +61,7 @@ STEP_KERNELS = {
     "velo_fill_candidates_t": ["fill_candidates_kernel"],
     "velo_search_by_triplet_t": ["search_by_triplet_kernel"],
     "velo_weak_tracks_adder_t": ["weak_tracks_adder_kernel"],
-    "consolidate_velo_tracks_t": ["consolidate_kernel"],
+    "consolidate_velo_tracks_t": ["consolidate_kernel<false>"],
 }
 
 
@@ -123,12 +123,18 @@ def cpu_baseline(ev, geometry, zs, budget_s=12.0, threads=None):
             "host_cpu": cpu, "host_threads_available": os.cpu_count()}
 
 
+GHOST_MODEL = ROOT / "cuda_hlt_amd" / "filters" / "velo_ghost_mi355x.npz"
+
+
 def batched_run(args, geometry, zs):
     """Uneven events (nu ~ U(2,11) per event) in one batch: the sequence
     captured once in a hipGraph, persistent steps in largest-first order,
-    one graph launch per step; outputs are those of the plain run."""
+    one graph launch per step.  Timed without and with the ghost filter
+    (inline in consolidation); the checker compares the filtered output's
+    efficiency and ghost rate with the unfiltered one."""
     import torch
-    from cuda_hlt_amd import autotune, gen
+    from cuda_hlt_amd import autotune, checker, gen
+    from cuda_hlt_amd.ghost import GhostModel
     from cuda_hlt_amd.stream import VeloSequence
     n = args.batched_events
     t0 = time.perf_counter()
@@ -142,39 +148,60 @@ def batched_run(args, geometry, zs):
     need_mb = autotune.batched_reserve_mb(n, ev.events.nbytes)
     seq = VeloSequence(geometry, zs, max_events=n, reserve_mb=need_mb)
     seq.enable_step_timing(True)
+    model_path = Path(args.ghost_model) if args.ghost_model else GHOST_MODEL
+    model = GhostModel.load(model_path) if model_path.exists() else None
+    model_dev = model.to_device() if model is not None else None
 
-    def run():
+    def run(filtered, copy=False):
         return seq.run(n, dev_events=raw.data_ptr(), dev_offsets=off.data_ptr(), events_size=ev.events.nbytes,
-                       copy_outputs=False, batched=True, tuning=tuning, dev_work_order=order.data_ptr())
+                       copy_outputs=copy, batched=True, tuning=tuning, dev_work_order=order.data_ptr(),
+                       dev_ghost_model=model_dev.data_ptr() if filtered else None)
 
-    t0 = time.perf_counter()
-    run()  # capture + first replay
-    capture_s = time.perf_counter() - t0
-    run()
-    torch.cuda.synchronize()
-    lat = []
-    graph_ms = []
-    t_start = time.perf_counter()
-    for _ in range(args.batched_steps):
-        t1 = time.perf_counter()
-        o = run()
-        lat.append(time.perf_counter() - t1)
-        graph_ms.append(seq.step_times()[0][1])
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t_start
-    hits = int(o.total_estimated_hits)
+    def timed(filtered):
+        t0 = time.perf_counter()
+        run(filtered)  # capture + first replay
+        capture_s = time.perf_counter() - t0
+        run(filtered)
+        torch.cuda.synchronize()
+        lat, graph_ms = [], []
+        t_start = time.perf_counter()
+        for _ in range(args.batched_steps):
+            t1 = time.perf_counter()
+            o = run(filtered)
+            lat.append(time.perf_counter() - t1)
+            graph_ms.append(seq.step_times()[0][1])
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t_start
+        return o, {"value": round(args.batched_steps * n / elapsed, 1), "unit": "events/s",
+                   "ms_per_step": round(1e3 * elapsed / args.batched_steps, 3),
+                   "graph_ms_p50": round(float(np.percentile(graph_ms, 50)), 3),
+                   "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 3),
+                                  "p99": round(1e3 * float(np.percentile(lat, 99)), 3)},
+                   "capture_and_first_run_s": round(capture_s, 3)}
+
+    o, plain = timed(False)
+    out = {**plain, "events_per_launch": n, "steps": args.batched_steps,
+           "workload": f"uneven nu~U(2,11), {n} events (seed {args.seed + 1}), sequence captured in one hipGraph, "
+                       f"persistent steps in largest-first order",
+           "tuning": tuning, "reserve_mb": need_mb, "tracks": int(o.total_tracks),
+           "raw_bytes": int(ev.events.nbytes), "estimated_clusters": int(o.total_estimated_hits)}
+    run(False, copy=True)
+    unfiltered = checker.check(ev, seq.host_tracks())
+    out["efficiency"] = {"category": checker.HEADLINE, "value": round(unfiltered.efficiency(), 5),
+                         "ghost_rate": round(unfiltered.ghost_rate, 5)}
+    if model is not None:
+        _, filt = timed(True)
+        run(True, copy=True)
+        host = seq.host_tracks()
+        f = checker.check(ev, host, accepted=host.accepted)
+        filt["efficiency"] = {"category": checker.HEADLINE, "value": round(f.efficiency(), 5),
+                              "ghost_rate": round(f.ghost_rate, 5), "tracks_kept": int(host.accepted.sum()),
+                              "efficiency_loss": round(unfiltered.efficiency() - f.efficiency(), 5)}
+        filt["model"] = str(model_path.relative_to(ROOT)) if model_path.is_relative_to(ROOT) else str(model_path)
+        filt["threshold"] = model.threshold
+        out["filtered"] = filt
     seq.close()
-    return {"value": round(args.batched_steps * n / elapsed, 1), "unit": "events/s",
-            "events_per_launch": n, "steps": args.batched_steps,
-            "ms_per_step": round(1e3 * elapsed / args.batched_steps, 3),
-            "graph_ms_p50": round(float(np.percentile(graph_ms, 50)), 3),
-            "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 3),
-                           "p99": round(1e3 * float(np.percentile(lat, 99)), 3)},
-            "capture_and_first_run_s": round(capture_s, 3),
-            "workload": f"uneven nu~U(2,11), {n} events (seed {args.seed + 1}), sequence captured in one hipGraph, "
-                        f"persistent steps in largest-first order",
-            "tuning": tuning, "reserve_mb": need_mb, "tracks": int(o.total_tracks),
-            "raw_bytes": int(ev.events.nbytes), "estimated_clusters": hits}
+    return out
 
 
 def main():
@@ -194,6 +221,8 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=5)
     ap.add_argument("--batched-tuning", default=None,
                     help="launch shapes of the batched run (cuda_hlt_amd.autotune JSON)")
+    ap.add_argument("--ghost-model", default=None,
+                    help="ghost filter for the batched run (.npz from python -m cuda_hlt_amd.ghost)")
     ap.add_argument("--traffic", default=str(ROOT / "profiles" / "traffic_minbias_10k.json"),
                     help="per-kernel HBM traffic measured by rocprofv3 --pmc (cuda_hlt_amd.profiling)")
     args = ap.parse_args()

@@ -74,25 +74,32 @@ def _outputs(seq, n):
 
 
 def tune(seq, n_events, dev_events, dev_offsets, repeats=3, log=print):
-    """Returns {algorithm: (block_dim, lds_tile)} and the per-config times."""
+    """Per step: workgroup size x LDS tile (SPACE), then the persistent grid
+    at the best shape (BATCHED_GRIDS, 0 = one workgroup per event).
+    Returns {algorithm: (block_dim, lds_tile, grid_dim)} and the times."""
     seq.enable_step_timing(True)
     seq.run(n_events, dev_events=dev_events, dev_offsets=dev_offsets, copy_outputs=False)
     reference = _outputs(seq, n_events)
     best, table = {}, {}
+
+    def time_cfg(alg, cfg):
+        ms = []
+        for _ in range(repeats):
+            seq.run(n_events, dev_events=dev_events, dev_offsets=dev_offsets, copy_outputs=False, tuning={alg: cfg})
+            ms.append(dict(seq.step_times())[STEP_OF[alg]])
+        return float(np.median(ms))
+
     for alg, space in SPACE.items():
-        times = {}
-        for cfg in space:
-            ms = []
-            for _ in range(repeats):
-                seq.run(n_events, dev_events=dev_events, dev_offsets=dev_offsets, copy_outputs=False,
-                        tuning={alg: cfg})
-                ms.append(dict(seq.step_times())[STEP_OF[alg]])
-            times[cfg] = float(np.median(ms))
+        times = {(b, t, 0): time_cfg(alg, (b, t, 0)) for b, t in space}
+        b, t, _ = min(times, key=times.get)
+        for g in BATCHED_GRIDS[1:]:
+            times[(b, t, g)] = time_cfg(alg, (b, t, g))
         cfg = min(times, key=times.get)
         best[alg] = cfg
-        table[alg] = {f"{b}x{t}": round(v, 4) for (b, t), v in times.items()}
-        log(f"[autotune] {alg}: best block={cfg[0]} tile={cfg[1]} {times[cfg]:.3f} ms "
-            f"(default {times.get(space[0], float('nan')):.3f})")
+        table[alg] = {f"{b}x{t}x{g}": round(v, 4) for (b, t, g), v in times.items()}
+        first = (*space[0], 0)
+        log(f"[autotune] {alg}: best block={cfg[0]} tile={cfg[1]} grid={cfg[2]} {times[cfg]:.3f} ms "
+            f"(first shape {times.get(first, float('nan')):.3f})")
     seq.run(n_events, dev_events=dev_events, dev_offsets=dev_offsets, copy_outputs=False, tuning=best)
     if _outputs(seq, n_events) != reference:
         raise RuntimeError("tuned launch shapes changed the output")

@@ -103,6 +103,31 @@ __device__ __forceinline__ float ghost_logit(const float* w, const float* x, flo
   return z;
 }
 
+// The same logit without keeping the second hidden layer: each unit's
+// ReLU output is added to the logit as soon as it is formed (identical
+// operations in identical order, so identical results), which keeps the
+// inference paths free of per-lane arrays indexed at run time.
+__device__ __forceinline__ float ghost_logit_infer(const float* w, const float* x)
+{
+  float h1[VELO_GHOST_HIDDEN];
+#pragma unroll
+  for (uint32_t j = 0; j < VELO_GHOST_HIDDEN; ++j) {
+    float s = w[kGhostB1 + j];
+#pragma unroll
+    for (uint32_t i = 0; i < VELO_GHOST_FEATURES; ++i) s += w[kGhostW1 + j * VELO_GHOST_FEATURES + i] * x[i];
+    h1[j] = s > 0.0f ? s : 0.0f;
+  }
+  float z = w[kGhostB3];
+  for (uint32_t k = 0; k < VELO_GHOST_HIDDEN; ++k) {
+    float s = w[kGhostB2 + k];
+#pragma unroll
+    for (uint32_t j = 0; j < VELO_GHOST_HIDDEN; ++j) s += w[kGhostW2 + k * VELO_GHOST_HIDDEN + j] * h1[j];
+    const float h2 = s > 0.0f ? s : 0.0f;
+    z += w[kGhostW3 + k] * h2;
+  }
+  return z;
+}
+
 __device__ inline float ghost_sigmoid(float z)
 {
   if (z >= 0.0f) return 1.0f / (1.0f + expf(-z));

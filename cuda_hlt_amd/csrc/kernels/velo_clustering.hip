@@ -275,7 +275,8 @@ __global__ void init_event_list_kernel(uint32_t* event_list, uint32_t n)
 // it fits (tuning.lds_tile words); otherwise the same code reads it from
 // global memory.  Each wave owns whole modules; the SPs of a module's four
 // banks are taken as one list in bank order, one SP per lane.
-constexpr uint32_t kDefaultEventWords = 6144; // 24 KB of LDS per workgroup
+constexpr uint32_t kDefaultEventWords = 6144;   // estimate: 24 KB of LDS per workgroup
+constexpr uint32_t kDefaultClusterWords = 8192; // clustering: 32 KB
 
 __device__ inline uint32_t sel4(uint32_t k, uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3)
 {
@@ -792,7 +793,7 @@ int velo_masked_clustering_launch(const velo_masked_clustering_args* in, void* s
   velo_masked_clustering_args a = *in;
   const uint32_t block = block_of(a.tuning);
   if (!block) return (int) hipErrorInvalidValue;
-  if (!a.tuning.lds_tile) a.tuning.lds_tile = kDefaultEventWords;
+  if (!a.tuning.lds_tile) a.tuning.lds_tile = kDefaultClusterWords;
   const size_t shmem = (size_t) a.tuning.lds_tile * sizeof(uint32_t);
   hipLaunchKernelGGL(masked_clustering_kernel, dim3(event_grid(a.tuning, a.n_events)), dim3(block), shmem, (hipStream_t) stream, a);
   return (int) hipGetLastError();

@@ -72,8 +72,7 @@ __global__ void __launch_bounds__(256) ghost_infer_kernel(velo_ghost_infer_args 
   float x[VELO_GHOST_FEATURES];
   for (uint32_t k = 0; k < VELO_GHOST_FEATURES; ++k)
     x[k] = (a.dev_features[(size_t) s * VELO_GHOST_FEATURES + k] - a.dev_model->mean[k]) * a.dev_model->scale[k];
-  float h1[VELO_GHOST_HIDDEN], h2[VELO_GHOST_HIDDEN];
-  a.dev_score[s] = ghost_sigmoid(ghost_logit(w, x, h1, h2));
+  a.dev_score[s] = ghost_sigmoid(ghost_logit_infer(w, x));
 }
 
 // ---- training: gradient of the weighted cross-entropy ----------------------------

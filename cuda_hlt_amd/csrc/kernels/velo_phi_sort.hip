@@ -2,7 +2,7 @@
 // (FillCandidates.cu:8-97) for gfx950.
 //
 // One workgroup per event, wave w owns modules w, w+W, ...  Phi sort: when
-// the event fits (tuning.lds_tile hits; off by default) the workgroup first
+// the event fits (tuning.lds_tile hits, default 1024) the workgroup first
 // copies its x, y, z, id slots to LDS in one coalesced pass; the wave then
 // computes the module's phis into its LDS slice, ranks every hit against
 // all others (stable: ties broken by slot, CalculatePhi.cu:47-53) and
@@ -19,7 +19,7 @@ constexpr uint32_t kSortCap = 512;
 constexpr uint32_t kSortChunks = kSortCap / kWave;
 constexpr uint32_t kSortBlock = 256;
 constexpr uint32_t kHoleBits = 0x7FC00000u; // quiet NaN
-constexpr uint32_t kDefaultSortStage = 0;  // hits of an event staged in LDS (0: never)
+constexpr uint32_t kDefaultSortStage = 1024;  // hits of an event staged in LDS
 constexpr size_t kMaxSortLds = 65536;
 
 __device__ inline float module_phi(uint32_t m, float x, float y) { return (m & 1u) ? phi_odd(x, y) : phi_even(x, y); }

@@ -257,8 +257,7 @@ __device__ __forceinline__ void consolidate_event(
       float f[VELO_GHOST_FEATURES];
       track_features(TrackHitsOf<V> {hv, t}, nh, x0, y0, tx, ty, zb, chi2, f);
       for (uint32_t k = 0; k < VELO_GHOST_FEATURES; ++k) f[k] = (f[k] - model->mean[k]) * model->scale[k];
-      float h1[VELO_GHOST_HIDDEN], h2[VELO_GHOST_HIDDEN];
-      const float p = ghost_sigmoid(ghost_logit(gw, f, h1, h2));
+      const float p = ghost_sigmoid(ghost_logit_infer(gw, f));
       accept = p < model->threshold ? 1 : 0;
       if (a.dev_ghost_score) a.dev_ghost_score[ti] = p;
     }

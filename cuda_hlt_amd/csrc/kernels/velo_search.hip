@@ -10,9 +10,9 @@
 //
 // State: hit_used as a bitmask in LDS (events up to 16384 estimated hits;
 // larger events use the global byte array directly); x, y, z, phi of the
-// nine modules a step can touch (f+3 .. f-5 for first module f) are copied
-// into an LDS window at the start of every step, other hits are read from
-// global memory; the tracks-to-follow ring lives in dev_tracks_to_follow
+// six modules a step searches (f .. f-5 for first module f) are copied into
+// an LDS window at the start of every step when they fit, other hits are
+// read from global memory; the tracks-to-follow ring lives in dev_tracks_to_follow
 // (2000 entries of 8 bytes per event):
 //   word0 = h[n-2] | h[n-1] << 16
 //   word1 = (tracklet ? first hit : track number) | n << 16 | skipped << 28
@@ -546,7 +546,7 @@ __device__ inline void forwarding(
   }
 }
 
-constexpr uint32_t kDefaultWindowHits = 256;
+constexpr uint32_t kDefaultWindowHits = 512;
 
 __global__ void __launch_bounds__(64) search_by_triplet_kernel(velo_search_by_triplet_args a)
 {
@@ -571,11 +571,11 @@ __global__ void __launch_bounds__(64) search_by_triplet_kernel(velo_search_by_tr
     H.cap = cap;
     H.lo = 0;
     H.n = 0;
-    // window of modules first+3 .. first-5 (contiguous in the container)
+    // window of the six modules a step searches, first .. first-5
+    // (contiguous in the container): seeding's h0, h1, h2 and forwarding's
+    // phi windows; the tracks' previous hits are read from global memory
     const auto stage_window = [&](uint32_t first) {
-      const uint32_t lo_m = first - 5;
-      const uint32_t hi_m = first + 3 < VELO_N_MODULES ? first + 3 : VELO_N_MODULES - 1;
-      H.stage(starts[lo_m] - off, starts[hi_m + 1] - off);
+      H.stage(starts[first - 5] - off, starts[first + 1] - off);
     };
     const int16_t* h0c = a.dev_h0_candidates + 2 * (size_t) off;
     const int16_t* h2c = a.dev_h2_candidates + 2 * (size_t) off;

@@ -6,6 +6,7 @@ tables kept beside the benchmark:
 
 * ``stats``    kernel time per kernel (calls, total, average, share);
 * ``counters`` the mean of each counter per dispatch, per kernel;
+* ``trace``    mean duration per kernel and launch shape (--kernel-trace);
 * ``traffic``  HBM bytes per dispatch from two ``--pmc`` runs (FETCH_SIZE,
   then WRITE_SIZE), each scaled by what the same counter reads for a copy
   kernel of known size run in the same process.
@@ -14,6 +15,7 @@ Usage::
 
     python -m cuda_hlt_amd.profiling stats --csv DIR --command "..." --title "..."
     python -m cuda_hlt_amd.profiling counters --csv DIR --command "..." --title "..."
+    python -m cuda_hlt_amd.profiling trace --csv DIR --command "..." --title "..."
     python -m cuda_hlt_amd.profiling traffic --fetch DIR --write DIR --copy-bytes N \\
         --events-per-launch 10000 --command "..." --out-json F --out-md F
 """
@@ -29,11 +31,20 @@ from pathlib import Path
 
 
 def short_name(name: str) -> str:
-    """'velo::search_by_triplet_kernel(args)' -> 'search_by_triplet_kernel'."""
+    """'velo::search_by_triplet_kernel(args)' -> 'search_by_triplet_kernel',
+    'void velo::consolidate_kernel<false>(args)' -> 'consolidate_kernel<false>'."""
     head = name.split("(")[0].strip()
+    if head.startswith("void "):
+        head = head[5:]
+    if head.startswith("velo::"):
+        return head[len("velo::"):]
     if "<" in head:  # templated library kernels keep a readable prefix
         return re.sub(r"\s+", " ", head)[:120]
     return head.split("::")[-1]
+
+
+def _is_velo_kernel(name: str) -> bool:
+    return "_kernel" in name and not name.startswith(("at::", "__amd"))
 
 
 def _find(path: str | Path, suffix: str) -> Path:
@@ -73,6 +84,22 @@ def read_stats(path: str | Path) -> list[dict]:
     return out
 
 
+def read_trace(path: str | Path) -> list[dict]:
+    """Per (kernel, workgroups, workgroup size): dispatches and mean duration
+    from a --kernel-trace CSV, largest total first."""
+    f = _find(path, "kernel_trace.csv")
+    groups: dict[tuple, list[float]] = defaultdict(list)
+    with open(f, newline="") as fh:
+        for row in csv.DictReader(fh):
+            wg = int(row.get("Workgroup_Size_X") or row.get("Workgroup_Size") or 1)
+            grid = int(row.get("Grid_Size_X") or row.get("Grid_Size") or 0)
+            dur = (int(row["End_Timestamp"]) - int(row["Start_Timestamp"])) / 1e3
+            groups[(short_name(row["Kernel_Name"]), grid // max(wg, 1), wg)].append(dur)
+    out = [{"kernel": k, "workgroups": g, "workgroup_size": w, "dispatches": len(v), "average_us": sum(v) / len(v),
+            "total_us": sum(v)} for (k, g, w), v in groups.items()]
+    return sorted(out, key=lambda r: -r["total_us"])
+
+
 def _header(title: str, command: str) -> list[str]:
     return [f"# {title}", "", "Command (run from the repository root):", "", "```", command, "```", ""]
 
@@ -82,6 +109,15 @@ def stats_markdown(path, title, command) -> str:
     for r in read_stats(path):
         lines.append(f"| `{r['kernel']}` | {r['calls']} | {r['total_us']:.1f} | {r['average_us']:.1f} | "
                      f"{r['percent']:.2f} |")
+    return "\n".join(lines) + "\n"
+
+
+def trace_markdown(path, title, command) -> str:
+    lines = _header(title, command) + ["| kernel | workgroups | workgroup size | dispatches | average us | total us |",
+                                       "|---|---|---|---|---|---|"]
+    for r in read_trace(path):
+        lines.append(f"| `{r['kernel']}` | {r['workgroups']} | {r['workgroup_size']} | {r['dispatches']} | "
+                     f"{r['average_us']:.1f} | {r['total_us']:.1f} |")
     return "\n".join(lines) + "\n"
 
 
@@ -104,7 +140,7 @@ def traffic(fetch_dir, write_dir, copy_bytes: int, copy_match: str = "add") -> d
 
     def calibration(data, counter):
         cands = [(v[counter], k) for k, v in data.items() if counter in v and copy_match in k.lower()
-                 and not k.endswith("_kernel")]
+                 and not _is_velo_kernel(k)]
         if not cands:
             raise ValueError(f"no copy kernel matching '{copy_match}' with {counter}")
         kib, name = max(cands)
@@ -114,7 +150,7 @@ def traffic(fetch_dir, write_dir, copy_bytes: int, copy_match: str = "add") -> d
     wf, wname, wkib = calibration(write, "WRITE_SIZE")
     kernels = {}
     for k in sorted(set(fetch) | set(write)):
-        if not k.endswith("_kernel"):
+        if not _is_velo_kernel(k):
             continue
         fb = fetch.get(k, {}).get("FETCH_SIZE")
         wb = write.get(k, {}).get("WRITE_SIZE")
@@ -143,7 +179,7 @@ def traffic_markdown(t: dict, title: str, command: str) -> str:
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     sub = ap.add_subparsers(dest="cmd", required=True)
-    for name in ("stats", "counters"):
+    for name in ("stats", "counters", "trace"):
         p = sub.add_parser(name)
         p.add_argument("--csv", required=True)
         p.add_argument("--command", required=True)
@@ -163,6 +199,8 @@ def main(argv=None):
         sys.stdout.write(stats_markdown(a.csv, a.title, a.command))
     elif a.cmd == "counters":
         sys.stdout.write(counters_markdown(a.csv, a.title, a.command))
+    elif a.cmd == "trace":
+        sys.stdout.write(trace_markdown(a.csv, a.title, a.command))
     else:
         t = traffic(a.fetch, a.write, a.copy_bytes, a.copy_match)
         t["events_per_launch"] = a.events_per_launch
