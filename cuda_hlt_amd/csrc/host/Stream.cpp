@@ -35,6 +35,7 @@ bool Stream::GraphKey::operator==(const GraphKey& o) const
 {
   return dev_events == o.dev_events && dev_offsets == o.dev_offsets && dev_work_order == o.dev_work_order &&
          events_size == o.events_size && n_events == o.n_events && dev_ghost_model == o.dev_ghost_model &&
+         run_velo_kalman_fit == o.run_velo_kalman_fit &&
          std::memcmp(&tuning, &o.tuning, sizeof(tuning)) == 0;
 }
 
@@ -114,6 +115,7 @@ hipError_t Stream::run_batched(const RuntimeOptions& ro)
   key.dev_offsets = ro.dev_velopix_event_offsets;
   key.dev_work_order = ro.dev_work_order;
   key.dev_ghost_model = ro.dev_ghost_model;
+  key.run_velo_kalman_fit = ro.run_velo_kalman_fit;
   key.events_size = ro.host_velopix_events_size;
   key.n_events = ro.number_of_events;
   key.tuning = ro.tuning;
@@ -157,6 +159,10 @@ hipError_t Stream::run_batched(const RuntimeOptions& ro)
     d2h(host_buffers.host_velo_track_hits, output<dev_velo_track_hits>(), h * 16);
     d2h(host_buffers.host_velo_states, output<dev_velo_states>(), t * VELO_STATE_BYTES);
     d2h(host_buffers.host_accepted_velo_tracks, output<dev_accepted_velo_tracks>(), t);
+    if (ro.run_velo_kalman_fit) {
+      host_buffers.ensure_kalman(t);
+      d2h(host_buffers.host_kalmanvelo_states, output<dev_velo_kalman_beamline_states>(), t * VELO_STATE_BYTES);
+    }
     wait();
   }
   return hipSuccess;

@@ -484,3 +484,41 @@ void SequenceVisitor::visit<consolidate_velo_tracks_t>(
   hipCheck(hipMemcpyAsync(host_buffers.host_accepted_velo_tracks, arguments.offset<dev_accepted_velo_tracks>(), t,
                           hipMemcpyDeviceToHost, stream));
 }
+
+// ---- velo_kalman_fit_t (KalmanFilterVisitor.cu:4-44) -------------------------------
+template<>
+void SequenceVisitor::set_arguments_size<velo_kalman_fit_t>(
+  velo_kalman_fit_t::arguments_t arguments,
+  const RuntimeOptions& runtime_options,
+  const Constants&,
+  const HostBuffers& host_buffers)
+{
+  const size_t t = runtime_options.run_velo_kalman_fit ? host_buffers.host_number_of_reconstructed_velo_tracks[0] : 0;
+  arguments.set_size<dev_velo_kalman_beamline_states>(t * VELO_STATE_BYTES);
+}
+
+template<>
+void SequenceVisitor::visit<velo_kalman_fit_t>(
+  velo_kalman_fit_t& state,
+  const velo_kalman_fit_t::arguments_t& arguments,
+  const RuntimeOptions& runtime_options,
+  const Constants&,
+  HostBuffers& host_buffers,
+  hipStream_t& stream,
+  hipEvent_t&)
+{
+  if (!runtime_options.run_velo_kalman_fit) return;
+  state.set_opts(dim3(host_buffers.host_number_of_selected_events[0]), dim3(256), stream);
+  state.set_arguments(
+    arguments.offset<dev_atomics_velo>(),
+    arguments.offset<dev_velo_track_hit_number>(),
+    arguments.offset<dev_velo_track_hits>(),
+    arguments.offset<dev_velo_states>(),
+    arguments.offset<dev_velo_kalman_beamline_states>());
+  state.invoke();
+  if (!runtime_options.copy_outputs_to_host || runtime_options.batched) return;
+  const size_t t = host_buffers.host_number_of_reconstructed_velo_tracks[0];
+  host_buffers.ensure_kalman(t);
+  hipCheck(hipMemcpyAsync(host_buffers.host_kalmanvelo_states, arguments.offset<dev_velo_kalman_beamline_states>(),
+                          t * VELO_STATE_BYTES, hipMemcpyDeviceToHost, stream));
+}

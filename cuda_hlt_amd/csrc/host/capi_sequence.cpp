@@ -81,6 +81,7 @@ int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, char* err
   ro.batched = in->batched != 0;
   ro.dev_work_order = in->dev_work_order;
   ro.dev_ghost_model = in->dev_ghost_model;
+  ro.run_velo_kalman_fit = in->run_velo_kalman_fit != 0;
   try {
     s->stream->run_sequence(ro);
   }
@@ -108,12 +109,14 @@ int velo_sequence_output_get(const velo_sequence* s, velo_sequence_output* out)
     out->host_track_hits = hb.host_velo_track_hits;
     out->host_states = hb.host_velo_states;
     out->host_accepted = hb.host_accepted_velo_tracks;
+    if (s->last.run_velo_kalman_fit) out->host_kalman_states = hb.host_kalmanvelo_states;
   }
   out->dev_atomics_velo = st.output<dev_atomics_velo>();
   out->dev_track_hit_number = st.output<dev_velo_track_hit_number>();
   out->dev_track_hits = st.output<dev_velo_track_hits>();
   out->dev_states = st.output<dev_velo_states>();
   out->dev_accepted = st.output<dev_accepted_velo_tracks>();
+  out->dev_kalman_states = st.output<dev_velo_kalman_beamline_states>();
   out->dev_event_status = st.output<dev_velo_event_status>();
   out->dev_module_cluster_start = st.output<dev_estimated_input_size>();
   out->dev_module_cluster_num = st.output<dev_module_cluster_num>();

@@ -35,5 +35,6 @@ ARGUMENT(dev_velo_track_hit_number, uint)
 ARGUMENT(dev_prefix_sum_auxiliary_array_2, uint)
 ARGUMENT(dev_velo_track_hits, char)
 ARGUMENT(dev_velo_states, char)
+ARGUMENT(dev_velo_kalman_beamline_states, char)
 // Per-event capacity-overflow bits (no reference counterpart).
 ARGUMENT(dev_velo_event_status, uint)

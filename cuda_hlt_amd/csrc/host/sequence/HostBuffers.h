@@ -22,7 +22,8 @@ struct HostBuffers {
   char* host_velo_track_hits = nullptr;
   char* host_velo_states = nullptr;
   uint8_t* host_accepted_velo_tracks = nullptr;
-  size_t cap_track_hit_number = 0, cap_track_hits = 0, cap_states = 0, cap_accepted = 0;
+  char* host_kalmanvelo_states = nullptr;
+  size_t cap_track_hit_number = 0, cap_track_hits = 0, cap_states = 0, cap_accepted = 0, cap_kalman = 0;
 
   void reserve(unsigned max_number_of_events)
   {
@@ -56,13 +57,15 @@ struct HostBuffers {
     grow(reinterpret_cast<char**>(&host_accepted_velo_tracks), &cap_accepted, tracks + 64);
   }
 
+  void ensure_kalman(size_t tracks) { grow(&host_kalmanvelo_states, &cap_kalman, tracks * VELO_STATE_BYTES + 64); }
+
   size_t velo_track_hit_number_size() const { return host_number_of_reconstructed_velo_tracks[0] + 1; }
 
   void release()
   {
     for (void* p : {(void*) host_number_of_selected_events, (void*) host_event_list, (void*) host_atomics_velo,
                     (void*) host_velo_event_status, (void*) host_velo_track_hit_number, (void*) host_velo_track_hits,
-                    (void*) host_velo_states, (void*) host_accepted_velo_tracks})
+                    (void*) host_velo_states, (void*) host_accepted_velo_tracks, (void*) host_kalmanvelo_states})
       if (p) (void) hipHostFree(p);
     host_number_of_selected_events = nullptr;
     host_event_list = nullptr;
@@ -72,6 +75,7 @@ struct HostBuffers {
     host_velo_track_hits = nullptr;
     host_velo_states = nullptr;
     host_accepted_velo_tracks = nullptr;
-    cap_track_hit_number = cap_track_hits = cap_states = cap_accepted = 0;
+    host_kalmanvelo_states = nullptr;
+    cap_track_hit_number = cap_track_hits = cap_states = cap_accepted = cap_kalman = 0;
   }
 };

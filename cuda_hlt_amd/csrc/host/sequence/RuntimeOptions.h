@@ -40,6 +40,8 @@ struct RuntimeOptions {
   // Persistent launches (tuning.grid_dim > 0): processing order of the
   // events (largest first balances uneven events), NULL = identity.
   const unsigned* dev_work_order = nullptr;
+  // Run velo_kalman_fit after consolidation (KalmanFilterVisitor.cu:4-44).
+  bool run_velo_kalman_fit = false;
   // Ghost filter run inline by consolidation (a device velo_ghost_model),
   // NULL = every track accepted.
   const void* dev_ghost_model = nullptr;

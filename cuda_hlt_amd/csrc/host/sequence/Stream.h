@@ -36,6 +36,7 @@ struct Stream {
     const unsigned* dev_offsets = nullptr;
     const unsigned* dev_work_order = nullptr;
     const void* dev_ghost_model = nullptr;
+    bool run_velo_kalman_fit = false;
     size_t events_size = 0;
     unsigned n_events = 0;
     VeloTuningOptions tuning {};

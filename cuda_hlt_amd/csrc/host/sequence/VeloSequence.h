@@ -121,6 +121,19 @@ ALGORITHM(
     dev_velo_states,
     dev_accepted_velo_tracks))
 
+// VeloKalmanFilter.cuh:94-102; run when RuntimeOptions::run_velo_kalman_fit
+// is set (the reference's Velo sequence ends with consolidation, its
+// default sequence continues with this fit)
+ALGORITHM(
+  velo_kalman_fit_launch,
+  velo_kalman_fit_t,
+  ARGUMENTS(
+    dev_atomics_velo,
+    dev_velo_track_hit_number,
+    dev_velo_track_hits,
+    dev_velo_states,
+    dev_velo_kalman_beamline_states))
+
 #define SEQUENCE_T(...) typedef std::tuple<__VA_ARGS__> configured_sequence_t;
 
 SEQUENCE_T(
@@ -135,7 +148,8 @@ SEQUENCE_T(
   copy_and_prefix_sum_single_block_velo_t,
   copy_velo_track_hit_number_t,
   prefix_sum_velo_track_hit_number_t,
-  consolidate_velo_tracks_t)
+  consolidate_velo_tracks_t,
+  velo_kalman_fit_t)
 
 // Kept allocated from first use to the end of the sequence
 // (OutputArguments.cuh:16-34, VELO part).
@@ -145,6 +159,7 @@ typedef std::tuple<
   dev_velo_track_hits,
   dev_velo_states,
   dev_accepted_velo_tracks,
+  dev_velo_kalman_beamline_states,
   dev_event_list,
   dev_number_of_selected_events,
   dev_velo_event_status>

@@ -97,6 +97,10 @@ ConsolidateArgs = _args("ConsolidateArgs", [
     "dev_module_cluster_start", "dev_module_cluster_num", "dev_velo_track_hits", "dev_velo_states",
     "dev_accepted_velo_tracks", "dev_ghost_model", "dev_ghost_score"])
 
+KalmanFitArgs = _args("KalmanFitArgs", [
+    "dev_atomics_velo", "dev_velo_track_hit_number", "dev_velo_track_hits", "dev_velo_states",
+    "dev_velo_kalman_beamline_states"])
+
 LAUNCHERS = {
     "velo_init_event_list_launch": InitEventListArgs,
     "velo_estimate_input_size_launch": EstimateInputSizeArgs,
@@ -109,6 +113,7 @@ LAUNCHERS = {
     "velo_copy_and_prefix_sum_single_block_launch": CopyAndPrefixSumArgs,
     "velo_copy_velo_track_hit_number_launch": CopyTrackHitNumberArgs,
     "velo_consolidate_velo_tracks_launch": ConsolidateArgs,
+    "velo_kalman_fit_launch": KalmanFitArgs,
 }
 
 _lib = None

@@ -20,15 +20,16 @@ class SequenceInput(C.Structure):
     _fields_ = [("host_events", P), ("host_offsets", P), ("events_size", C.c_size_t), ("dev_events", P),
                 ("dev_offsets", P), ("n_events", C.c_uint32), ("repetitions", C.c_uint32),
                 ("copy_outputs_to_host", C.c_int), ("tuning", nat.Tuning * 6), ("batched", C.c_int),
-                ("dev_work_order", P), ("dev_ghost_model", P)]
+                ("dev_work_order", P), ("dev_ghost_model", P), ("run_velo_kalman_fit", C.c_int)]
 
 
 class SequenceOutput(C.Structure):
     _fields_ = [("n_events", C.c_uint32), ("total_estimated_hits", C.c_uint32), ("total_tracks", C.c_uint32),
                 ("total_track_hits", C.c_uint32), ("host_atomics_velo", P), ("host_track_hit_number", P),
-                ("host_track_hits", P), ("host_states", P), ("host_accepted", P), ("dev_atomics_velo", P),
-                ("dev_track_hit_number", P),
-                ("dev_track_hits", P), ("dev_states", P), ("dev_accepted", P), ("dev_event_status", P),
+                ("host_track_hits", P), ("host_states", P), ("host_accepted", P), ("host_kalman_states", P),
+                ("dev_atomics_velo", P), ("dev_track_hit_number", P),
+                ("dev_track_hits", P), ("dev_states", P), ("dev_accepted", P), ("dev_kalman_states", P),
+                ("dev_event_status", P),
                 ("dev_module_cluster_start", P), ("dev_module_cluster_num", P), ("dev_cluster_container", P),
                 ("memory_required", C.c_size_t)]
 
@@ -69,6 +70,7 @@ class HostTracks:
     track_hits: np.ndarray    # uint8[16 Ht]
     states: np.ndarray        # uint8[49 T]
     accepted: np.ndarray      # uint8[T]: dev_accepted_velo_tracks
+    kalman: np.ndarray | None = None  # uint8[49 T]: velo_kalman_fit states, when run
 
 
 class VeloSequence:
@@ -101,7 +103,7 @@ class VeloSequence:
 
     def run(self, n_events, host_events=None, host_offsets=None, dev_events=None, dev_offsets=None,
             copy_outputs=True, repetitions=1, tuning=None, events_size=0, batched=False,
-            dev_work_order=None, dev_ghost_model=None) -> SequenceOutput:
+            dev_work_order=None, dev_ghost_model=None, velo_kalman_fit=False) -> SequenceOutput:
         """Run the sequence on host input (copied by init_event_list) or on
         device-resident input (dev_events/dev_offsets; events_size = their
         bytes, needed by batched runs to bound the buffers).  batched=True
@@ -128,6 +130,7 @@ class VeloSequence:
         inp.batched = int(batched)
         inp.dev_work_order = None if dev_work_order is None else int(dev_work_order)
         inp.dev_ghost_model = None if dev_ghost_model is None else int(dev_ghost_model)
+        inp.run_velo_kalman_fit = int(velo_kalman_fit)
         err = C.create_string_buffer(1024)
         rc = _lib().velo_sequence_run(self.h, C.byref(inp), err, 1024)
         if rc != 0:
@@ -161,4 +164,5 @@ class VeloSequence:
         return HostTracks(n, arr(o.host_atomics_velo, 2 * n + 1, np.uint32),
                           arr(o.host_track_hit_number, T + 1, np.uint32),
                           arr(o.host_track_hits, 16 * Ht, np.uint8), arr(o.host_states, 49 * T, np.uint8),
-                          arr(o.host_accepted, T, np.uint8))
+                          arr(o.host_accepted, T, np.uint8),
+                          arr(o.host_kalman_states, 49 * T, np.uint8) if o.host_kalman_states else None)

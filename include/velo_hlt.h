@@ -256,6 +256,24 @@ typedef struct {
 } velo_copy_velo_track_hit_number_args;
 int velo_copy_velo_track_hit_number_launch(const velo_copy_velo_track_hit_number_args* a, void* stream);
 
+/* ---- velo_kalman_fit (VeloKalmanFilter.cu:43-73, VeloKalmanFilter.cuh:22-91)
+ * Simplified Kalman fit of every consolidated track from the hit farthest
+ * from the beamline towards it (scattering noise 1e-8 + 7e-6 (tx^2 + ty^2)
+ * per layer); the states go to dev_velo_kalman_beamline_states in the
+ * consolidated state layout (12 float arrays of total_tracks, then one
+ * byte per track).  The reference leaves the backward byte of these states
+ * uninitialised; here it is the input state's. */
+typedef struct {
+  const int32_t* dev_atomics_velo;
+  const uint32_t* dev_velo_track_hit_number;
+  const char* dev_velo_track_hits;
+  const char* dev_velo_states;
+  char* dev_velo_kalman_beamline_states;
+  uint32_t n_events;
+  velo_tuning tuning;
+} velo_kalman_fit_args;
+int velo_kalman_fit_launch(const velo_kalman_fit_args* a, void* stream);
+
 /* ---- K15: consolidate_velo_tracks (ConsolidateVelo.cu:116-170) ----
  * Also sets dev_accepted_velo_tracks[0, total_tracks) to 1 (the memset of
  * ConsolidateVeloVisitor.cu:42-43) when non-NULL.  With dev_ghost_model

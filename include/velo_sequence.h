@@ -51,6 +51,8 @@ typedef struct {
   /* Ghost filter run inline by consolidation (device velo_ghost_model,
    * velo_ghost.h), NULL = every track accepted.  No reference counterpart. */
   const void* dev_ghost_model;
+  /* Also run velo_kalman_fit after consolidation (KalmanFilterVisitor.cu:4-44). */
+  int run_velo_kalman_fit;
 } velo_sequence_input;
 
 typedef struct {
@@ -64,12 +66,14 @@ typedef struct {
   const char* host_track_hits;             /* 16*Ht: SoA x, y, z, LHCbID */
   const char* host_states;                 /* 49*T: SoA 12 floats + bool */
   const uint8_t* host_accepted;            /* T: dev_accepted_velo_tracks */
+  const char* host_kalman_states;          /* 49*T when velo_kalman_fit ran */
   /* device outputs (valid until the next run) */
   const int32_t* dev_atomics_velo;
   const uint32_t* dev_track_hit_number;
   const char* dev_track_hits;
   const char* dev_states;
   const uint8_t* dev_accepted;
+  const char* dev_kalman_states;
   const uint32_t* dev_event_status;
   const uint32_t* dev_module_cluster_start;
   const uint32_t* dev_module_cluster_num;

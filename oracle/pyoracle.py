@@ -62,6 +62,11 @@ def lib():
         L.oracle_copy_velo_track_hit_number.argtypes = [C.c_uint32, _P, _P, _P]
         L.oracle_consolidate_velo_tracks.argtypes = [C.c_uint32] + [_P] * 7
         L.oracle_ccl_event.argtypes = [_P, _P, _P, _P, _P, _P, C.c_uint32]
+        L.oracle_velo_kalman_fit.argtypes = [C.c_uint32] + [_P] * 5
+        _F = C.c_float
+        _PF = C.POINTER(C.c_float)
+        L.oracle_kalman_step.argtypes = [_F, _F, _F, _F, _PF, _PF, _PF, _PF, _PF]
+        L.oracle_kalman_step.restype = C.c_float
         L.oracle_ccl_event.restype = C.c_uint32
         _lib = L
     return _lib
@@ -186,6 +191,24 @@ def consolidate(n, atomics, tracks, hit_number, container, starts):
     lib().oracle_consolidate_velo_tracks(n, _ptr(atomics), _ptr(tracks), _ptr(hit_number), _ptr(container),
                                          _ptr(starts), _ptr(hits), _ptr(states))
     return hits, states
+
+
+def velo_kalman_fit(n, atomics, hit_number, track_hits, states):
+    """Kalman-fitted beamline states of the consolidated tracks (49 B each, SoA)."""
+    T = int(atomics[2 * n])
+    out = np.zeros(49 * T + 49, np.uint8)
+    lib().oracle_velo_kalman_fit(n, _ptr(np.ascontiguousarray(atomics, np.uint32)),
+                                 _ptr(np.ascontiguousarray(hit_number, np.uint32)),
+                                 _ptr(np.ascontiguousarray(track_hits, np.uint8)),
+                                 _ptr(np.ascontiguousarray(states, np.uint8)), _ptr(out))
+    return out
+
+
+def kalman_step(z, zhit, xhit, whit, x, tx, cxx, cxtx, ctxtx):
+    """One filter step; returns (chi2, x, tx, cxx, cxtx, ctxtx) as float32."""
+    v = [C.c_float(x), C.c_float(tx), C.c_float(cxx), C.c_float(cxtx), C.c_float(ctxtx)]
+    chi2 = lib().oracle_kalman_step(z, zhit, xhit, whit, *[C.byref(a) for a in v])
+    return (np.float32(chi2),) + tuple(np.float32(a.value) for a in v)
 
 
 def ccl_event(event: np.ndarray, capacity: int = 200000):

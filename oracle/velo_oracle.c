@@ -41,6 +41,7 @@ static const float k_max_scatter_seeding = 0.004f;
 static const float k_max_scatter_forwarding = 0.004f;
 static const unsigned k_max_skipped_modules = 1u;
 static const float k_param_w = 3966.94f;
+static const float k_param_w_inverted = 0.000252083f; /* VeloDefinitions.cuh:74 */
 static const float k_max_chi2 = 20.0f;
 static const float k_pixel_size = 0.055f; /* VeloDefinitions.cuh:29 */
 
@@ -1097,6 +1098,83 @@ void oracle_copy_velo_track_hit_number(
 /* ------------------------------------------------------------------------ */
 /* a16: consolidation                                                       */
 /* ------------------------------------------------------------------------ */
+
+/* VeloKalmanFilter.cu:6-41: prediction to zhit, gain, update of the
+ * (x, tx) state and its covariance, chi2 = r^2 R. */
+float oracle_kalman_step(float z, float zhit, float xhit, float whit, float* x, float* tx, float* cov_xx,
+                         float* cov_xtx, float* cov_txtx)
+{
+  const float dz = zhit - z;
+  const float predx = *x + dz * *tx;
+  const float dz_t_covtxtx = dz * *cov_txtx;
+  const float predcov_xtx = *cov_xtx + dz_t_covtxtx;
+  const float dx_t_covxtx = dz * *cov_xtx;
+  const float predcov_xx = *cov_xx + 2.0f * dx_t_covxtx + dz * dz_t_covtxtx;
+  const float predcov_txtx = *cov_txtx;
+  const float r_gain = 1.0f / ((1.0f / whit) + predcov_xx);
+  const float kx = predcov_xx * r_gain;
+  const float ktx = predcov_xtx * r_gain;
+  const float r = xhit - predx;
+  *x = predx + kx * r;
+  *tx = *tx + ktx * r;
+  *cov_xx = (1.0f - kx) * predcov_xx;
+  *cov_xtx = (1.0f - kx) * predcov_xtx;
+  *cov_txtx = predcov_txtx - ktx * predcov_xtx;
+  return r * r * r_gain;
+}
+
+/* VeloKalmanFilter.cuh:22-91 (simplified_fit<true>) for every track. */
+void oracle_velo_kalman_fit(
+  uint32_t n_events, const uint32_t* atomics, const uint32_t* hit_number, const char* track_hits,
+  const char* states, char* out)
+{
+  const uint32_t T = atomics[2 * n_events];
+  const uint32_t Ht = hit_number[T];
+  const float* hx = (const float*) track_hits;
+  const float* hy = hx + Ht;
+  const float* hz = hy + Ht;
+  const float* S[12];
+  float* O[12];
+  for (int k = 0; k < 12; ++k) {
+    S[k] = (const float*) states + (size_t) k * T;
+    O[k] = (float*) out + (size_t) k * T;
+  }
+  const uint8_t* bwd_in = (const uint8_t*) (states + 12 * sizeof(float) * (size_t) T);
+  uint8_t* bwd_out = (uint8_t*) (out + 12 * sizeof(float) * (size_t) T);
+  for (uint32_t t = 0; t < T; ++t) {
+    const uint32_t h0 = hit_number[t];
+    const int nh = (int) (hit_number[t + 1] - h0);
+    const float* x = hx + h0;
+    const float* y = hy + h0;
+    const float* z = hz + h0;
+    const float beam_tx = S[2][t], beam_ty = S[3][t], beam_z = S[11][t];
+    const int backward = beam_z > z[0];
+    const int direction = backward ? 1 : -1; /* upstream fit */
+    const float slope2 = beam_tx * beam_tx + beam_ty * beam_ty;
+    const float noise2 = (float) (1e-8 + 7e-6 * (double) slope2);
+    int first = 0, last = nh - 1, step = 1;
+    if ((z[last] - z[first]) * (float) direction < 0.0f) {
+      first = nh - 1;
+      last = 0;
+      step = -1;
+    }
+    float sx = x[first], sy = y[first], sz = z[first], stx = beam_tx, sty = beam_ty;
+    float c00 = k_param_w_inverted, c11 = k_param_w_inverted, c20 = 0.0f, c31 = 0.0f, c22 = 1.0f, c33 = 1.0f;
+    float chi2 = 0.0f;
+    for (int i = first + step; i != last + step; i += step) {
+      c22 += noise2;
+      c33 += noise2;
+      chi2 += oracle_kalman_step(sz, z[i], x[i], k_param_w, &sx, &stx, &c00, &c20, &c22);
+      chi2 += oracle_kalman_step(sz, z[i], y[i], k_param_w, &sy, &sty, &c11, &c31, &c33);
+      sz = z[i];
+    }
+    c22 += noise2;
+    c33 += noise2;
+    const float v[12] = {sx, sy, stx, sty, c00, c20, c22, c11, c31, c33, chi2, sz};
+    for (int k = 0; k < 12; ++k) O[k][t] = v[k];
+    bwd_out[t] = bwd_in[t];
+  }
+}
 
 void oracle_consolidate_velo_tracks(
   uint32_t n_events, const uint32_t* atomics, const or_track_hits* tracks, const uint32_t* hit_number,

@@ -133,6 +133,20 @@ void oracle_consolidate_velo_tracks(
   const uint32_t* velo_track_hit_number, const uint32_t* cluster_container,
   const uint32_t* module_cluster_start, char* velo_track_hits, char* velo_states);
 
+/* ---- velo_kalman_fit (VeloKalmanFilter.cu:43-73, VeloKalmanFilter.cuh:22-91):
+ *      simplified Kalman fit of every consolidated track from its first
+ *      hit towards the beamline; output in the consolidated state layout.
+ *      The reference leaves the output's backward flag uninitialised; here
+ *      it is the input state's flag. ---- */
+void oracle_velo_kalman_fit(
+  uint32_t n_events, const uint32_t* atomics_velo, const uint32_t* velo_track_hit_number,
+  const char* velo_track_hits, const char* velo_states, char* kalman_states);
+
+/* One filter step of one projection (VeloKalmanFilter.cu:6-41); returns
+ * the chi2 contribution.  Exposed for the closed-form tests. */
+float oracle_kalman_step(float z, float zhit, float xhit, float whit, float* x, float* tx, float* cov_xx,
+                         float* cov_xtx, float* cov_txtx);
+
 /* ---- textbook 8-connected component labelling (Clustering.cpp:3-280
  *      semantics): LHCbIDs of all clusters of one event, plus centroid
  *      sums so coordinates can be formed; used to pin a6 ---- */

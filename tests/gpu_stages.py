@@ -181,3 +181,15 @@ def consolidate(n, atomics, tracks, hit_number, container, starts, nums, tuning=
     nat.launch("velo_consolidate_velo_tracks_launch", a)
     sync()
     return host(hits), host(states), host(acc)
+
+
+def velo_kalman_fit(n, atomics, hit_number, track_hits, states, tuning=(0, 0)):
+    T = int(atomics[2 * n])
+    da, dh = dev(atomics.view(np.int32)), dev(hit_number.view(np.int32))
+    dhits, dst = dev(track_hits), dev(states)
+    out = torch.full((49 * T + 49,), 0x5A, dtype=torch.uint8, device="cuda")
+    a = nat.KalmanFitArgs(da.data_ptr(), dh.data_ptr(), dhits.data_ptr(), dst.data_ptr(), out.data_ptr(), n,
+                          tuning_of(tuning))
+    nat.launch("velo_kalman_fit_launch", a)
+    sync()
+    return host(out)

@@ -42,3 +42,40 @@ def test_error_strings():
     from cuda_hlt_amd import native as nat
     assert nat.lib().velo_error_string(0)
     assert b"gfx950" in nat.lib().velo_hlt_version()
+
+
+def _header_structs(header):
+    text = re.sub(r"/\*.*?\*/", "", (ROOT / "include" / header).read_text(), flags=re.S)
+    out = {}
+    for m in re.finditer(r"typedef struct \{([^{}]*)\} (\w+);", text):
+        body, name = m.groups()
+        fields = []
+        for decl in body.split(";"):
+            decl = decl.strip()
+            if not decl:
+                continue
+            decl = re.sub(r"\[[^\]]*\]", "", decl)
+            fields += [re.findall(r"(\w+)\s*$", part.strip())[0] for part in decl.split(",")]
+        out[name] = fields
+    return out
+
+
+def test_sequence_structs_match_header():
+    from cuda_hlt_amd import stream
+    s = _header_structs("velo_sequence.h")
+    for name, st in (("velo_sequence_config", stream.SequenceConfig), ("velo_sequence_input", stream.SequenceInput),
+                     ("velo_sequence_output", stream.SequenceOutput)):
+        assert [f for f, _ in st._fields_] == s[name], name
+
+
+def test_ghost_and_tuning_structs_match_header():
+    from cuda_hlt_amd import ghost
+    from cuda_hlt_amd import native as nat
+    g = _header_structs("velo_ghost.h")
+    for name, st in (("velo_ghost_model", ghost.ModelStruct), ("velo_track_features_args", ghost.TrackFeaturesArgs),
+                     ("velo_ghost_infer_args", ghost.InferArgs), ("velo_ghost_grad_args", ghost.GradArgs),
+                     ("velo_ghost_adam_args", ghost.AdamArgs)):
+        assert [f for f, _ in st._fields_] == g[name], name
+    h = _header_structs("velo_hlt.h")
+    assert [f for f, _ in nat.Tuning._fields_] == h["velo_tuning"]
+    assert [f for f, _ in nat.Constants._fields_] == h["velo_constants"]

@@ -72,3 +72,19 @@ def test_sequence_too_many_events_raises(sequence, minbias_1000):
                           + [[3 * minbias_1000.offsets[-1]]]).astype(np.uint32)
     with pytest.raises(nat.VeloError):
         sequence.run(3000, host_events=big, host_offsets=offs)
+
+
+def test_sequence_with_kalman_fit(sequence, minbias_200, geometry, module_zs):
+    """The opt-in velo_kalman_fit step: consolidated output unchanged,
+    Kalman states equal to the oracle's."""
+    ev = minbias_200
+    sequence.run(ev.n_events, host_events=ev.events, host_offsets=ev.offsets, velo_kalman_fit=True)
+    out = sequence.host_tracks()
+    ref = po.pipeline(ev.events, ev.offsets, geometry, module_zs)
+    _compare(out, ref, ev.n_events)
+    n = ev.n_events
+    T = int(ref.atomics[2 * n])
+    want = po.velo_kalman_fit(n, ref.atomics, ref.hit_number, ref.track_hits, ref.states)
+    np.testing.assert_array_equal(out.kalman[: 49 * T], want[: 49 * T])
+    sequence.run(ev.n_events, host_events=ev.events, host_offsets=ev.offsets)
+    assert sequence.host_tracks().kalman is None

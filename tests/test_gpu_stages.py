@@ -283,6 +283,25 @@ def test_consolidate(stages, chain):
     assert acc[:T].all()
 
 
+def _check_kalman(stages, c: Chain, tuning=(0, 0)):
+    n = c.n
+    T = int(c.atomics[2 * n])
+    got = stages.velo_kalman_fit(n, c.atomics, c.hn, c.hits, c.states, tuning)
+    want = po.velo_kalman_fit(n, c.atomics, c.hn, c.hits, c.states)
+    np.testing.assert_array_equal(got[: 49 * T], want[: 49 * T])
+
+
+def test_velo_kalman_fit(stages, chain):
+    _check_kalman(stages, chain)
+
+
+def test_velo_kalman_fit_uneven_persistent(stages, chain_uneven):
+    _check_kalman(stages, chain_uneven)
+    t, counters = stages.persistent(chain_uneven.n, 37, seed=4)
+    _check_kalman(stages, chain_uneven, t)
+    assert not stages.host(counters).any()
+
+
 def test_coordinates_within_tolerance(stages, consts, chain):
     """Cluster x, y, z agree with the oracle within 1e-5 mm."""
     n = chain.n
