@@ -34,7 +34,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
-HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-pthread", "-ffp-contract=off", "-Wall",
               "-D__HIP_PLATFORM_AMD__"]
 
 
@@ -76,7 +76,7 @@ def build_velo_hlt(jobs: int = 8) -> Path:
     target = LIB / "libvelo_hlt.so"
     tmp = LIB / "libvelo_hlt.so.tmp"
     _run([_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp),
-          *map(str, objs), f"-L{ROCM / 'lib'}", "-lamdhip64",
+          *map(str, objs), f"-L{ROCM / 'lib'}", "-lamdhip64", "-lz",
           f"-Wl,-rpath,{ROCM / 'lib'}"])
     os.replace(tmp, target)
     return target

@@ -1,0 +1,57 @@
+/* velo_mdf.h -- VP raw banks from MDF files into the VELO event layout.
+ *
+ * Replaces, for the VP banks, the reference's MDF input path:
+ *   MDF::read_events   mdf/src/read_mdf.cpp:28-189  (bank selection, the
+ *                      per-event blob: u32 n_banks, u32 offsets[n_banks+1]
+ *                      in bytes, then per bank u32 source id + the bank's
+ *                      data words)
+ *   MDF::read_banks    mdf/src/read_mdf.cpp:213-337 (record header,
+ *                      checksum, decompression)
+ *   LHCb::genChecksum  mdf/src/raw_helpers.cpp:186-199 (flag 1: the
+ *                      one-at-a-time hash of raw_helpers.cpp:8-21, over the
+ *                      record from byte 16 on, bytes taken as signed char)
+ *   Compression::unzip mdf/src/compression.cpp:51-130 (ROOT "ZL" blocks:
+ *                      9-byte header, zlib stream)
+ * The output is what VeloRawEvent reads (cuda/velo/common/src/ClusteringDefinitions.cu:3-11):
+ * concatenated events plus u32 byte offsets[n_events + 1].
+ */
+#ifndef VELO_MDF_H
+#define VELO_MDF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VELO_MDF_BANK_VP 63u
+
+typedef struct {
+  char* events;        /* malloc'ed; free with velo_mdf_free */
+  size_t events_size;  /* bytes */
+  uint32_t* offsets;   /* n_events + 1 byte offsets */
+  uint32_t n_events;
+  uint32_t n_records;  /* MDF records read (events skipped included) */
+} velo_mdf_events;
+
+/* Reads the VP banks of up to max_events events (0: all), after skipping
+ * the first start_event records, from the files in order.  With
+ * verify_checksum set, a record whose stored checksum (when non-zero)
+ * differs from the computed one is an error.  Records are decoded
+ * (checksum, inflate, bank walk) on n_threads host threads (0 or 1: the
+ * calling thread); the output does not depend on n_threads.  Returns 0, or
+ * -1 with a message in err (the earliest failing record's). */
+int velo_mdf_read_vp(const char* const* paths, uint32_t n_paths, uint32_t max_events, uint32_t start_event,
+                     int verify_checksum, uint32_t n_threads, velo_mdf_events* out, char* err, size_t err_len);
+
+void velo_mdf_free(velo_mdf_events* ev);
+
+/* The MDF record checksum (genChecksum flag 1) of len bytes. */
+uint32_t velo_mdf_checksum(const void* data, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

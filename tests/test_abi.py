@@ -16,7 +16,7 @@ def declared(header):
 
 @pytest.mark.parametrize("header,lib", [("velo_hlt.h", "libvelo_hlt.so"), ("velo_gen.h", "libvelo_gen.so"),
                                         ("velo_sequence.h", "libvelo_hlt.so"), ("velo_checker.h", "libvelo_hlt.so"),
-                                        ("velo_ghost.h", "libvelo_hlt.so")])
+                                        ("velo_ghost.h", "libvelo_hlt.so"), ("velo_mdf.h", "libvelo_hlt.so")])
 def test_library_exports_declared_symbols(header, lib):
     import ctypes
     names = declared(header)
