@@ -305,8 +305,14 @@ def main():
     kernels = {}
     for k, ms in per_step.items():
         b = ab.get(k, 0)
+        gbps = b / (ms * 1e-3) / 1e9 if ms > 0 else None
         kernels[k] = {"ms": round(ms, 4), "algorithmic_bytes": b,
-                      "GBps": round(b / (ms * 1e-3) / 1e9, 1) if ms > 0 else None}
+                      "GBps": round(gbps, 1) if gbps is not None else None,
+                      "frac": round(gbps / HBM_PEAK_GBS, 5) if gbps is not None else None}
+        t = measured_traffic(args.traffic, k, B)
+        if t is not None and ms > 0:
+            kernels[k]["traffic_bytes"] = t
+            kernels[k]["traffic_GBps"] = round(t / (ms * 1e-3) / 1e9, 1)
     dom_ms = per_step[dominant]
     achieved = ab[dominant] / (dom_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,

@@ -110,6 +110,7 @@ hipError_t Stream::run_batched(const RuntimeOptions& ro)
 {
   if (!ro.dev_velopix_events || !ro.dev_velopix_event_offsets)
     throw std::invalid_argument("batched run: the input must be device-resident");
+  if (ro.run_global_event_cut) throw std::invalid_argument("batched run: the global event cut runs in plain runs only");
   GraphKey key;
   key.dev_events = ro.dev_velopix_events;
   key.dev_offsets = ro.dev_velopix_event_offsets;

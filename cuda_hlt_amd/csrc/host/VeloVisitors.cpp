@@ -26,6 +26,8 @@ void tune(S& state, const velo_tuning& t, const RuntimeOptions& ro, WorkSlot slo
     if (!ro.dev_work_counters) throw std::invalid_argument("persistent launch without work counters");
     x.dev_work_counter = ro.dev_work_counters + 2 * slot;
     if (!x.dev_work_order) x.dev_work_order = ro.dev_work_order;
+    // a work order lists input events; after the cut, slots are selected events
+    if (ro.run_global_event_cut) x.dev_work_order = nullptr;
   }
   state.handler.set_tuning(x);
 }
@@ -62,6 +64,33 @@ void SequenceVisitor::set_arguments_size<init_event_list_t>(
     arguments.set_size<dev_velo_raw_input>(runtime_options.host_velopix_events_size);
     arguments.set_size<dev_velo_raw_input_offsets>(runtime_options.host_velopix_event_offsets_size);
   }
+  // UT and SciFi banks travel only when the global event cut runs
+  const bool gec = runtime_options.run_global_event_cut;
+  const size_t n1 = (size_t) runtime_options.number_of_events + 1;
+  if (gec && runtime_options.dev_ut_events) {
+    arguments.set_external<dev_ut_raw_input>(
+      const_cast<char*>(runtime_options.dev_ut_events), runtime_options.host_ut_events_size);
+    arguments.set_external<dev_ut_raw_input_offsets>(
+      const_cast<unsigned*>(runtime_options.dev_ut_event_offsets), n1 * sizeof(unsigned));
+  }
+  else {
+    arguments.clear_external<dev_ut_raw_input>();
+    arguments.clear_external<dev_ut_raw_input_offsets>();
+    arguments.set_size<dev_ut_raw_input>(gec ? runtime_options.host_ut_events_size : 0);
+    arguments.set_size<dev_ut_raw_input_offsets>(gec ? n1 : 0);
+  }
+  if (gec && runtime_options.dev_scifi_events) {
+    arguments.set_external<dev_scifi_raw_input>(
+      const_cast<char*>(runtime_options.dev_scifi_events), runtime_options.host_scifi_events_size);
+    arguments.set_external<dev_scifi_raw_input_offsets>(
+      const_cast<unsigned*>(runtime_options.dev_scifi_event_offsets), n1 * sizeof(unsigned));
+  }
+  else {
+    arguments.clear_external<dev_scifi_raw_input>();
+    arguments.clear_external<dev_scifi_raw_input_offsets>();
+    arguments.set_size<dev_scifi_raw_input>(gec ? runtime_options.host_scifi_events_size : 0);
+    arguments.set_size<dev_scifi_raw_input_offsets>(gec ? n1 : 0);
+  }
   arguments.set_size<dev_event_list>(runtime_options.number_of_events);
   arguments.set_size<dev_number_of_selected_events>(1);
 }
@@ -84,9 +113,64 @@ void SequenceVisitor::visit<init_event_list_t>(
       arguments.offset<dev_velo_raw_input_offsets>(), runtime_options.host_velopix_event_offsets,
       arguments.size<dev_velo_raw_input_offsets>(), hipMemcpyHostToDevice, stream));
   }
+  if (runtime_options.run_global_event_cut) {
+    const size_t n1 = (size_t) runtime_options.number_of_events + 1;
+    if (!runtime_options.dev_ut_events) {
+      hipCheck(hipMemcpyAsync(
+        arguments.offset<dev_ut_raw_input>(), runtime_options.host_ut_events, runtime_options.host_ut_events_size,
+        hipMemcpyHostToDevice, stream));
+      hipCheck(hipMemcpyAsync(
+        arguments.offset<dev_ut_raw_input_offsets>(), runtime_options.host_ut_event_offsets, n1 * sizeof(unsigned),
+        hipMemcpyHostToDevice, stream));
+    }
+    if (!runtime_options.dev_scifi_events) {
+      hipCheck(hipMemcpyAsync(
+        arguments.offset<dev_scifi_raw_input>(), runtime_options.host_scifi_events,
+        runtime_options.host_scifi_events_size, hipMemcpyHostToDevice, stream));
+      hipCheck(hipMemcpyAsync(
+        arguments.offset<dev_scifi_raw_input_offsets>(), runtime_options.host_scifi_event_offsets,
+        n1 * sizeof(unsigned), hipMemcpyHostToDevice, stream));
+    }
+  }
   state.set_opts(dim3(runtime_options.number_of_events), dim3(256), stream);
   state.set_arguments(arguments.offset<dev_event_list>());
   state.invoke();
+}
+
+// ---- global_event_cut_t (GlobalEventCutVisitor.cu:4-55) ------------------------
+DEFINE_EMPTY_SET_ARGUMENTS_SIZE(global_event_cut_t)
+
+template<>
+void SequenceVisitor::visit<global_event_cut_t>(
+  global_event_cut_t& state,
+  const global_event_cut_t::arguments_t& arguments,
+  const RuntimeOptions& runtime_options,
+  const Constants&,
+  HostBuffers& host_buffers,
+  hipStream_t& stream,
+  hipEvent_t& event)
+{
+  if (!runtime_options.run_global_event_cut) {
+    for (unsigned i = 0; i < runtime_options.number_of_events; ++i) host_buffers.host_event_list[i] = i;
+    return;
+  }
+  // the launcher writes the count itself: no host-to-device reset
+  state.set_opts(dim3(runtime_options.number_of_events), dim3(256), stream);
+  state.set_arguments(
+    arguments.offset<dev_ut_raw_input>(),
+    arguments.offset<dev_ut_raw_input_offsets>(),
+    arguments.offset<dev_scifi_raw_input>(),
+    arguments.offset<dev_scifi_raw_input_offsets>(),
+    arguments.offset<dev_number_of_selected_events>(),
+    arguments.offset<dev_event_list>(),
+    arguments.size<dev_ut_raw_input>(),
+    arguments.size<dev_scifi_raw_input>());
+  state.invoke();
+  hipCheck(hipMemcpyAsync(
+    host_buffers.host_event_list, arguments.offset<dev_event_list>(),
+    (size_t) runtime_options.number_of_events * sizeof(unsigned), hipMemcpyDeviceToHost, stream));
+  read_total(host_buffers.host_number_of_selected_events, arguments.offset<dev_number_of_selected_events>(), stream,
+             event);
 }
 
 // ---- velo_estimate_input_size_t ----------------------------------------------

@@ -82,6 +82,28 @@ int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, char* err
   ro.dev_work_order = in->dev_work_order;
   ro.dev_ghost_model = in->dev_ghost_model;
   ro.run_velo_kalman_fit = in->run_velo_kalman_fit != 0;
+  ro.run_global_event_cut = in->run_global_event_cut != 0;
+  if (ro.run_global_event_cut) {
+    const bool ut_ok = in->dev_ut_events ? in->dev_ut_offsets != nullptr : in->host_ut_events && in->host_ut_offsets;
+    const bool scifi_ok =
+      in->dev_scifi_events ? in->dev_scifi_offsets != nullptr : in->host_scifi_events && in->host_scifi_offsets;
+    if (!ut_ok || !scifi_ok || ro.batched) {
+      set_error(err, err_len,
+                ro.batched ? "velo_sequence_run: the global event cut runs in plain runs only"
+                           : "velo_sequence_run: the global event cut needs UT and SciFi events with offsets");
+      return (int) hipErrorInvalidValue;
+    }
+    ro.host_ut_events = in->host_ut_events;
+    ro.host_ut_event_offsets = in->host_ut_offsets;
+    ro.host_ut_events_size = in->ut_events_size;
+    ro.host_scifi_events = in->host_scifi_events;
+    ro.host_scifi_event_offsets = in->host_scifi_offsets;
+    ro.host_scifi_events_size = in->scifi_events_size;
+    ro.dev_ut_events = in->dev_ut_events;
+    ro.dev_ut_event_offsets = in->dev_ut_offsets;
+    ro.dev_scifi_events = in->dev_scifi_events;
+    ro.dev_scifi_event_offsets = in->dev_scifi_offsets;
+  }
   try {
     s->stream->run_sequence(ro);
   }
@@ -122,6 +144,8 @@ int velo_sequence_output_get(const velo_sequence* s, velo_sequence_output* out)
   out->dev_module_cluster_num = st.output<dev_module_cluster_num>();
   out->dev_cluster_container = st.output<dev_velo_cluster_container>();
   out->memory_required = st.scheduler.memory_manager.total_memory_required;
+  out->host_event_list = hb.host_event_list;
+  out->dev_event_list = st.output<dev_event_list>();
   return 0;
 }
 

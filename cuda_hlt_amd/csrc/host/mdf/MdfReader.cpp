@@ -1,4 +1,4 @@
-// MDF reading for the VP banks (mdf/src/read_mdf.cpp:28-337,
+// MDF reading into the per-type raw-event layout (mdf/src/read_mdf.cpp:28-337,
 // mdf/src/raw_helpers.cpp:8-21, mdf/src/compression.cpp:51-130).
 //
 // An MDF record is a header (three copies of the record size, checksum,
@@ -72,41 +72,53 @@ std::vector<unsigned char> inflate_blocks(const unsigned char* src, size_t len)
   return out;
 }
 
-// Builds the event blob made of the VP banks in [banks, banks + len):
-// u32 n_banks, u32 byte offsets[n_banks + 1], then per bank the source id
-// and the bank's data words (read_mdf.cpp:131-183).  A first walk checks
-// the banks and sizes the blob, a second copies each bank's data at once.
-std::vector<char> vp_event(const unsigned char* banks, size_t len)
+// Builds, for each requested bank type, the event blob made of the banks
+// of that type in [banks, banks + len): u32 n_banks, u32 byte offsets
+// [n_banks + 1], then per bank the source id and the bank's data words
+// (read_mdf.cpp:131-183).  A first walk checks the banks and sizes the
+// blobs, a second copies each bank's data at once.
+std::vector<std::vector<char>> bank_events(const unsigned char* banks, size_t len, const uint32_t* types,
+                                           uint32_t n_types)
 {
-  uint32_t n_banks = 0;
-  size_t n_words = 0;
+  auto slot = [&](uint8_t type) {
+    for (uint32_t t = 0; t < n_types; ++t)
+      if (types[t] == type) return (int) t;
+    return -1;
+  };
+  std::vector<uint32_t> n_banks(n_types, 0);
+  std::vector<size_t> n_words(n_types, 0);
   size_t q = 0;
   while (q + 8 <= len) {
     const uint16_t length = rd16(banks + q + 2);
     if (rd16(banks + q) != kBankMagic) throw std::runtime_error("raw bank with a bad magic pattern");
     if (length < 8 || q + length > len) throw std::runtime_error("raw bank overruns its record");
-    if (banks[q + 4] == VELO_MDF_BANK_VP) {
-      ++n_banks;
-      n_words += 1 + (length - 8) / 4;
+    const int t = slot(banks[q + 4]);
+    if (t >= 0) {
+      ++n_banks[t];
+      n_words[t] += 1 + (length - 8) / 4;
     }
     q += (length + 3u) & ~3u;
   }
-  std::vector<char> ev(4 * (2 + (size_t) n_banks + n_words));
-  char* offsets = ev.data() + 4;
-  char* data = ev.data() + 4 * (2 + (size_t) n_banks);
-  std::memcpy(ev.data(), &n_banks, 4);
-  uint32_t at = 0, b = 0;
-  std::memcpy(offsets, &at, 4);
+  std::vector<std::vector<char>> out(n_types);
+  std::vector<uint32_t> at(n_types, 0), b(n_types, 0);
+  for (uint32_t t = 0; t < n_types; ++t) {
+    out[t].resize(4 * (2 + (size_t) n_banks[t] + n_words[t]));
+    std::memcpy(out[t].data(), &n_banks[t], 4);
+    std::memcpy(out[t].data() + 4, &at[t], 4);
+  }
   for (q = 0; q + 8 <= len; q += (rd16(banks + q + 2) + 3u) & ~3u) {
-    if (banks[q + 4] != VELO_MDF_BANK_VP) continue;
+    const int t = slot(banks[q + 4]);
+    if (t < 0) continue;
+    char* offsets = out[t].data() + 4;
+    char* data = out[t].data() + 4 * (2 + (size_t) n_banks[t]);
     const uint32_t source = rd16(banks + q + 6);
     const size_t bytes = 4 * ((rd16(banks + q + 2) - 8) / 4);
-    std::memcpy(data + at, &source, 4);
-    std::memcpy(data + at + 4, banks + q + 8, bytes);
-    at += (uint32_t) (4 + bytes);
-    std::memcpy(offsets + 4 * (++b), &at, 4);
+    std::memcpy(data + at[t], &source, 4);
+    std::memcpy(data + at[t] + 4, banks + q + 8, bytes);
+    at[t] += (uint32_t) (4 + bytes);
+    std::memcpy(offsets + 4 * (++b[t]), &at[t], 4);
   }
-  return ev;
+  return out;
 }
 
 struct Record {
@@ -116,7 +128,8 @@ struct Record {
 };
 
 // Checksum, decompression and bank walk of one whole record.
-std::vector<char> decode_record(const Record& r, bool verify_checksum)
+std::vector<std::vector<char>> decode_record(const Record& r, bool verify_checksum, const uint32_t* types,
+                                             uint32_t n_types)
 {
   const unsigned char* rec = r.bytes;
   const size_t size = r.size;
@@ -126,9 +139,9 @@ std::vector<char> decode_record(const Record& r, bool verify_checksum)
   const size_t start = kHeaderFixed + (size_t) (rec[17] & 0x0F) * 4 + 1; // pad byte after the subheader
   if (rec[16] & 0x0F) {
     const std::vector<unsigned char> banks = inflate_blocks(rec + start, size - start);
-    return vp_event(banks.data(), banks.size());
+    return bank_events(banks.data(), banks.size(), types, n_types);
   }
-  return vp_event(rec + start, size - start);
+  return bank_events(rec + start, size - start, types, n_types);
 }
 
 // A read-only mapping of one file.
@@ -208,15 +221,21 @@ uint32_t velo_mdf_checksum(const void* data, size_t len)
   return hash;
 }
 
-int velo_mdf_read_vp(const char* const* paths, uint32_t n_paths, uint32_t max_events, uint32_t start_event,
-                     int verify_checksum, uint32_t n_threads, velo_mdf_events* out, char* err, size_t err_len)
+int velo_mdf_read_banks(const char* const* paths, uint32_t n_paths, uint32_t max_events, uint32_t start_event,
+                        int verify_checksum, uint32_t n_threads, const uint32_t* bank_types, uint32_t n_types,
+                        velo_mdf_events* outs, char* err, size_t err_len)
 {
   auto fail = [&](const std::string& msg) {
     if (err && err_len) std::snprintf(err, err_len, "%s", msg.c_str());
     return -1;
   };
-  if (!out || (n_paths && !paths)) return fail("velo_mdf_read_vp: null argument");
-  std::memset(out, 0, sizeof(*out));
+  if (!outs || !bank_types || n_types == 0 || (n_paths && !paths)) return fail("velo_mdf_read_banks: null argument");
+  for (uint32_t t = 0; t < n_types; ++t) std::memset(&outs[t], 0, sizeof(outs[t]));
+  for (uint32_t t = 0; t < n_types; ++t) {
+    if (bank_types[t] > 255) return fail("bank type " + std::to_string(bank_types[t]) + " is not a raw bank type");
+    for (uint32_t u = 0; u < t; ++u)
+      if (bank_types[u] == bank_types[t]) return fail("bank type " + std::to_string(bank_types[t]) + " requested twice");
+  }
   std::vector<std::unique_ptr<Mapping>> files;
   std::vector<Record> records;
   uint32_t n_records = 0;
@@ -229,7 +248,7 @@ int velo_mdf_read_vp(const char* const* paths, uint32_t n_paths, uint32_t max_ev
 
   // Records decode independently: thread t takes records t, t + T, ...
   const size_t n = records.size();
-  std::vector<std::vector<char>> events(n);
+  std::vector<std::vector<std::vector<char>>> events(n);
   const size_t T = std::max<size_t>(1, std::min<size_t>(n_threads ? n_threads : 1, n));
   auto parallel = [T](auto&& body) {
     if (T == 1) return body(0);
@@ -240,7 +259,8 @@ int velo_mdf_read_vp(const char* const* paths, uint32_t n_paths, uint32_t max_ev
   std::vector<std::string> errors(T);
   parallel([&](size_t t) {
     try {
-      for (size_t i = t; i < n; i += T) events[i] = decode_record(records[i], verify_checksum != 0);
+      for (size_t i = t; i < n; i += T)
+        events[i] = decode_record(records[i], verify_checksum != 0, bank_types, n_types);
     } catch (const std::exception& e) {
       errors[t] = e.what();
     }
@@ -256,27 +276,42 @@ int velo_mdf_read_vp(const char* const* paths, uint32_t n_paths, uint32_t max_ev
         }
   if (first_bad < n) return fail(errors[first_bad % T]);
 
-  std::vector<size_t> offsets(n + 1, 0);
-  for (size_t i = 0; i < n; ++i) offsets[i + 1] = offsets[i] + events[i].size();
-  const size_t total = offsets[n];
-  if (total > 0xFFFFFFFFull) return fail("more than 4 GiB of VP events (offsets are 32-bit)");
-  out->events = static_cast<char*>(std::malloc(total ? total : 1));
-  out->offsets = static_cast<uint32_t*>(std::malloc(4 * (n + 1)));
-  if (!out->events || !out->offsets) {
-    velo_mdf_free(out);
-    return fail("out of host memory");
-  }
-  for (size_t i = 0; i <= n; ++i) out->offsets[i] = (uint32_t) offsets[i];
-  parallel([&](size_t t) {
-    for (size_t i = t; i < n; i += T) {
-      std::memcpy(out->events + offsets[i], events[i].data(), events[i].size());
-      std::vector<char>().swap(events[i]);
+  for (uint32_t k = 0; k < n_types; ++k) {
+    velo_mdf_events* out = &outs[k];
+    std::vector<size_t> offsets(n + 1, 0);
+    for (size_t i = 0; i < n; ++i) offsets[i + 1] = offsets[i] + events[i][k].size();
+    const size_t total = offsets[n];
+    if (total > 0xFFFFFFFFull) {
+      for (uint32_t u = 0; u < n_types; ++u) velo_mdf_free(&outs[u]);
+      return fail("more than 4 GiB of events of bank type " + std::to_string(bank_types[k]) +
+                  " (offsets are 32-bit)");
     }
-  });
-  out->n_events = (uint32_t) n;
-  out->n_records = n_records;
-  out->events_size = total;
+    out->events = static_cast<char*>(std::malloc(total ? total : 1));
+    out->offsets = static_cast<uint32_t*>(std::malloc(4 * (n + 1)));
+    if (!out->events || !out->offsets) {
+      for (uint32_t u = 0; u < n_types; ++u) velo_mdf_free(&outs[u]);
+      return fail("out of host memory");
+    }
+    for (size_t i = 0; i <= n; ++i) out->offsets[i] = (uint32_t) offsets[i];
+    parallel([&](size_t t) {
+      for (size_t i = t; i < n; i += T) {
+        std::memcpy(out->events + offsets[i], events[i][k].data(), events[i][k].size());
+        std::vector<char>().swap(events[i][k]);
+      }
+    });
+    out->n_events = (uint32_t) n;
+    out->n_records = n_records;
+    out->events_size = total;
+  }
   return 0;
+}
+
+int velo_mdf_read_vp(const char* const* paths, uint32_t n_paths, uint32_t max_events, uint32_t start_event,
+                     int verify_checksum, uint32_t n_threads, velo_mdf_events* out, char* err, size_t err_len)
+{
+  const uint32_t vp = VELO_MDF_BANK_VP;
+  return velo_mdf_read_banks(paths, n_paths, max_events, start_event, verify_checksum, n_threads, &vp, 1, out, err,
+                             err_len);
 }
 
 void velo_mdf_free(velo_mdf_events* ev)

@@ -11,6 +11,10 @@ using uint = unsigned int;
 
 ARGUMENT(dev_velo_raw_input, char)
 ARGUMENT(dev_velo_raw_input_offsets, uint)
+ARGUMENT(dev_ut_raw_input, char)
+ARGUMENT(dev_ut_raw_input_offsets, uint)
+ARGUMENT(dev_scifi_raw_input, char)
+ARGUMENT(dev_scifi_raw_input_offsets, uint)
 ARGUMENT(dev_event_list, uint)
 ARGUMENT(dev_event_order, uint)
 ARGUMENT(dev_number_of_selected_events, uint)

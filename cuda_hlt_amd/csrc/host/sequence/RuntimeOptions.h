@@ -40,6 +40,21 @@ struct RuntimeOptions {
   // Persistent launches (tuning.grid_dim > 0): processing order of the
   // events (largest first balances uneven events), NULL = identity.
   const unsigned* dev_work_order = nullptr;
+  // UT and SciFi banks for the global event cut (main/include/
+  // RuntimeOptions.h:14-21): host input copied by init_event_list, or
+  // device-resident input used in place.  Raw-event layout (u32 n_banks,
+  // byte offsets, per bank the source id and its words).
+  bool run_global_event_cut = false;
+  const char* host_ut_events = nullptr;
+  const unsigned* host_ut_event_offsets = nullptr;
+  size_t host_ut_events_size = 0;
+  const char* host_scifi_events = nullptr;
+  const unsigned* host_scifi_event_offsets = nullptr;
+  size_t host_scifi_events_size = 0;
+  const char* dev_ut_events = nullptr;
+  const unsigned* dev_ut_event_offsets = nullptr;
+  const char* dev_scifi_events = nullptr;
+  const unsigned* dev_scifi_event_offsets = nullptr;
   // Run velo_kalman_fit after consolidation (KalmanFilterVisitor.cu:4-44).
   bool run_velo_kalman_fit = false;
   // Ghost filter run inline by consolidation (a device velo_ghost_model),

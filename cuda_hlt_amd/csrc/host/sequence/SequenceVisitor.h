@@ -46,6 +46,7 @@ struct SequenceVisitor {
   {}
 
 DECLARE_VISITOR(init_event_list_t)
+DECLARE_VISITOR(global_event_cut_t)
 DECLARE_VISITOR(velo_estimate_input_size_t)
 DECLARE_VISITOR(prefix_sum_velo_clusters_t)
 DECLARE_VISITOR(velo_masked_clustering_t)
@@ -57,3 +58,4 @@ DECLARE_VISITOR(copy_and_prefix_sum_single_block_velo_t)
 DECLARE_VISITOR(copy_velo_track_hit_number_t)
 DECLARE_VISITOR(prefix_sum_velo_track_hit_number_t)
 DECLARE_VISITOR(consolidate_velo_tracks_t)
+DECLARE_VISITOR(velo_kalman_fit_t)

@@ -1,8 +1,8 @@
 // The algorithms of the VELO sequence and the configured sequence itself
-// (configuration/sequences/Velo.h:5-18 without the global event cut, which
-// needs UT and SciFi banks).  Each ALGORITHM binds a C-ABI launcher of
-// include/velo_hlt.h with the reference's dependency list (plus the status
-// word).
+// (configuration/sequences/Velo.h:5-18; the global event cut runs when the
+// caller supplies UT and SciFi banks and asks for it).  Each ALGORITHM binds
+// a C-ABI launcher of include/velo_hlt.h with the reference's dependency
+// list (plus the status word).
 #pragma once
 
 #include <tuple>
@@ -14,7 +14,26 @@
 ALGORITHM(
   velo_init_event_list_launch,
   init_event_list_t,
-  ARGUMENTS(dev_velo_raw_input, dev_velo_raw_input_offsets, dev_number_of_selected_events, dev_event_list))
+  ARGUMENTS(
+    dev_velo_raw_input,
+    dev_velo_raw_input_offsets,
+    dev_ut_raw_input,
+    dev_ut_raw_input_offsets,
+    dev_scifi_raw_input,
+    dev_scifi_raw_input_offsets,
+    dev_number_of_selected_events,
+    dev_event_list))
+
+ALGORITHM(
+  velo_global_event_cut_launch,
+  global_event_cut_t,
+  ARGUMENTS(
+    dev_ut_raw_input,
+    dev_ut_raw_input_offsets,
+    dev_scifi_raw_input,
+    dev_scifi_raw_input_offsets,
+    dev_number_of_selected_events,
+    dev_event_list))
 
 ALGORITHM(
   velo_estimate_input_size_launch,
@@ -138,6 +157,7 @@ ALGORITHM(
 
 SEQUENCE_T(
   init_event_list_t,
+  global_event_cut_t,
   velo_estimate_input_size_t,
   prefix_sum_velo_clusters_t,
   velo_masked_clustering_t,

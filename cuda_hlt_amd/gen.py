@@ -134,3 +134,66 @@ def minbias(n_events: int, seed: int = 0) -> Events:
 def uneven(n_events: int, seed: int = 0) -> Events:
     """Per-event nu ~ U(2, 11): very uneven occupancy."""
     return generate(n_events, seed=seed, nu_min=2.0, nu_max=11.0)
+
+
+# --- UT and SciFi banks for the global event cut ---------------------------------
+
+UT_BANKS = 180          # banks per event in the reference's UT sample
+SCIFI_BANKS = 240       # banks per event in the reference's FTCluster sample
+GEC_SCALE = 0.72        # UT + SciFi clusters per VP raw byte
+UT_FRACTION = 0.21      # UT share of the two counts (the reference sample's)
+
+
+@dataclass
+class GecBanks:
+    """UT and SciFi raw events (u32 n_banks, u32 byte offsets, per bank the
+    source id and its data words) whose only content the global event cut
+    reads is right: UT hit counts in the low 16 bits of each bank's first
+    data word, SciFi bank sizes."""
+    ut_events: np.ndarray
+    ut_offsets: np.ndarray
+    scifi_events: np.ndarray
+    scifi_offsets: np.ndarray
+    ut_hits: np.ndarray          # per event, as the cut counts them
+    scifi_clusters: np.ndarray   # per event: data bytes / 2 - 2
+
+
+def _raw_event(words_per_bank: np.ndarray, first_words: np.ndarray | None, rng) -> np.ndarray:
+    nb = len(words_per_bank)
+    sizes = 4 * (1 + words_per_bank.astype(np.int64))
+    offs = np.zeros(nb + 1, np.uint32)
+    offs[1:] = np.cumsum(sizes)
+    payload = rng.integers(0, 2**32, size=int(offs[-1]) // 4, dtype=np.uint32)
+    starts = offs[:-1] // 4
+    payload[starts] = np.arange(nb, dtype=np.uint32)          # source ids
+    if first_words is not None:
+        payload[starts + 1] = first_words
+    return np.concatenate([np.array([nb], np.uint32), offs, payload]).view(np.uint8)
+
+
+def gec_banks(events: Events, seed: int = 0, scale: float = GEC_SCALE) -> GecBanks:
+    """UT and SciFi events matching VP events: UT + SciFi clusters
+    ~ scale x VP raw bytes (8 % spread), scaled so that about a tenth of
+    nu = 7.6 minimum-bias events fail the cut, the removal the reference's
+    threshold aims at (GlobalEventCut.cuh:9)."""
+    rng = np.random.default_rng(seed)
+    vp = np.diff(events.offsets.astype(np.int64))
+    total = np.maximum(0, vp * scale * (1 + 0.08 * rng.standard_normal(len(vp)))).astype(np.int64)
+    ut_total = (total * UT_FRACTION).astype(np.int64)
+    scifi_total = 2 * ((total - ut_total) // 2)                # the estimate is always even
+    uts, sfs = [], []
+    for e in range(len(vp)):
+        hits = rng.multinomial(int(ut_total[e]), np.full(UT_BANKS, 1.0 / UT_BANKS))
+        uts.append(_raw_event(1 + (hits + 1) // 2, (0x0080 << 16) | hits.astype(np.uint32), rng))
+        # data words = (estimate + 2) / 2 (4 bytes hold 2 clusters)
+        words = rng.multinomial(int(scifi_total[e]) // 2 + 1, np.full(SCIFI_BANKS, 1.0 / SCIFI_BANKS))
+        sfs.append(_raw_event(words, None, rng))
+
+    def cat(parts):
+        offs = np.zeros(len(parts) + 1, np.uint32)
+        offs[1:] = np.cumsum([p.nbytes for p in parts])
+        return (np.concatenate(parts) if parts else np.zeros(0, np.uint8)), offs
+
+    ue, uo = cat(uts)
+    se, so = cat(sfs)
+    return GecBanks(ue, uo, se, so, ut_total.astype(np.uint32), scifi_total.astype(np.uint32))

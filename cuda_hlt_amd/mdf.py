@@ -28,7 +28,10 @@ import numpy as np
 
 from . import native as nat
 
+BANK_MUON = 13
 BANK_VP = 63
+BANK_FT_CLUSTER = 64
+BANK_UT = 66
 BANK_VP_VERSION = 2
 BANK_MAGIC = 0xCBCB
 HEADER_VERSION = 3
@@ -52,6 +55,10 @@ def _lib():
         L.velo_mdf_read_vp.argtypes = [C.POINTER(C.c_char_p), C.c_uint32, C.c_uint32, C.c_uint32, C.c_int,
                                        C.c_uint32, C.POINTER(_Events), C.c_char_p, C.c_size_t]
         L.velo_mdf_read_vp.restype = C.c_int
+        L.velo_mdf_read_banks.argtypes = [C.POINTER(C.c_char_p), C.c_uint32, C.c_uint32, C.c_uint32, C.c_int,
+                                          C.c_uint32, C.POINTER(C.c_uint32), C.c_uint32, C.POINTER(_Events),
+                                          C.c_char_p, C.c_size_t]
+        L.velo_mdf_read_banks.restype = C.c_int
         L.velo_mdf_free.argtypes = [C.POINTER(_Events)]
         L.velo_mdf_free.restype = None
         L.velo_mdf_checksum.argtypes = [C.c_void_p, C.c_size_t]
@@ -78,19 +85,7 @@ class MdfEvents:
         return self.events[int(self.offsets[i]):int(self.offsets[i + 1])].tobytes()
 
 
-def read_vp(paths: str | Path | Sequence[str | Path], max_events: int = 0, start_event: int = 0,
-            verify_checksum: bool = True, threads: int = 1) -> MdfEvents:
-    """VP events of the MDF files, in file and record order."""
-    if isinstance(paths, (str, Path)):
-        paths = [paths]
-    L = _lib()
-    names = (C.c_char_p * len(paths))(*[str(p).encode() for p in paths])
-    out = _Events()
-    err = C.create_string_buffer(512)
-    rc = L.velo_mdf_read_vp(names, len(paths), max_events, start_event, int(verify_checksum), threads,
-                            C.byref(out), err, 512)
-    if rc != 0:
-        raise MdfError(err.value.decode())
+def _take(L, out: _Events) -> MdfEvents:
     try:
         events = (np.ctypeslib.as_array((C.c_uint8 * out.events_size).from_address(out.events)).copy()
                   if out.events_size else np.zeros(0, np.uint8))
@@ -98,6 +93,41 @@ def read_vp(paths: str | Path | Sequence[str | Path], max_events: int = 0, start
         return MdfEvents(events, offsets, int(out.n_records))
     finally:
         L.velo_mdf_free(C.byref(out))
+
+
+def _names(paths):
+    if isinstance(paths, (str, Path)):
+        paths = [paths]
+    return (C.c_char_p * len(paths))(*[str(p).encode() for p in paths]), len(paths)
+
+
+def read_vp(paths: str | Path | Sequence[str | Path], max_events: int = 0, start_event: int = 0,
+            verify_checksum: bool = True, threads: int = 1) -> MdfEvents:
+    """VP events of the MDF files, in file and record order."""
+    L = _lib()
+    names, n = _names(paths)
+    out = _Events()
+    err = C.create_string_buffer(512)
+    rc = L.velo_mdf_read_vp(names, n, max_events, start_event, int(verify_checksum), threads, C.byref(out), err, 512)
+    if rc != 0:
+        raise MdfError(err.value.decode())
+    return _take(L, out)
+
+
+def read_banks(paths: str | Path | Sequence[str | Path], bank_types: Sequence[int], max_events: int = 0,
+               start_event: int = 0, verify_checksum: bool = True, threads: int = 1) -> dict[int, MdfEvents]:
+    """Events of each bank type in one pass over the files (as the
+    reference's MDF::read_events fills one buffer per type)."""
+    L = _lib()
+    names, n = _names(paths)
+    types = (C.c_uint32 * len(bank_types))(*bank_types)
+    outs = (_Events * len(bank_types))()
+    err = C.create_string_buffer(512)
+    rc = L.velo_mdf_read_banks(names, n, max_events, start_event, int(verify_checksum), threads, types,
+                               len(bank_types), outs, err, 512)
+    if rc != 0:
+        raise MdfError(err.value.decode())
+    return {int(t): _take(L, outs[k]) for k, t in enumerate(bank_types)}
 
 
 def checksum(data: bytes) -> int:

@@ -101,8 +101,14 @@ KalmanFitArgs = _args("KalmanFitArgs", [
     "dev_atomics_velo", "dev_velo_track_hit_number", "dev_velo_track_hits", "dev_velo_states",
     "dev_velo_kalman_beamline_states"])
 
+GlobalEventCutArgs = type("GlobalEventCutArgs", (C.Structure,), {"_fields_": [
+    ("dev_ut_raw_input", P), ("dev_ut_raw_input_offsets", P), ("dev_scifi_raw_input", P),
+    ("dev_scifi_raw_input_offsets", P), ("dev_number_of_selected_events", P), ("dev_event_list", P),
+    ("ut_events_size", C.c_size_t), ("scifi_events_size", C.c_size_t), ("n_events", U32), ("tuning", Tuning)]})
+
 LAUNCHERS = {
     "velo_init_event_list_launch": InitEventListArgs,
+    "velo_global_event_cut_launch": GlobalEventCutArgs,
     "velo_estimate_input_size_launch": EstimateInputSizeArgs,
     "velo_prefix_sum_launch": PrefixSumArgs,
     "velo_masked_clustering_launch": MaskedClusteringArgs,

@@ -9,7 +9,13 @@ tables kept beside the benchmark:
 * ``trace``    mean duration per kernel and launch shape (--kernel-trace);
 * ``traffic``  HBM bytes per dispatch from two ``--pmc`` runs (FETCH_SIZE,
   then WRITE_SIZE), each scaled by what the same counter reads for a copy
-  kernel of known size run in the same process.
+  kernel of known size run in the same process;
+* ``lds``      LDS throughput, bank conflicts and occupancy per kernel from
+  one ``--pmc`` run of SQ_INSTS_LDS_LOAD_BANDWIDTH,
+  SQ_INSTS_LDS_STORE_BANDWIDTH, SQ_LDS_BANK_CONFLICT, SQ_WAVE_CYCLES and
+  GRBM_GUI_ACTIVE, normalised as rocprofv3's derived LDSBankConflict and
+  OccupancyPercent are, beside the occupancy the kernel's registers and
+  LDS allow.
 
 Usage::
 
@@ -18,6 +24,8 @@ Usage::
     python -m cuda_hlt_amd.profiling trace --csv DIR --command "..." --title "..."
     python -m cuda_hlt_amd.profiling traffic --fetch DIR --write DIR --copy-bytes N \\
         --events-per-launch 10000 --command "..." --out-json F --out-md F
+    python -m cuda_hlt_amd.profiling lds --csv DIR [--trace DIR] --command "..." --title "..." \\
+        [--out-json F]
 """
 from __future__ import annotations
 
@@ -176,6 +184,103 @@ def traffic_markdown(t: dict, title: str, command: str) -> str:
     return "\n".join(lines) + "\n"
 
 
+def read_agent(path: str | Path) -> dict:
+    """The GPU agent's row of rocprofv3's agent_info.csv."""
+    f = _find(path, "agent_info.csv")
+    with open(f, newline="") as fh:
+        for row in csv.DictReader(fh):
+            if row.get("Agent_Type", "").upper() == "GPU" or int(row.get("Simd_Count") or 0) > 0:
+                return row
+    raise ValueError(f"no GPU agent in {f}")
+
+
+def theoretical_waves_per_cu(wg_size: int, vgprs: int, agprs: int, lds_bytes: int, agent: dict) -> dict:
+    """Resident waves per CU the kernel's resources allow: the register file
+    (512 registers per lane per SIMD, VGPR and AGPR together, granule 8),
+    the LDS per CU and the wave slots."""
+    simds = int(agent.get("Simd_Per_Cu") or 4)
+    slots = int(agent.get("Max_Waves_Per_Simd") or 8)
+    lds_cu = int(float(agent.get("Lds_Size_In_Kb") or 160) * 1024)
+    waves_per_wg = max(1, -(-wg_size // 64))
+    regs = -(-(vgprs + agprs) // 8) * 8
+    by_regs = min(slots, 512 // regs) if regs else slots
+    wg_by_waves = (simds * by_regs) // waves_per_wg
+    wg_by_lds = lds_cu // lds_bytes if lds_bytes else wg_by_waves
+    wgs = min(wg_by_waves, wg_by_lds)
+    limit = "lds" if wg_by_lds < wg_by_waves else ("registers" if by_regs < slots else "wave slots")
+    return {"waves": wgs * waves_per_wg, "max_waves": simds * slots, "limit": limit}
+
+
+def lds_occupancy(path: str | Path, trace: str | Path | None = None) -> dict:
+    """Per kernel (mean per dispatch): LDS bytes moved, LDS throughput,
+    bank-conflict share of active cycles, mean resident waves per CU and
+    their share of the wave slots, and the theoretical occupancy."""
+    f = _find(path, "counter_collection.csv")
+    agent = read_agent(path)
+    cus = int(agent["Cu_Count"])
+    xcc = int(agent.get("Num_Xcc") or 1)
+    clk_mhz = float(agent.get("Max_Engine_Clk_Fcompute") or 0)
+    slots_cu = int(agent.get("Simd_Per_Cu") or 4) * int(agent.get("Max_Waves_Per_Simd") or 8)
+    disp: dict[tuple, dict] = {}
+    with open(f, newline="") as fh:
+        for row in csv.DictReader(fh):
+            key = (short_name(row["Kernel_Name"]), row["Dispatch_Id"])
+            d = disp.setdefault(key, {"ns": int(row["End_Timestamp"]) - int(row["Start_Timestamp"]),
+                                      "wg": int(row.get("Workgroup_Size") or 64),
+                                      "vgpr": int(row.get("VGPR_Count") or 0),
+                                      "agpr": int(row.get("Accum_VGPR_Count") or 0),
+                                      "lds": int(row.get("LDS_Block_Size") or 0)})
+            d[row["Counter_Name"]] = d.get(row["Counter_Name"], 0.0) + float(row["Counter_Value"])
+    durations = {}
+    if trace is not None:
+        for r in read_trace(trace):
+            durations.setdefault(r["kernel"], r["average_us"])
+    per_kernel: dict[str, list[dict]] = defaultdict(list)
+    for (k, _), d in disp.items():
+        if not _is_velo_kernel(k) or "GRBM_GUI_ACTIVE" not in d:
+            continue
+        gui = d["GRBM_GUI_ACTIVE"]
+        # one GRBM_GUI_ACTIVE per XCD summed into the row: keep one of them
+        if clk_mhz and d["ns"] and gui > 1.5 * d["ns"] * clk_mhz * 1e-3:
+            gui /= xcc
+        lds_bytes = 64.0 * (d.get("SQ_INSTS_LDS_LOAD_BANDWIDTH", 0.0) + d.get("SQ_INSTS_LDS_STORE_BANDWIDTH", 0.0))
+        waves = 4.0 * d.get("SQ_WAVE_CYCLES", 0.0) / gui / cus if gui else 0.0
+        per_kernel[k].append({"ns": d["ns"], "gui": gui, "lds_bytes": lds_bytes,
+                              "conflict": 100.0 * d.get("SQ_LDS_BANK_CONFLICT", 0.0) / gui / cus if gui else 0.0,
+                              "waves": waves, "wg": d["wg"], "vgpr": d["vgpr"], "agpr": d["agpr"], "lds": d["lds"]})
+    out = {}
+    for k, rs in per_kernel.items():
+        n = len(rs)
+        mean = {f: sum(r[f] for r in rs) / n for f in ("ns", "gui", "lds_bytes", "conflict", "waves")}
+        us = durations.get(k, mean["ns"] / 1e3)
+        theo = theoretical_waves_per_cu(rs[0]["wg"], rs[0]["vgpr"], rs[0]["agpr"], rs[0]["lds"], agent)
+        out[k] = {"dispatches": n, "duration_us": round(us, 2), "lds_bytes": round(mean["lds_bytes"]),
+                  "lds_TBps": round(mean["lds_bytes"] / (us * 1e-6) / 1e12, 3) if us else None,
+                  "lds_bytes_per_cu_cycle": round(mean["lds_bytes"] / (mean["gui"] * cus), 2) if mean["gui"] else None,
+                  "lds_bank_conflict_pct": round(mean["conflict"], 2),
+                  "waves_per_cu": round(mean["waves"], 2), "occupancy_pct": round(100 * mean["waves"] / slots_cu, 1),
+                  "theoretical_waves_per_cu": theo["waves"], "theoretical_limit": theo["limit"],
+                  "workgroup_size": rs[0]["wg"], "vgprs": rs[0]["vgpr"], "agprs": rs[0]["agpr"],
+                  "lds_per_workgroup": rs[0]["lds"]}
+    return {"cu_count": cus, "wave_slots_per_cu": slots_cu, "num_xcc": xcc, "kernels": out}
+
+
+def lds_markdown(t: dict, title: str, command: str) -> str:
+    lines = _header(title, command) + [
+        f"{t['cu_count']} CUs, {t['wave_slots_per_cu']} wave slots per CU.  LDS bytes = 64 x "
+        "(SQ_INSTS_LDS_LOAD_BANDWIDTH + SQ_INSTS_LDS_STORE_BANDWIDTH); conflicts = SQ_LDS_BANK_CONFLICT / "
+        "(GRBM_GUI_ACTIVE x CUs); resident waves per CU = 4 x SQ_WAVE_CYCLES / (GRBM_GUI_ACTIVE x CUs).", "",
+        "| kernel | dispatches | us | LDS bytes | LDS TB/s | LDS B/CU/cycle | bank conflict % | waves/CU | "
+        "occupancy % | resources allow | limited by | WG size | VGPR | AGPR | LDS/WG |",
+        "|---|---|---|---|---|---|---|---|---|---|---|---|---|---|---|"]
+    for k, v in sorted(t["kernels"].items(), key=lambda kv: -kv[1]["duration_us"]):
+        lines.append(f"| `{k}` | {v['dispatches']} | {v['duration_us']} | {v['lds_bytes']} | {v['lds_TBps']} | "
+                     f"{v['lds_bytes_per_cu_cycle']} | {v['lds_bank_conflict_pct']} | {v['waves_per_cu']} | "
+                     f"{v['occupancy_pct']} | {v['theoretical_waves_per_cu']} | {v['theoretical_limit']} | "
+                     f"{v['workgroup_size']} | {v['vgprs']} | {v['agprs']} | {v['lds_per_workgroup']} |")
+    return "\n".join(lines) + "\n"
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -194,6 +299,12 @@ def main(argv=None):
     p.add_argument("--title", default="HBM traffic per dispatch")
     p.add_argument("--out-json", required=True)
     p.add_argument("--out-md", required=True)
+    p = sub.add_parser("lds")
+    p.add_argument("--csv", required=True)
+    p.add_argument("--trace")
+    p.add_argument("--command", required=True)
+    p.add_argument("--title", required=True)
+    p.add_argument("--out-json")
     a = ap.parse_args(argv)
     if a.cmd == "stats":
         sys.stdout.write(stats_markdown(a.csv, a.title, a.command))
@@ -201,6 +312,12 @@ def main(argv=None):
         sys.stdout.write(counters_markdown(a.csv, a.title, a.command))
     elif a.cmd == "trace":
         sys.stdout.write(trace_markdown(a.csv, a.title, a.command))
+    elif a.cmd == "lds":
+        t = lds_occupancy(a.csv, a.trace)
+        t["command"] = a.command
+        if a.out_json:
+            Path(a.out_json).write_text(json.dumps(t, indent=1) + "\n")
+        sys.stdout.write(lds_markdown(t, a.title, a.command))
     else:
         t = traffic(a.fetch, a.write, a.copy_bytes, a.copy_match)
         t["events_per_launch"] = a.events_per_launch

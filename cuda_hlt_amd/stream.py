@@ -20,7 +20,11 @@ class SequenceInput(C.Structure):
     _fields_ = [("host_events", P), ("host_offsets", P), ("events_size", C.c_size_t), ("dev_events", P),
                 ("dev_offsets", P), ("n_events", C.c_uint32), ("repetitions", C.c_uint32),
                 ("copy_outputs_to_host", C.c_int), ("tuning", nat.Tuning * 6), ("batched", C.c_int),
-                ("dev_work_order", P), ("dev_ghost_model", P), ("run_velo_kalman_fit", C.c_int)]
+                ("dev_work_order", P), ("dev_ghost_model", P), ("run_velo_kalman_fit", C.c_int),
+                ("run_global_event_cut", C.c_int), ("host_ut_events", P), ("host_ut_offsets", P),
+                ("ut_events_size", C.c_size_t), ("host_scifi_events", P), ("host_scifi_offsets", P),
+                ("scifi_events_size", C.c_size_t), ("dev_ut_events", P), ("dev_ut_offsets", P),
+                ("dev_scifi_events", P), ("dev_scifi_offsets", P)]
 
 
 class SequenceOutput(C.Structure):
@@ -31,7 +35,7 @@ class SequenceOutput(C.Structure):
                 ("dev_track_hits", P), ("dev_states", P), ("dev_accepted", P), ("dev_kalman_states", P),
                 ("dev_event_status", P),
                 ("dev_module_cluster_start", P), ("dev_module_cluster_num", P), ("dev_cluster_container", P),
-                ("memory_required", C.c_size_t)]
+                ("memory_required", C.c_size_t), ("host_event_list", P), ("dev_event_list", P)]
 
 
 TUNING_SLOTS = ("estimate_input_size", "masked_clustering", "calculate_phi_and_sort", "fill_candidates",
@@ -71,6 +75,7 @@ class HostTracks:
     states: np.ndarray        # uint8[49 T]
     accepted: np.ndarray      # uint8[T]: dev_accepted_velo_tracks
     kalman: np.ndarray | None = None  # uint8[49 T]: velo_kalman_fit states, when run
+    event_list: np.ndarray | None = None  # uint32[N]: input event of each slot
 
 
 class VeloSequence:
@@ -103,7 +108,7 @@ class VeloSequence:
 
     def run(self, n_events, host_events=None, host_offsets=None, dev_events=None, dev_offsets=None,
             copy_outputs=True, repetitions=1, tuning=None, events_size=0, batched=False,
-            dev_work_order=None, dev_ghost_model=None, velo_kalman_fit=False) -> SequenceOutput:
+            dev_work_order=None, dev_ghost_model=None, velo_kalman_fit=False, gec=None) -> SequenceOutput:
         """Run the sequence on host input (copied by init_event_list) or on
         device-resident input (dev_events/dev_offsets; events_size = their
         bytes, needed by batched runs to bound the buffers).  batched=True
@@ -112,7 +117,12 @@ class VeloSequence:
         name to (block_dim, lds_tile) or (block_dim, lds_tile, grid_dim);
         grid_dim > 0 runs that step persistent in dev_work_order.
         dev_ghost_model (a device velo_ghost_model) makes consolidation
-        score every track and clear the accepted flag of likely ghosts."""
+        score every track and clear the accepted flag of likely ghosts.
+        gec runs the global event cut first: host arrays (ut_events,
+        ut_offsets, scifi_events, scifi_offsets), e.g. a gen.GecBanks, or
+        a dict of device pointers with the same keys plus ut_events_size and
+        scifi_events_size; the output then covers the kept events, listed
+        in HostTracks.event_list."""
         inp = SequenceInput()
         if dev_events is not None:
             inp.dev_events, inp.dev_offsets = int(dev_events), int(dev_offsets)
@@ -131,6 +141,19 @@ class VeloSequence:
         inp.dev_work_order = None if dev_work_order is None else int(dev_work_order)
         inp.dev_ghost_model = None if dev_ghost_model is None else int(dev_ghost_model)
         inp.run_velo_kalman_fit = int(velo_kalman_fit)
+        if gec is not None:
+            inp.run_global_event_cut = 1
+            if isinstance(gec, dict):
+                inp.dev_ut_events, inp.dev_ut_offsets = int(gec["ut_events"]), int(gec["ut_offsets"])
+                inp.dev_scifi_events, inp.dev_scifi_offsets = int(gec["scifi_events"]), int(gec["scifi_offsets"])
+                inp.ut_events_size, inp.scifi_events_size = int(gec["ut_events_size"]), int(gec["scifi_events_size"])
+            else:
+                self._keep_gec = [np.ascontiguousarray(a) for a in (gec.ut_events, gec.scifi_events)] + \
+                    [np.ascontiguousarray(a, np.uint32) for a in (gec.ut_offsets, gec.scifi_offsets)]
+                ue, se, uo, so = self._keep_gec
+                inp.host_ut_events, inp.host_ut_offsets, inp.ut_events_size = ue.ctypes.data, uo.ctypes.data, ue.nbytes
+                inp.host_scifi_events, inp.host_scifi_offsets = se.ctypes.data, so.ctypes.data
+                inp.scifi_events_size = se.nbytes
         err = C.create_string_buffer(1024)
         rc = _lib().velo_sequence_run(self.h, C.byref(inp), err, 1024)
         if rc != 0:
@@ -165,4 +188,5 @@ class VeloSequence:
                           arr(o.host_track_hit_number, T + 1, np.uint32),
                           arr(o.host_track_hits, 16 * Ht, np.uint8), arr(o.host_states, 49 * T, np.uint8),
                           arr(o.host_accepted, T, np.uint8),
-                          arr(o.host_kalman_states, 49 * T, np.uint8) if o.host_kalman_states else None)
+                          arr(o.host_kalman_states, 49 * T, np.uint8) if o.host_kalman_states else None,
+                          arr(o.host_event_list, n, np.uint32))

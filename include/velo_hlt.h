@@ -110,6 +110,34 @@ typedef struct {
 } velo_init_event_list_args;
 int velo_init_event_list_launch(const velo_init_event_list_args* a, void* stream);
 
+/* ---- global_event_cut (GlobalEventCut.cu:3-55, GlobalEventCut.cuh:9) ----
+ * Keeps an event when its SciFi cluster estimate (bank bytes without the
+ * 4-byte source word, halved, minus 2, in 32-bit unsigned arithmetic) and
+ * that estimate plus its UT hit count (low 16 bits of each UT bank's first
+ * data word) both stay below VELO_GEC_MAX_SCIFI_UT_CLUSTERS.  UT and SciFi
+ * events use the raw-event layout (u32 n_banks, u32 byte offsets[n_banks+1],
+ * per bank the source id and its data words).  dev_event_list receives the
+ * kept events in increasing order (the reference appends them in
+ * atomic-arrival order; the set and the count are the same) and
+ * dev_number_of_selected_events their count.  An event whose banks reach
+ * past ut_events_size / scifi_events_size is not kept (the reference has no
+ * sizes and reads on).  dev_event_list holds n_events words; it is used as
+ * scratch for per-event flags before the compaction. */
+#define VELO_GEC_MAX_SCIFI_UT_CLUSTERS 9750u
+typedef struct {
+  const char* dev_ut_raw_input;
+  const uint32_t* dev_ut_raw_input_offsets;
+  const char* dev_scifi_raw_input;
+  const uint32_t* dev_scifi_raw_input_offsets;
+  uint32_t* dev_number_of_selected_events;
+  uint32_t* dev_event_list;
+  size_t ut_events_size;
+  size_t scifi_events_size;
+  uint32_t n_events;
+  velo_tuning tuning;
+} velo_global_event_cut_args;
+int velo_global_event_cut_launch(const velo_global_event_cut_args* a, void* stream);
+
 /* ---- K1: estimate_input_size (EstimateInputSize.cu:3-255) ----
  * Per module: clusters of isolated SPs plus candidates of the others.
  * Candidates are listed per event in bank, SP, low-then-high-nibble order. */

@@ -1,4 +1,4 @@
-/* velo_mdf.h -- VP raw banks from MDF files into the VELO event layout.
+/* velo_mdf.h -- raw banks from MDF files into the per-type event layout.
  *
  * Replaces, for the VP banks, the reference's MDF input path:
  *   MDF::read_events   mdf/src/read_mdf.cpp:28-189  (bank selection, the
@@ -25,7 +25,12 @@
 extern "C" {
 #endif
 
+/* LHCb raw bank types (mdf/include/raw_bank.hpp:57-110) that the reference
+ * reads (mdf/include/read_mdf.hpp:20-24). */
+#define VELO_MDF_BANK_MUON 13u
 #define VELO_MDF_BANK_VP 63u
+#define VELO_MDF_BANK_FT_CLUSTER 64u
+#define VELO_MDF_BANK_UT 66u
 
 typedef struct {
   char* events;        /* malloc'ed; free with velo_mdf_free */
@@ -44,6 +49,14 @@ typedef struct {
  * -1 with a message in err (the earliest failing record's). */
 int velo_mdf_read_vp(const char* const* paths, uint32_t n_paths, uint32_t max_events, uint32_t start_event,
                      int verify_checksum, uint32_t n_threads, velo_mdf_events* out, char* err, size_t err_len);
+
+/* The same for several bank types in one pass over the files: outs[k]
+ * receives the events of bank_types[k] (each a distinct type below 256),
+ * in the layout above.  An event without banks of a type holds
+ * n_banks = 0 and one offset.  On an error every outs[k] is empty. */
+int velo_mdf_read_banks(const char* const* paths, uint32_t n_paths, uint32_t max_events, uint32_t start_event,
+                        int verify_checksum, uint32_t n_threads, const uint32_t* bank_types, uint32_t n_types,
+                        velo_mdf_events* outs, char* err, size_t err_len);
 
 void velo_mdf_free(velo_mdf_events* ev);
 

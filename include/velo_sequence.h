@@ -53,6 +53,23 @@ typedef struct {
   const void* dev_ghost_model;
   /* Also run velo_kalman_fit after consolidation (KalmanFilterVisitor.cu:4-44). */
   int run_velo_kalman_fit;
+  /* Run global_event_cut (GlobalEventCut.cu:3-55) on UT and SciFi banks in
+   * the raw-event layout (velo_mdf.h), from host memory (copied by
+   * init_event_list) or device memory (used in place when non-NULL); the
+   * VELO steps then run on the kept events only, in increasing order
+   * (output host_event_list / dev_event_list map each slot to its input
+   * event).  Plain runs only (batched = 0). */
+  int run_global_event_cut;
+  const char* host_ut_events;
+  const uint32_t* host_ut_offsets; /* n_events+1 */
+  size_t ut_events_size;
+  const char* host_scifi_events;
+  const uint32_t* host_scifi_offsets;
+  size_t scifi_events_size;
+  const char* dev_ut_events;
+  const uint32_t* dev_ut_offsets;
+  const char* dev_scifi_events;
+  const uint32_t* dev_scifi_offsets;
 } velo_sequence_input;
 
 typedef struct {
@@ -79,6 +96,10 @@ typedef struct {
   const uint32_t* dev_module_cluster_num;
   const uint32_t* dev_cluster_container;
   size_t memory_required; /* arena high-water mark, bytes */
+  /* input event of each output slot (n_events entries; identity without
+   * the global event cut) */
+  const uint32_t* host_event_list;
+  const uint32_t* dev_event_list;
 } velo_sequence_output;
 
 velo_sequence* velo_sequence_create(

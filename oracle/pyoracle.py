@@ -63,6 +63,8 @@ def lib():
         L.oracle_consolidate_velo_tracks.argtypes = [C.c_uint32] + [_P] * 7
         L.oracle_ccl_event.argtypes = [_P, _P, _P, _P, _P, _P, C.c_uint32]
         L.oracle_velo_kalman_fit.argtypes = [C.c_uint32] + [_P] * 5
+        L.oracle_global_event_cut.argtypes = [C.c_uint32] + [_P] * 5
+        L.oracle_global_event_cut.restype = C.c_uint32
         _F = C.c_float
         _PF = C.POINTER(C.c_float)
         L.oracle_kalman_step.argtypes = [_F, _F, _F, _F, _PF, _PF, _PF, _PF, _PF]
@@ -191,6 +193,18 @@ def consolidate(n, atomics, tracks, hit_number, container, starts):
     lib().oracle_consolidate_velo_tracks(n, _ptr(atomics), _ptr(tracks), _ptr(hit_number), _ptr(container),
                                          _ptr(starts), _ptr(hits), _ptr(states))
     return hits, states
+
+
+def global_event_cut(ut_events, ut_offsets, scifi_events, scifi_offsets):
+    """Kept events (increasing) of the global event cut."""
+    n = len(ut_offsets) - 1
+    assert len(scifi_offsets) - 1 == n
+    ut = np.ascontiguousarray(ut_events, np.uint8)
+    sf = np.ascontiguousarray(scifi_events, np.uint8)
+    lst = np.zeros(max(n, 1), np.uint32)
+    k = lib().oracle_global_event_cut(n, _ptr(ut), _ptr(np.ascontiguousarray(ut_offsets, np.uint32)), _ptr(sf),
+                                      _ptr(np.ascontiguousarray(scifi_offsets, np.uint32)), _ptr(lst))
+    return lst[:k].copy()
 
 
 def velo_kalman_fit(n, atomics, hit_number, track_hits, states):

@@ -1432,3 +1432,40 @@ void oracle_pipeline_free(or_pipeline_result* r)
   free(r->velo_states);
   memset(r, 0, sizeof(*r));
 }
+
+/* GlobalEventCut.cu:3-55, one event after the other.  The SciFi estimate
+ * is the summed bank bytes without the 4-byte header word, halved, minus 2,
+ * all in 32-bit unsigned arithmetic (an event with under 4 bytes of SciFi
+ * data wraps and is cut).  UT banks are added in bank order, one valid
+ * order of the reference's atomics, with its per-bank early exit. */
+uint32_t oracle_global_event_cut(
+  uint32_t n_events, const char* ut_raw, const uint32_t* ut_offsets, const char* scifi_raw,
+  const uint32_t* scifi_offsets, uint32_t* event_list)
+{
+  uint32_t selected = 0;
+  for (uint32_t e = 0; e < n_events; ++e) {
+    const uint32_t* sw = (const uint32_t*) (scifi_raw + scifi_offsets[e]);
+    const uint32_t scifi_banks = sw[0];
+    uint32_t n_scifi = 0;
+    for (uint32_t i = 0; i < scifi_banks; ++i) n_scifi += sw[1 + i + 1] - sw[1 + i] - 4u;
+    n_scifi = n_scifi / 2u - 2u;
+    if (n_scifi >= OR_GEC_MAX_SCIFI_UT_CLUSTERS) continue;
+
+    const uint32_t* uw = (const uint32_t*) (ut_raw + ut_offsets[e]);
+    const uint32_t ut_banks = uw[0];
+    const char* payload = (const char*) (uw + 2 + ut_banks);
+    uint32_t n_ut = 0;
+    int cut = 0;
+    for (uint32_t i = 0; i < ut_banks && !cut; ++i) {
+      uint32_t head;
+      memcpy(&head, payload + uw[1 + i] + 4, 4);
+      const uint32_t hits = head & 0xFFFFu;
+      const uint32_t before = n_ut;
+      n_ut += hits;
+      if (before + hits >= OR_GEC_MAX_SCIFI_UT_CLUSTERS) cut = 1;
+    }
+    if (cut || n_ut + n_scifi >= OR_GEC_MAX_SCIFI_UT_CLUSTERS) continue;
+    event_list[selected++] = e;
+  }
+  return selected;
+}

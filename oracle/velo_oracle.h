@@ -147,6 +147,17 @@ void oracle_velo_kalman_fit(
 float oracle_kalman_step(float z, float zhit, float xhit, float whit, float* x, float* tx, float* cov_xx,
                          float* cov_xtx, float* cov_txtx);
 
+/* ---- global_event_cut (GlobalEventCut.cu:3-55, GlobalEventCut.cuh:9)
+ *      UT and SciFi events in the raw-event layout (u32 n_banks, u32 byte
+ *      offsets[n_banks+1], per bank the source id and its data words).
+ *      Writes the kept events to event_list in increasing order (the
+ *      reference appends them in atomic-arrival order) and returns their
+ *      count. ---- */
+#define OR_GEC_MAX_SCIFI_UT_CLUSTERS 9750u
+uint32_t oracle_global_event_cut(
+  uint32_t n_events, const char* ut_raw, const uint32_t* ut_offsets, const char* scifi_raw,
+  const uint32_t* scifi_offsets, uint32_t* event_list);
+
 /* ---- textbook 8-connected component labelling (Clustering.cpp:3-280
  *      semantics): LHCbIDs of all clusters of one event, plus centroid
  *      sums so coordinates can be formed; used to pin a6 ---- */

@@ -193,3 +193,19 @@ def velo_kalman_fit(n, atomics, hit_number, track_hits, states, tuning=(0, 0)):
     nat.launch("velo_kalman_fit_launch", a)
     sync()
     return host(out)
+
+
+def global_event_cut(ut_events, ut_offsets, scifi_events, scifi_offsets, block=0):
+    """(kept events, count) of velo_global_event_cut_launch; the list buffer
+    starts as noise."""
+    n = len(ut_offsets) - 1
+    du, duo = dev(ut_events), dev(np.asarray(ut_offsets, np.uint32).view(np.int32))
+    ds, dso = dev(scifi_events), dev(np.asarray(scifi_offsets, np.uint32).view(np.int32))
+    lst, cnt = garbage_u32(n), garbage_u32(1, seed=2)
+    a = nat.GlobalEventCutArgs(du.data_ptr(), duo.data_ptr(), ds.data_ptr(), dso.data_ptr(), cnt.data_ptr(),
+                               lst.data_ptr(), int(np.asarray(ut_events).nbytes),
+                               int(np.asarray(scifi_events).nbytes), n, nat.Tuning(block, 0, 0, 0, None, None, None))
+    nat.launch("velo_global_event_cut_launch", a)
+    sync()
+    k = int(host(cnt)[0])
+    return host(lst).view(np.uint32)[:k].copy(), k

@@ -76,3 +76,58 @@ def test_trace_groups_by_launch_shape(tmp_path):
     assert rows[0] == {"kernel": "a_kernel", "workgroups": 10, "workgroup_size": 64, "dispatches": 2,
                        "average_us": 3.0, "total_us": 6.0}
     assert rows[1]["workgroups"] == 2 and rows[1]["average_us"] == 1.0
+
+
+AGENT_COLS = ["Node_Id", "Agent_Type", "Cu_Count", "Simd_Count", "Num_Xcc", "Simd_Per_Cu", "Max_Waves_Per_Simd",
+              "Lds_Size_In_Kb", "Max_Engine_Clk_Fcompute", "Name"]
+
+
+def write_agents(path):
+    with open(path, "w", newline="") as f:
+        w = csv.DictWriter(f, AGENT_COLS)
+        w.writeheader()
+        w.writerow({"Node_Id": 0, "Agent_Type": "CPU", "Cu_Count": 16, "Simd_Count": 0, "Num_Xcc": 1,
+                    "Simd_Per_Cu": 0, "Max_Waves_Per_Simd": 0, "Lds_Size_In_Kb": 0, "Max_Engine_Clk_Fcompute": 0,
+                    "Name": "cpu"})
+        w.writerow({"Node_Id": 1, "Agent_Type": "GPU", "Cu_Count": 256, "Simd_Count": 1024, "Num_Xcc": 8,
+                    "Simd_Per_Cu": 4, "Max_Waves_Per_Simd": 8, "Lds_Size_In_Kb": 160,
+                    "Max_Engine_Clk_Fcompute": 2400, "Name": "gfx950"})
+
+
+def test_theoretical_occupancy_limits():
+    agent = {"Simd_Per_Cu": 4, "Max_Waves_Per_Simd": 8, "Lds_Size_In_Kb": 160}
+    # one-wave workgroups with 11 KiB of LDS: 14 fit in 160 KiB
+    assert pf.theoretical_waves_per_cu(64, 64, 0, 11 * 1024, agent) == {"waves": 14, "max_waves": 32, "limit": "lds"}
+    # 256 registers (VGPR + AGPR) leave two waves per SIMD
+    assert pf.theoretical_waves_per_cu(256, 200, 56, 0, agent)["waves"] == 8
+    assert pf.theoretical_waves_per_cu(256, 200, 56, 0, agent)["limit"] == "registers"
+    # small kernels fill the wave slots
+    assert pf.theoretical_waves_per_cu(1024, 32, 0, 0, agent) == {"waves": 32, "max_waves": 32,
+                                                                 "limit": "wave slots"}
+
+
+def test_lds_occupancy(tmp_path):
+    write_agents(tmp_path / "run_agent_info.csv")
+    rows = []
+    k = "velo::search_by_triplet_kernel(a)"
+    # 1 ms at 2.4 GHz = 2.4e6 cycles per XCD, summed over 8 XCDs in the row
+    for d in (1, 2):
+        for c, v in (("SQ_INSTS_LDS_LOAD_BANDWIDTH", 3e9 / 64), ("SQ_INSTS_LDS_STORE_BANDWIDTH", 1e9 / 64),
+                     ("SQ_LDS_BANK_CONFLICT", 0.05 * 2.4e6 * 256), ("SQ_WAVE_CYCLES", 2.4e6 * 256 * 16 / 4),
+                     ("GRBM_GUI_ACTIVE", 8 * 2.4e6)):
+            rows.append({"Dispatch_Id": d, "Kernel_Name": k, "Counter_Name": c, "Counter_Value": v,
+                         "Start_Timestamp": 0, "End_Timestamp": 1_000_000, "Workgroup_Size": 64,
+                         "VGPR_Count": 64, "LDS_Block_Size": 11 * 1024})
+    with open(tmp_path / "run_counter_collection.csv", "w", newline="") as f:
+        w = csv.DictWriter(f, COUNTER_COLS)
+        w.writeheader()
+        for r in rows:
+            w.writerow({c: 0 for c in COUNTER_COLS} | r)
+    t = pf.lds_occupancy(tmp_path)
+    s = t["kernels"]["search_by_triplet_kernel"]
+    assert s["dispatches"] == 2 and s["duration_us"] == pytest.approx(1000.0)
+    assert s["lds_bytes"] == pytest.approx(4e9) and s["lds_TBps"] == pytest.approx(4.0)
+    assert s["lds_bank_conflict_pct"] == pytest.approx(5.0)
+    assert s["waves_per_cu"] == pytest.approx(16.0) and s["occupancy_pct"] == pytest.approx(50.0)
+    assert s["theoretical_waves_per_cu"] == 14 and s["theoretical_limit"] == "lds"
+    assert "search_by_triplet_kernel" in pf.lds_markdown(t, "t", "c")
