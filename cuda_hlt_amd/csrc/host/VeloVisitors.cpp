@@ -501,11 +501,12 @@ void SequenceVisitor::visit<prefix_sum_velo_track_hit_number_t>(
   state.set_size(t);
   state.set_opts(stream);
   state.set_arguments(arguments.offset<dev_velo_track_hit_number>(), arguments.offset<dev_prefix_sum_auxiliary_array_2>());
-  if (runtime_options.batched) {
-    // t is the capacity; the track total sits at dev_atomics_velo[2N]
-    const unsigned n = host_buffers.host_number_of_selected_events[0];
-    state.set_device_count(reinterpret_cast<const uint32_t*>(arguments.offset<dev_atomics_velo>()) + 2 * n);
-  }
+  // batched: t is the capacity and the track total sits at dev_atomics_velo[2N];
+  // plain: t is the total (the handler outlives the run, so always set)
+  const unsigned n = host_buffers.host_number_of_selected_events[0];
+  state.set_device_count(
+    runtime_options.batched ? reinterpret_cast<const uint32_t*>(arguments.offset<dev_atomics_velo>()) + 2 * n
+                            : nullptr);
   state.invoke();
   if (!runtime_options.batched)
     read_total(host_buffers.host_accumulated_number_of_hits_in_velo_tracks,

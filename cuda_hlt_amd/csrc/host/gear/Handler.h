@@ -44,7 +44,7 @@ struct Handler {
   void set_tuning(const velo_tuning& t)
   {
     if (t.block_dim) args.tuning.block_dim = t.block_dim;
-    if (t.lds_tile) args.tuning.lds_tile = t.lds_tile;
+    args.tuning.lds_tile = t.lds_tile; // 0: the launcher's default (nothing kept from an earlier run)
     args.tuning.grid_dim = t.grid_dim;
     args.tuning.dev_work_counter = t.dev_work_counter;
     args.tuning.dev_work_order = t.dev_work_order;

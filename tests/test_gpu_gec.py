@@ -19,7 +19,7 @@ def _banks(n, seed, kind="minbias"):
 
 @pytest.mark.parametrize("n,block", [(1, 0), (700, 64), (3000, 256), (5000, 1024)])
 def test_gec_kernel_matches_oracle(gpu, n, block):
-    import gpu_stages as gs
+    from tests import gpu_stages as gs
     print(f"[gec] n={n} block={block}", flush=True)
     _, g = _banks(n, seed=n)
     ref = po.global_event_cut(g.ut_events, g.ut_offsets, g.scifi_events, g.scifi_offsets)
@@ -30,8 +30,8 @@ def test_gec_kernel_matches_oracle(gpu, n, block):
 
 def test_gec_kernel_boundaries(gpu):
     """The hand-built boundary events of the oracle tests, in one batch."""
-    import gpu_stages as gs
-    import test_gec_oracle as t
+    from tests import gpu_stages as gs
+    from tests import test_gec_oracle as t
     cases = [(9748, []), (9748, [1]), (9748, [2]), (9750, []), (0, [9749]), (0, [9750]), (0, [5000, 4749]),
              (0, [5000, 4750]), (4000, [3000, 2749]), (4000, [3000, 2750])]
     ut = [t.ut_event(u) for _, u in cases] + [t.ut_event([0]), t.ut_event([9749], high=0xFFFF)]
@@ -45,7 +45,7 @@ def test_gec_kernel_boundaries(gpu):
 
 
 def test_gec_kernel_no_events(gpu):
-    import gpu_stages as gs
+    from tests import gpu_stages as gs
     z = np.zeros(8, np.uint8)
     got, k = gs.global_event_cut(z, np.zeros(1, np.uint32), z, np.zeros(1, np.uint32))
     assert k == 0 and len(got) == 0
