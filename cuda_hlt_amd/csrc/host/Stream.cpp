@@ -35,7 +35,7 @@ bool Stream::GraphKey::operator==(const GraphKey& o) const
 {
   return dev_events == o.dev_events && dev_offsets == o.dev_offsets && dev_work_order == o.dev_work_order &&
          events_size == o.events_size && n_events == o.n_events && dev_ghost_model == o.dev_ghost_model &&
-         run_velo_kalman_fit == o.run_velo_kalman_fit &&
+         run_velo_kalman_fit == o.run_velo_kalman_fit && run_pv_beamline == o.run_pv_beamline &&
          std::memcmp(&tuning, &o.tuning, sizeof(tuning)) == 0;
 }
 
@@ -117,6 +117,7 @@ hipError_t Stream::run_batched(const RuntimeOptions& ro)
   key.dev_work_order = ro.dev_work_order;
   key.dev_ghost_model = ro.dev_ghost_model;
   key.run_velo_kalman_fit = ro.run_velo_kalman_fit;
+  key.run_pv_beamline = ro.run_pv_beamline;
   key.events_size = ro.host_velopix_events_size;
   key.n_events = ro.number_of_events;
   key.tuning = ro.tuning;
@@ -163,6 +164,11 @@ hipError_t Stream::run_batched(const RuntimeOptions& ro)
     if (ro.run_velo_kalman_fit) {
       host_buffers.ensure_kalman(t);
       d2h(host_buffers.host_kalmanvelo_states, output<dev_velo_kalman_beamline_states>(), t * VELO_STATE_BYTES);
+    }
+    if (ro.run_pv_beamline) {
+      d2h(host_buffers.host_reconstructed_multi_pvs, output<dev_multi_fit_vertices>(),
+          (size_t) n * VELO_PV_MAX_VERTICES * sizeof(velo_pv_vertex));
+      d2h(host_buffers.host_number_of_multivertex, output<dev_number_of_multi_fit_vertices>(), n * sizeof(unsigned));
     }
     wait();
   }

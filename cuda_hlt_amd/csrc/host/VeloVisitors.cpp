@@ -607,3 +607,140 @@ void SequenceVisitor::visit<velo_kalman_fit_t>(
   hipCheck(hipMemcpyAsync(host_buffers.host_kalmanvelo_states, arguments.offset<dev_velo_kalman_beamline_states>(),
                           t * VELO_STATE_BYTES, hipMemcpyDeviceToHost, stream));
 }
+
+// ---- pv_beamline_extrapolate_t (ExtrapolateVisitor.cu:5-33) --------------------
+template<>
+void SequenceVisitor::set_arguments_size<pv_beamline_extrapolate_t>(
+  pv_beamline_extrapolate_t::arguments_t arguments,
+  const RuntimeOptions& runtime_options,
+  const Constants&,
+  const HostBuffers& host_buffers)
+{
+  const size_t t = runtime_options.run_pv_beamline ? host_buffers.host_number_of_reconstructed_velo_tracks[0] : 0;
+  arguments.set_size<dev_pvtracks>(t);
+}
+
+template<>
+void SequenceVisitor::visit<pv_beamline_extrapolate_t>(
+  pv_beamline_extrapolate_t& state,
+  const pv_beamline_extrapolate_t::arguments_t& arguments,
+  const RuntimeOptions& runtime_options,
+  const Constants&,
+  HostBuffers& host_buffers,
+  hipStream_t& stream,
+  hipEvent_t&)
+{
+  if (!runtime_options.run_pv_beamline) return;
+  state.set_opts(dim3(host_buffers.host_number_of_selected_events[0]), dim3(128), stream);
+  state.set_arguments(
+    arguments.offset<dev_velo_kalman_beamline_states>(),
+    arguments.offset<dev_atomics_velo>(),
+    arguments.offset<dev_velo_track_hit_number>(),
+    arguments.offset<dev_pvtracks>());
+  state.invoke();
+}
+
+// ---- pv_beamline_histo_t (HistoVisitor.cu:5-33) ---------------------------------
+template<>
+void SequenceVisitor::set_arguments_size<pv_beamline_histo_t>(
+  pv_beamline_histo_t::arguments_t arguments,
+  const RuntimeOptions& runtime_options,
+  const Constants&,
+  const HostBuffers& host_buffers)
+{
+  const size_t n = runtime_options.run_pv_beamline ? host_buffers.host_number_of_selected_events[0] : 0;
+  arguments.set_size<dev_zhisto>(n * VELO_PV_NBINS);
+}
+
+template<>
+void SequenceVisitor::visit<pv_beamline_histo_t>(
+  pv_beamline_histo_t& state,
+  const pv_beamline_histo_t::arguments_t& arguments,
+  const RuntimeOptions& runtime_options,
+  const Constants&,
+  HostBuffers& host_buffers,
+  hipStream_t& stream,
+  hipEvent_t&)
+{
+  if (!runtime_options.run_pv_beamline) return;
+  state.set_opts(dim3(host_buffers.host_number_of_selected_events[0]), dim3(64), stream);
+  state.set_arguments(
+    arguments.offset<dev_atomics_velo>(),
+    arguments.offset<dev_velo_track_hit_number>(),
+    arguments.offset<dev_pvtracks>(),
+    arguments.offset<dev_zhisto>());
+  state.invoke();
+}
+
+// ---- pv_beamline_peak_t (PeakVisitor.cu:5-40) -----------------------------------
+template<>
+void SequenceVisitor::set_arguments_size<pv_beamline_peak_t>(
+  pv_beamline_peak_t::arguments_t arguments,
+  const RuntimeOptions& runtime_options,
+  const Constants&,
+  const HostBuffers& host_buffers)
+{
+  const size_t n = runtime_options.run_pv_beamline ? host_buffers.host_number_of_selected_events[0] : 0;
+  arguments.set_size<dev_zpeaks>(n * VELO_PV_MAX_VERTICES);
+  arguments.set_size<dev_number_of_zpeaks>(n);
+}
+
+template<>
+void SequenceVisitor::visit<pv_beamline_peak_t>(
+  pv_beamline_peak_t& state,
+  const pv_beamline_peak_t::arguments_t& arguments,
+  const RuntimeOptions& runtime_options,
+  const Constants&,
+  HostBuffers& host_buffers,
+  hipStream_t& stream,
+  hipEvent_t&)
+{
+  if (!runtime_options.run_pv_beamline) return;
+  state.set_opts(dim3(host_buffers.host_number_of_selected_events[0]), dim3(64), stream);
+  state.set_arguments(
+    arguments.offset<dev_zhisto>(), arguments.offset<dev_zpeaks>(), arguments.offset<dev_number_of_zpeaks>());
+  state.invoke();
+}
+
+// ---- pv_beamline_multi_fitter_t (MultiFitterVisitor.cu:5-60) ---------------------
+template<>
+void SequenceVisitor::set_arguments_size<pv_beamline_multi_fitter_t>(
+  pv_beamline_multi_fitter_t::arguments_t arguments,
+  const RuntimeOptions& runtime_options,
+  const Constants&,
+  const HostBuffers& host_buffers)
+{
+  const size_t n = runtime_options.run_pv_beamline ? host_buffers.host_number_of_selected_events[0] : 0;
+  arguments.set_size<dev_multi_fit_vertices>(n * VELO_PV_MAX_VERTICES);
+  arguments.set_size<dev_number_of_multi_fit_vertices>(n);
+}
+
+template<>
+void SequenceVisitor::visit<pv_beamline_multi_fitter_t>(
+  pv_beamline_multi_fitter_t& state,
+  const pv_beamline_multi_fitter_t::arguments_t& arguments,
+  const RuntimeOptions& runtime_options,
+  const Constants&,
+  HostBuffers& host_buffers,
+  hipStream_t& stream,
+  hipEvent_t&)
+{
+  if (!runtime_options.run_pv_beamline) return;
+  const unsigned n = host_buffers.host_number_of_selected_events[0];
+  // the kernel writes every event's vertex count: no memset first
+  state.set_opts(dim3(n), dim3(64), stream);
+  state.set_arguments(
+    arguments.offset<dev_atomics_velo>(),
+    arguments.offset<dev_velo_track_hit_number>(),
+    arguments.offset<dev_pvtracks>(),
+    arguments.offset<dev_zpeaks>(),
+    arguments.offset<dev_number_of_zpeaks>(),
+    arguments.offset<dev_multi_fit_vertices>(),
+    arguments.offset<dev_number_of_multi_fit_vertices>());
+  state.invoke();
+  if (!runtime_options.copy_outputs_to_host || runtime_options.batched) return;
+  hipCheck(hipMemcpyAsync(host_buffers.host_reconstructed_multi_pvs, arguments.offset<dev_multi_fit_vertices>(),
+                          (size_t) n * VELO_PV_MAX_VERTICES * sizeof(velo_pv_vertex), hipMemcpyDeviceToHost, stream));
+  hipCheck(hipMemcpyAsync(host_buffers.host_number_of_multivertex, arguments.offset<dev_number_of_multi_fit_vertices>(),
+                          (size_t) n * sizeof(unsigned), hipMemcpyDeviceToHost, stream));
+}

@@ -81,7 +81,8 @@ int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, char* err
   ro.batched = in->batched != 0;
   ro.dev_work_order = in->dev_work_order;
   ro.dev_ghost_model = in->dev_ghost_model;
-  ro.run_velo_kalman_fit = in->run_velo_kalman_fit != 0;
+  ro.run_pv_beamline = in->run_pv_beamline != 0;
+  ro.run_velo_kalman_fit = in->run_velo_kalman_fit != 0 || ro.run_pv_beamline;
   ro.run_global_event_cut = in->run_global_event_cut != 0;
   if (ro.run_global_event_cut) {
     const bool ut_ok = in->dev_ut_events ? in->dev_ut_offsets != nullptr : in->host_ut_events && in->host_ut_offsets;
@@ -146,7 +147,25 @@ int velo_sequence_output_get(const velo_sequence* s, velo_sequence_output* out)
   out->memory_required = st.scheduler.memory_manager.total_memory_required;
   out->host_event_list = hb.host_event_list;
   out->dev_event_list = st.output<dev_event_list>();
+  if (s->last.run_pv_beamline) {
+    if (s->last.copy_outputs_to_host) {
+      out->host_pv_vertices = reinterpret_cast<const velo_pv_vertex*>(hb.host_reconstructed_multi_pvs);
+      out->host_pv_counts = hb.host_number_of_multivertex;
+    }
+    out->dev_pv_vertices = st.output<dev_multi_fit_vertices>();
+    out->dev_pv_counts = st.output<dev_number_of_multi_fit_vertices>();
+  }
   return 0;
+}
+
+size_t velo_sequence_struct_size(int which)
+{
+  switch (which) {
+  case 0: return sizeof(velo_sequence_config);
+  case 1: return sizeof(velo_sequence_input);
+  case 2: return sizeof(velo_sequence_output);
+  default: return 0;
+  }
 }
 
 void* velo_sequence_stream(const velo_sequence* s) { return s && s->stream ? (void*) s->stream->stream : nullptr; }

@@ -6,6 +6,7 @@
 
 #include "../gear/Argument.h"
 #include "VeloEventModel.h"
+#include "velo_pv.h"
 
 using uint = unsigned int;
 
@@ -40,5 +41,12 @@ ARGUMENT(dev_prefix_sum_auxiliary_array_2, uint)
 ARGUMENT(dev_velo_track_hits, char)
 ARGUMENT(dev_velo_states, char)
 ARGUMENT(dev_velo_kalman_beamline_states, char)
+// ArgumentsPV.cuh:11-16
+ARGUMENT(dev_pvtracks, velo_pv_track)
+ARGUMENT(dev_zhisto, float)
+ARGUMENT(dev_zpeaks, float)
+ARGUMENT(dev_number_of_zpeaks, uint)
+ARGUMENT(dev_multi_fit_vertices, velo_pv_vertex)
+ARGUMENT(dev_number_of_multi_fit_vertices, uint)
 // Per-event capacity-overflow bits (no reference counterpart).
 ARGUMENT(dev_velo_event_status, uint)

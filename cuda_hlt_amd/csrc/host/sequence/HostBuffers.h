@@ -7,6 +7,7 @@
 #include <cstdint>
 
 #include "../gear/Common.h"
+#include "velo_pv.h"
 
 struct HostBuffers {
   unsigned* host_number_of_selected_events = nullptr;
@@ -23,6 +24,8 @@ struct HostBuffers {
   char* host_velo_states = nullptr;
   uint8_t* host_accepted_velo_tracks = nullptr;
   char* host_kalmanvelo_states = nullptr;
+  char* host_reconstructed_multi_pvs = nullptr;   // max_events * 32 vertices
+  unsigned* host_number_of_multivertex = nullptr;  // max_events
   size_t cap_track_hit_number = 0, cap_track_hits = 0, cap_states = 0, cap_accepted = 0, cap_kalman = 0;
 
   void reserve(unsigned max_number_of_events)
@@ -38,6 +41,11 @@ struct HostBuffers {
       (void**) &host_atomics_velo, (2 * (size_t) max_number_of_events + 1) * sizeof(unsigned), hipHostMallocDefault));
     hipCheck(
       hipHostMalloc((void**) &host_velo_event_status, max_number_of_events * sizeof(unsigned) + 4, hipHostMallocDefault));
+    hipCheck(hipHostMalloc((void**) &host_reconstructed_multi_pvs,
+                           (size_t) max_number_of_events * VELO_PV_MAX_VERTICES * sizeof(velo_pv_vertex) + 64,
+                           hipHostMallocDefault));
+    hipCheck(hipHostMalloc(
+      (void**) &host_number_of_multivertex, max_number_of_events * sizeof(unsigned) + 4, hipHostMallocDefault));
   }
 
   static void grow(char** p, size_t* cap, size_t need)
@@ -65,7 +73,8 @@ struct HostBuffers {
   {
     for (void* p : {(void*) host_number_of_selected_events, (void*) host_event_list, (void*) host_atomics_velo,
                     (void*) host_velo_event_status, (void*) host_velo_track_hit_number, (void*) host_velo_track_hits,
-                    (void*) host_velo_states, (void*) host_accepted_velo_tracks, (void*) host_kalmanvelo_states})
+                    (void*) host_velo_states, (void*) host_accepted_velo_tracks, (void*) host_kalmanvelo_states,
+                    (void*) host_reconstructed_multi_pvs, (void*) host_number_of_multivertex})
       if (p) (void) hipHostFree(p);
     host_number_of_selected_events = nullptr;
     host_event_list = nullptr;
@@ -76,6 +85,8 @@ struct HostBuffers {
     host_velo_states = nullptr;
     host_accepted_velo_tracks = nullptr;
     host_kalmanvelo_states = nullptr;
+    host_reconstructed_multi_pvs = nullptr;
+    host_number_of_multivertex = nullptr;
     cap_track_hit_number = cap_track_hits = cap_states = cap_accepted = cap_kalman = 0;
   }
 };

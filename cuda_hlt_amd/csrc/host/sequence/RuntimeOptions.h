@@ -57,6 +57,9 @@ struct RuntimeOptions {
   const unsigned* dev_scifi_event_offsets = nullptr;
   // Run velo_kalman_fit after consolidation (KalmanFilterVisitor.cu:4-44).
   bool run_velo_kalman_fit = false;
+  // Run the four pv_beamline steps after it (DefaultSequence.h:19-23);
+  // implies run_velo_kalman_fit.
+  bool run_pv_beamline = false;
   // Ghost filter run inline by consolidation (a device velo_ghost_model),
   // NULL = every track accepted.
   const void* dev_ghost_model = nullptr;

@@ -59,3 +59,7 @@ DECLARE_VISITOR(copy_velo_track_hit_number_t)
 DECLARE_VISITOR(prefix_sum_velo_track_hit_number_t)
 DECLARE_VISITOR(consolidate_velo_tracks_t)
 DECLARE_VISITOR(velo_kalman_fit_t)
+DECLARE_VISITOR(pv_beamline_extrapolate_t)
+DECLARE_VISITOR(pv_beamline_histo_t)
+DECLARE_VISITOR(pv_beamline_peak_t)
+DECLARE_VISITOR(pv_beamline_multi_fitter_t)

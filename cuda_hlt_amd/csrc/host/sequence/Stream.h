@@ -37,6 +37,7 @@ struct Stream {
     const unsigned* dev_work_order = nullptr;
     const void* dev_ghost_model = nullptr;
     bool run_velo_kalman_fit = false;
+    bool run_pv_beamline = false;
     size_t events_size = 0;
     unsigned n_events = 0;
     VeloTuningOptions tuning {};

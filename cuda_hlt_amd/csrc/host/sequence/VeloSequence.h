@@ -10,6 +10,7 @@
 #include "../gear/ArgumentManager.h"
 #include "../gear/Handler.h"
 #include "ArgumentsVelo.h"
+#include "velo_pv.h"
 
 ALGORITHM(
   velo_init_event_list_launch,
@@ -153,6 +154,31 @@ ALGORITHM(
     dev_velo_states,
     dev_velo_kalman_beamline_states))
 
+// pv_beamline_*.cuh (cuda/PV/beamlinePV/include)
+ALGORITHM(
+  velo_pv_beamline_extrapolate_launch,
+  pv_beamline_extrapolate_t,
+  ARGUMENTS(dev_velo_kalman_beamline_states, dev_atomics_velo, dev_velo_track_hit_number, dev_pvtracks))
+
+ALGORITHM(
+  velo_pv_beamline_histo_launch,
+  pv_beamline_histo_t,
+  ARGUMENTS(dev_atomics_velo, dev_velo_track_hit_number, dev_pvtracks, dev_zhisto))
+
+ALGORITHM(velo_pv_beamline_peak_launch, pv_beamline_peak_t, ARGUMENTS(dev_zhisto, dev_zpeaks, dev_number_of_zpeaks))
+
+ALGORITHM(
+  velo_pv_beamline_multi_fitter_launch,
+  pv_beamline_multi_fitter_t,
+  ARGUMENTS(
+    dev_atomics_velo,
+    dev_velo_track_hit_number,
+    dev_pvtracks,
+    dev_zpeaks,
+    dev_number_of_zpeaks,
+    dev_multi_fit_vertices,
+    dev_number_of_multi_fit_vertices))
+
 #define SEQUENCE_T(...) typedef std::tuple<__VA_ARGS__> configured_sequence_t;
 
 SEQUENCE_T(
@@ -169,7 +195,11 @@ SEQUENCE_T(
   copy_velo_track_hit_number_t,
   prefix_sum_velo_track_hit_number_t,
   consolidate_velo_tracks_t,
-  velo_kalman_fit_t)
+  velo_kalman_fit_t,
+  pv_beamline_extrapolate_t,
+  pv_beamline_histo_t,
+  pv_beamline_peak_t,
+  pv_beamline_multi_fitter_t)
 
 // Kept allocated from first use to the end of the sequence
 // (OutputArguments.cuh:16-34, VELO part).
@@ -180,6 +210,8 @@ typedef std::tuple<
   dev_velo_states,
   dev_accepted_velo_tracks,
   dev_velo_kalman_beamline_states,
+  dev_multi_fit_vertices,
+  dev_number_of_multi_fit_vertices,
   dev_event_list,
   dev_number_of_selected_events,
   dev_velo_event_status>

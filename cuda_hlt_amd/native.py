@@ -106,6 +106,14 @@ GlobalEventCutArgs = type("GlobalEventCutArgs", (C.Structure,), {"_fields_": [
     ("dev_scifi_raw_input_offsets", P), ("dev_number_of_selected_events", P), ("dev_event_list", P),
     ("ut_events_size", C.c_size_t), ("scifi_events_size", C.c_size_t), ("n_events", U32), ("tuning", Tuning)]})
 
+PvExtrapolateArgs = _args("PvExtrapolateArgs", [
+    "dev_velo_kalman_beamline_states", "dev_atomics_velo", "dev_velo_track_hit_number", "dev_pvtracks"])
+PvHistoArgs = _args("PvHistoArgs", ["dev_atomics_velo", "dev_velo_track_hit_number", "dev_pvtracks", "dev_zhisto"])
+PvPeakArgs = _args("PvPeakArgs", ["dev_zhisto", "dev_zpeaks", "dev_number_of_zpeaks"])
+PvMultiFitterArgs = _args("PvMultiFitterArgs", [
+    "dev_atomics_velo", "dev_velo_track_hit_number", "dev_pvtracks", "dev_zpeaks", "dev_number_of_zpeaks",
+    "dev_multi_fit_vertices", "dev_number_of_multi_fit_vertices"])
+
 LAUNCHERS = {
     "velo_init_event_list_launch": InitEventListArgs,
     "velo_global_event_cut_launch": GlobalEventCutArgs,
@@ -120,6 +128,10 @@ LAUNCHERS = {
     "velo_copy_velo_track_hit_number_launch": CopyTrackHitNumberArgs,
     "velo_consolidate_velo_tracks_launch": ConsolidateArgs,
     "velo_kalman_fit_launch": KalmanFitArgs,
+    "velo_pv_beamline_extrapolate_launch": PvExtrapolateArgs,
+    "velo_pv_beamline_histo_launch": PvHistoArgs,
+    "velo_pv_beamline_peak_launch": PvPeakArgs,
+    "velo_pv_beamline_multi_fitter_launch": PvMultiFitterArgs,
 }
 
 _lib = None
@@ -151,6 +163,8 @@ def lib():
         L.velo_constants_free.restype = C.c_int
         L.velo_sp_tables.argtypes = [P, P, P, P]
         L.velo_hlt_version.restype = C.c_char_p
+        L.velo_pv_exp.argtypes = [C.c_float]
+        L.velo_pv_exp.restype = C.c_float
         L.velo_error_string.argtypes = [C.c_int]
         L.velo_error_string.restype = C.c_char_p
         _lib = L

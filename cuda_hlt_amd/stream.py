@@ -24,7 +24,7 @@ class SequenceInput(C.Structure):
                 ("run_global_event_cut", C.c_int), ("host_ut_events", P), ("host_ut_offsets", P),
                 ("ut_events_size", C.c_size_t), ("host_scifi_events", P), ("host_scifi_offsets", P),
                 ("scifi_events_size", C.c_size_t), ("dev_ut_events", P), ("dev_ut_offsets", P),
-                ("dev_scifi_events", P), ("dev_scifi_offsets", P)]
+                ("dev_scifi_events", P), ("dev_scifi_offsets", P), ("run_pv_beamline", C.c_int)]
 
 
 class SequenceOutput(C.Structure):
@@ -35,8 +35,13 @@ class SequenceOutput(C.Structure):
                 ("dev_track_hits", P), ("dev_states", P), ("dev_accepted", P), ("dev_kalman_states", P),
                 ("dev_event_status", P),
                 ("dev_module_cluster_start", P), ("dev_module_cluster_num", P), ("dev_cluster_container", P),
-                ("memory_required", C.c_size_t), ("host_event_list", P), ("dev_event_list", P)]
+                ("memory_required", C.c_size_t), ("host_event_list", P), ("dev_event_list", P),
+                ("host_pv_vertices", P), ("host_pv_counts", P), ("dev_pv_vertices", P), ("dev_pv_counts", P)]
 
+
+PV_VERTEX_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("chi2", "<f4"), ("ndof", "<i4"),
+                            ("n_tracks", "<u4"), ("cov00", "<f4"), ("cov10", "<f4"), ("cov11", "<f4"),
+                            ("cov20", "<f4"), ("cov21", "<f4"), ("cov22", "<f4"), ("nTracks", "<i4")])
 
 TUNING_SLOTS = ("estimate_input_size", "masked_clustering", "calculate_phi_and_sort", "fill_candidates",
                 "search_by_triplet", "consolidate")
@@ -61,6 +66,12 @@ def _lib():
         L.velo_sequence_enable_step_timing.restype = C.c_int
         L.velo_sequence_step_times.argtypes = [P, C.POINTER(C.c_float), C.POINTER(C.c_char_p), C.c_int]
         L.velo_sequence_step_times.restype = C.c_int
+        L.velo_sequence_struct_size.argtypes = [C.c_int]
+        L.velo_sequence_struct_size.restype = C.c_size_t
+        for which, st in enumerate((SequenceConfig, SequenceInput, SequenceOutput)):
+            if L.velo_sequence_struct_size(which) != C.sizeof(st):
+                raise nat.VeloError(f"{nat.LIB_PATH}: {st.__name__} is {L.velo_sequence_struct_size(which)} bytes "
+                                    f"in the library, {C.sizeof(st)} here (rebuild it)")
         _bound = True
     return L
 
@@ -76,6 +87,8 @@ class HostTracks:
     accepted: np.ndarray      # uint8[T]: dev_accepted_velo_tracks
     kalman: np.ndarray | None = None  # uint8[49 T]: velo_kalman_fit states, when run
     event_list: np.ndarray | None = None  # uint32[N]: input event of each slot
+    pv_vertices: np.ndarray | None = None  # PV_VERTEX_DTYPE[N, 32], when the pv_beamline steps ran
+    pv_counts: np.ndarray | None = None    # uint32[N]
 
 
 class VeloSequence:
@@ -108,7 +121,8 @@ class VeloSequence:
 
     def run(self, n_events, host_events=None, host_offsets=None, dev_events=None, dev_offsets=None,
             copy_outputs=True, repetitions=1, tuning=None, events_size=0, batched=False,
-            dev_work_order=None, dev_ghost_model=None, velo_kalman_fit=False, gec=None) -> SequenceOutput:
+            dev_work_order=None, dev_ghost_model=None, velo_kalman_fit=False, gec=None,
+            pv_beamline=False) -> SequenceOutput:
         """Run the sequence on host input (copied by init_event_list) or on
         device-resident input (dev_events/dev_offsets; events_size = their
         bytes, needed by batched runs to bound the buffers).  batched=True
@@ -122,7 +136,8 @@ class VeloSequence:
         ut_offsets, scifi_events, scifi_offsets), e.g. a gen.GecBanks, or
         a dict of device pointers with the same keys plus ut_events_size and
         scifi_events_size; the output then covers the kept events, listed
-        in HostTracks.event_list."""
+        in HostTracks.event_list.  pv_beamline also runs velo_kalman_fit and
+        the four beamline PV steps (HostTracks.pv_vertices / pv_counts)."""
         inp = SequenceInput()
         if dev_events is not None:
             inp.dev_events, inp.dev_offsets = int(dev_events), int(dev_offsets)
@@ -141,6 +156,7 @@ class VeloSequence:
         inp.dev_work_order = None if dev_work_order is None else int(dev_work_order)
         inp.dev_ghost_model = None if dev_ghost_model is None else int(dev_ghost_model)
         inp.run_velo_kalman_fit = int(velo_kalman_fit)
+        inp.run_pv_beamline = int(pv_beamline)
         if gec is not None:
             inp.run_global_event_cut = 1
             if isinstance(gec, dict):
@@ -189,4 +205,7 @@ class VeloSequence:
                           arr(o.host_track_hits, 16 * Ht, np.uint8), arr(o.host_states, 49 * T, np.uint8),
                           arr(o.host_accepted, T, np.uint8),
                           arr(o.host_kalman_states, 49 * T, np.uint8) if o.host_kalman_states else None,
-                          arr(o.host_event_list, n, np.uint32))
+                          arr(o.host_event_list, n, np.uint32),
+                          arr(o.host_pv_vertices, 32 * n, PV_VERTEX_DTYPE).reshape(n, 32) if o.host_pv_vertices
+                          else None,
+                          arr(o.host_pv_counts, n, np.uint32) if o.host_pv_counts else None)

@@ -15,6 +15,7 @@
 #include <stdint.h>
 
 #include "velo_hlt.h"
+#include "velo_pv.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -70,6 +71,9 @@ typedef struct {
   const uint32_t* dev_ut_offsets;
   const char* dev_scifi_events;
   const uint32_t* dev_scifi_offsets;
+  /* Also run the four pv_beamline steps after velo_kalman_fit
+   * (DefaultSequence.h:19-23; velo_pv.h); implies run_velo_kalman_fit. */
+  int run_pv_beamline;
 } velo_sequence_input;
 
 typedef struct {
@@ -100,6 +104,12 @@ typedef struct {
    * the global event cut) */
   const uint32_t* host_event_list;
   const uint32_t* dev_event_list;
+  /* primary vertices when the pv_beamline steps ran: 32 velo_pv_vertex
+   * slots and a count per event */
+  const velo_pv_vertex* host_pv_vertices;
+  const uint32_t* host_pv_counts;
+  const velo_pv_vertex* dev_pv_vertices;
+  const uint32_t* dev_pv_counts;
 } velo_sequence_output;
 
 velo_sequence* velo_sequence_create(
@@ -108,6 +118,10 @@ velo_sequence* velo_sequence_create(
 void velo_sequence_destroy(velo_sequence* s);
 int velo_sequence_run(velo_sequence* s, const velo_sequence_input* in, char* err, size_t err_len);
 int velo_sequence_output_get(const velo_sequence* s, velo_sequence_output* out);
+/* sizeof of this header's structs as the library was built (0: config,
+ * 1: input, 2: output, else 0), so that a binding can refuse a library
+ * whose structs differ from its own. */
+size_t velo_sequence_struct_size(int which);
 void* velo_sequence_stream(const velo_sequence* s); /* the hipStream_t of the sequence */
 
 /* Per-step timing (no reference counterpart): when enabled, HIP events are

@@ -65,6 +65,14 @@ def lib():
         L.oracle_velo_kalman_fit.argtypes = [C.c_uint32] + [_P] * 5
         L.oracle_global_event_cut.argtypes = [C.c_uint32] + [_P] * 5
         L.oracle_global_event_cut.restype = C.c_uint32
+        L.oracle_pv_exp.argtypes = [C.c_float]
+        L.oracle_pv_exp.restype = C.c_float
+        L.oracle_pv_gauss_integral.argtypes = [C.c_float]
+        L.oracle_pv_gauss_integral.restype = C.c_float
+        L.oracle_pv_beamline_extrapolate.argtypes = [C.c_uint32, _P, _P, _P]
+        L.oracle_pv_beamline_histo.argtypes = [C.c_uint32, _P, _P, _P]
+        L.oracle_pv_beamline_peak.argtypes = [C.c_uint32, _P, _P, _P]
+        L.oracle_pv_beamline_multi_fitter.argtypes = [C.c_uint32] + [_P] * 6
         _F = C.c_float
         _PF = C.POINTER(C.c_float)
         L.oracle_kalman_step.argtypes = [_F, _F, _F, _F, _PF, _PF, _PF, _PF, _PF]
@@ -193,6 +201,66 @@ def consolidate(n, atomics, tracks, hit_number, container, starts):
     lib().oracle_consolidate_velo_tracks(n, _ptr(atomics), _ptr(tracks), _ptr(hit_number), _ptr(container),
                                          _ptr(starts), _ptr(hits), _ptr(states))
     return hits, states
+
+
+PV_NBINS = 2400
+PV_MAX_VERTICES = 32
+PV_VERTEX_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("chi2", "<f4"), ("ndof", "<i4"),
+                            ("n_tracks", "<u4"), ("cov00", "<f4"), ("cov10", "<f4"), ("cov11", "<f4"),
+                            ("cov20", "<f4"), ("cov21", "<f4"), ("cov22", "<f4"), ("nTracks", "<i4")])
+
+
+def pv_exp(x: float) -> float:
+    return float(lib().oracle_pv_exp(x))
+
+
+def pv_gauss_integral(x: float) -> float:
+    return float(lib().oracle_pv_gauss_integral(x))
+
+
+def pv_beamline_extrapolate(n, atomics, kalman_states):
+    """PV tracks (T x 7 float32: z, x, y, tx, ty, W_00, W_11)."""
+    atomics = np.ascontiguousarray(atomics, np.uint32)
+    T = int(atomics[2 * n])
+    out = np.zeros((max(T, 1), 7), np.float32)
+    lib().oracle_pv_beamline_extrapolate(n, _ptr(atomics), _ptr(np.ascontiguousarray(kalman_states, np.uint8)),
+                                         _ptr(out))
+    return out[:T]
+
+
+def pv_beamline_histo(n, atomics, pvtracks):
+    out = np.zeros((max(n, 1), PV_NBINS), np.float32)
+    lib().oracle_pv_beamline_histo(n, _ptr(np.ascontiguousarray(atomics, np.uint32)),
+                                   _ptr(np.ascontiguousarray(pvtracks, np.float32)), _ptr(out))
+    return out[:n]
+
+
+def pv_beamline_peak(n, zhisto):
+    """(zpeaks N x 32, counts N)."""
+    zp = np.zeros((max(n, 1), PV_MAX_VERTICES), np.float32)
+    cnt = np.zeros(max(n, 1), np.uint32)
+    lib().oracle_pv_beamline_peak(n, _ptr(np.ascontiguousarray(zhisto, np.float32)), _ptr(zp), _ptr(cnt))
+    return zp[:n], cnt[:n]
+
+
+def pv_beamline_multi_fitter(n, atomics, pvtracks, zpeaks, n_zpeaks):
+    """(vertices N x 32 PV_VERTEX_DTYPE, counts N)."""
+    v = np.zeros((max(n, 1), PV_MAX_VERTICES), PV_VERTEX_DTYPE)
+    cnt = np.zeros(max(n, 1), np.uint32)
+    lib().oracle_pv_beamline_multi_fitter(
+        n, _ptr(np.ascontiguousarray(atomics, np.uint32)), _ptr(np.ascontiguousarray(pvtracks, np.float32)),
+        _ptr(np.ascontiguousarray(zpeaks, np.float32)), _ptr(np.ascontiguousarray(n_zpeaks, np.uint32)), _ptr(v),
+        _ptr(cnt))
+    return v[:n], cnt[:n]
+
+
+def pv_beamline(n, atomics, kalman_states):
+    """The four steps: dict of pvtracks, zhisto, zpeaks, n_zpeaks, vertices, n_vertices."""
+    tr = pv_beamline_extrapolate(n, atomics, kalman_states)
+    h = pv_beamline_histo(n, atomics, tr)
+    zp, nz = pv_beamline_peak(n, h)
+    v, nv = pv_beamline_multi_fitter(n, atomics, tr, zp, nz)
+    return {"pvtracks": tr, "zhisto": h, "zpeaks": zp, "n_zpeaks": nz, "vertices": v, "n_vertices": nv}
 
 
 def global_event_cut(ut_events, ut_offsets, scifi_events, scifi_offsets):

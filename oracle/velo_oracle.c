@@ -1469,3 +1469,325 @@ uint32_t oracle_global_event_cut(
   }
   return selected;
 }
+
+/* ======================================================================
+ * Beamline primary vertices (cuda/PV/beamlinePV/src/*.cu)
+ * ====================================================================== */
+
+#define OR_PV_ZMIN (-300.f)
+#define OR_PV_ZMAX 300.f
+#define OR_PV_DZ 0.25f
+#define OR_PV_MAX_TRACK_Z0_ERR 1.5f
+#define OR_PV_MIN_DENSITY 1.0f
+#define OR_PV_MIN_DIP_DENSITY 2.0f
+#define OR_PV_MIN_TRACKS_IN_SEED 2.5f
+#define OR_PV_MAX_VERTEX_RHO2 0.09f
+#define OR_PV_MAX_FIT_ITER 2u
+#define OR_PV_MAX_DELTA_CHI2 9.f
+#define OR_PV_MIN_WEIGHT 0.3f
+#define OR_PV_MIN_TRACKS_PER_VERTEX 5u
+
+/* e^x by Cody-Waite reduction (x = k ln2 + r, ln2 split in a 16-bit high
+ * part and a low part), a degree-7 Taylor polynomial of e^r in Horner form
+ * and ldexp by k; the beamline fit's exponential. */
+float oracle_pv_exp(float x)
+{
+  if (!(x == x)) return x;
+  if (x > 88.72284f) return INFINITY;
+  if (x < -104.0f) return 0.0f;
+  const float k = rintf(x * 1.44269504f);
+  const float r = (x - k * 0.693145751953125f) - k * 1.42860677e-06f;
+  float p = 1.0f / 5040.0f;
+  p = 1.0f / 720.0f + r * p;
+  p = 1.0f / 120.0f + r * p;
+  p = 1.0f / 24.0f + r * p;
+  p = 1.0f / 6.0f + r * p;
+  p = 0.5f + r * p;
+  p = 1.0f + r * p;
+  p = 1.0f + r * p;
+  return ldexpf(p, (int) k);
+}
+
+/* pv_beamline_histo.cu:3-11 (polynomial order 2, a = sqrt(7)). */
+float oracle_pv_gauss_integral(float x)
+{
+  const float a = sqrtf(7.0f);
+  const float xi = x / a;
+  const float eta = 1.0f - xi * xi;
+  return 0.5f + xi * (0.5f + eta * (0.25f + eta * 0.1875f));
+}
+
+/* pv_beamline_extrapolate.cu:3-30 with the PVTrack constructor
+ * (TrackBeamLineVertexFinder.cuh:12-25): closest approach to (0, 0). */
+void oracle_pv_beamline_extrapolate(
+  uint32_t n_events, const uint32_t* atomics_velo, const char* kalman_states, float* pvtracks)
+{
+  const size_t T = atomics_velo[2 * n_events];
+  const float* S = (const float*) kalman_states;
+  for (size_t t = 0; t < T; ++t) {
+    const float x = S[t], y = S[T + t], tx = S[2 * T + t], ty = S[3 * T + t];
+    const float c00 = S[4 * T + t], c20 = S[5 * T + t], c22 = S[6 * T + t];
+    const float c11 = S[7 * T + t], c31 = S[8 * T + t], c33 = S[9 * T + t];
+    const float z = S[11 * T + t];
+    const float dz = (tx * (0.f - x) + ty * (0.f - y)) / (tx * tx + ty * ty);
+    float tmp_c00 = c00, tmp_c11 = c11;
+    const float dz2 = dz * dz;
+    tmp_c00 += dz2 * c22 + 2.f * dz * c20;
+    tmp_c11 += dz2 * c33 + 2.f * dz * c31;
+    float* p = pvtracks + 7 * t;
+    p[0] = z + dz;
+    p[1] = x + dz * tx;
+    p[2] = y + dz * ty;
+    p[3] = tx;
+    p[4] = ty;
+    p[5] = 1.f / tmp_c00;
+    p[6] = 1.f / tmp_c11;
+  }
+}
+
+/* pv_beamline_histo.cu:13-76, tracks in order. */
+void oracle_pv_beamline_histo(uint32_t n_events, const uint32_t* atomics_velo, const float* pvtracks, float* zhisto)
+{
+  const float a = sqrtf(7.0f);
+  for (uint32_t e = 0; e < n_events; ++e) {
+    float* h = zhisto + (size_t) e * OR_PV_NBINS;
+    for (uint32_t i = 0; i < OR_PV_NBINS; ++i) h[i] = 0.f;
+    const uint32_t n = atomics_velo[e], off = atomics_velo[n_events + e];
+    for (uint32_t k = 0; k < n; ++k) {
+      const float* trk = pvtracks + 7 * ((size_t) off + k);
+      const float z = trk[0], tx = trk[3], ty = trk[4], w00 = trk[5], w11 = trk[6];
+      if (!(OR_PV_ZMIN < z && z < OR_PV_ZMAX)) continue;
+      const float zbin = (z - OR_PV_ZMIN) / OR_PV_DZ;
+      const float zweight = tx * tx * w00 + ty * ty * w11;
+      const float zerr = 1.f / sqrtf(zweight);
+      if (!(zerr < OR_PV_MAX_TRACK_Z0_ERR)) continue;
+      const float halfwindow = a * zerr / OR_PV_DZ;
+      int minbin = (int) (zbin - halfwindow);
+      if (minbin < 1) minbin = 1;
+      int maxbin = (int) (zbin + halfwindow);
+      if (maxbin > (int) OR_PV_NBINS - 2) maxbin = (int) OR_PV_NBINS - 2;
+      if (maxbin < minbin) continue;
+      float integral = 0;
+      for (int i = minbin; i < maxbin; ++i) {
+        const float relz = (OR_PV_ZMIN + (i + 1) * OR_PV_DZ - z) / zerr;
+        const float thisintegral = oracle_pv_gauss_integral(relz);
+        h[i] += thisintegral - integral;
+        integral = thisintegral;
+      }
+      h[maxbin] += 1.f - integral;
+    }
+  }
+}
+
+/* pv_beamline_peak.cu:3-147 for one event; h[OR_PV_NBINS] reads as 0 (the
+ * reference reads the next event's first bin there, always empty).  A push
+ * beyond an array's size (200 edges, 32 extrema, 50 subclusters, 200
+ * clusters, 32 peaks) is dropped. */
+static uint32_t oracle_pv_peaks_event(const float* h, float* zpeaks)
+{
+  float zh[OR_PV_NBINS + 1];
+  memcpy(zh, h, OR_PV_NBINS * sizeof(float));
+  zh[OR_PV_NBINS] = 0.f;
+  unsigned short edges[200];
+  unsigned n_edges = 0;
+  unsigned short cluster_peak[200];
+  unsigned n_clusters = 0;
+  {
+    const float threshold = OR_PV_DZ * (1.f / (10.f * OR_PV_MAX_TRACK_Z0_ERR));
+    int prevempty = 1;
+    float integral = zh[0];
+    for (int i = 1; i < (int) OR_PV_NBINS; ++i) {
+      integral += zh[i];
+      const int empty = zh[i] < threshold;
+      if (empty != prevempty) {
+        if (prevempty || integral > OR_PV_MIN_TRACKS_IN_SEED) {
+          if (n_edges < 200) edges[n_edges++] = (unsigned short) i;
+        }
+        else {
+          n_edges--;
+        }
+        prevempty = empty;
+        integral = 0;
+      }
+    }
+  }
+  const unsigned n_proto = n_edges / 2;
+  for (unsigned p = 0; p < n_proto; ++p) {
+    const int ibegin = edges[2 * p], iend = edges[2 * p + 1];
+    const float mindip = OR_PV_MIN_DIP_DENSITY * OR_PV_DZ;
+    const float minpeak = OR_PV_MIN_DENSITY * OR_PV_DZ;
+    unsigned short ex_index[32];
+    float ex_value[32], ex_integral[32];
+    int n_ex = 0;
+#define OR_PV_PUSH(I, V, IN)                                                                                          \
+  do {                                                                                                                \
+    if (n_ex < 32) {                                                                                                  \
+      ex_index[n_ex] = (unsigned short) (I);                                                                          \
+      ex_value[n_ex] = (V);                                                                                           \
+      ex_integral[n_ex] = (IN);                                                                                       \
+      ++n_ex;                                                                                                         \
+    }                                                                                                                 \
+  } while (0)
+    {
+      int rising = 1;
+      float integral = zh[ibegin];
+      OR_PV_PUSH(ibegin, zh[ibegin], integral);
+      for (int i = ibegin; i < iend; ++i) {
+        const float value = zh[i];
+        const int stillrising = zh[i + 1] > value;
+        if (rising && !stillrising && value >= minpeak) {
+          const int n = n_ex;
+          if (n >= 2) {
+            const float dv1 = ex_value[n - 2] - ex_value[n - 1];
+            const float dv2 = value - ex_value[n - 1];
+            if (dv1 > mindip && dv2 > mindip) {
+              OR_PV_PUSH(i, value, integral + 0.5f * value);
+            }
+            else if (dv1 > dv2) {
+              n_ex = n_ex - 1 < 0 ? 0 : n_ex - 1;
+            }
+            else {
+              n_ex = n_ex - 2 < 0 ? 0 : n_ex - 2;
+              OR_PV_PUSH(i, value, integral + 0.5f * value);
+            }
+          }
+          else {
+            OR_PV_PUSH(i, value, integral + 0.5f * value);
+          }
+        }
+        else if (rising != stillrising) {
+          OR_PV_PUSH(i, value, integral + 0.5f * value);
+        }
+        rising = stillrising;
+        integral += value;
+      }
+      OR_PV_PUSH(iend, zh[iend], integral);
+    }
+#undef OR_PV_PUSH
+    const int n = n_ex;
+    unsigned short sub_peak[50];
+    unsigned n_sub = 0;
+    if (n > 3) {
+      for (int i = 1; i < n / 2 + 1; ++i)
+        if (ex_integral[2 * i] - ex_integral[2 * i - 2] > OR_PV_MIN_TRACKS_IN_SEED && n_sub < 50)
+          sub_peak[n_sub++] = ex_index[2 * i - 1];
+    }
+    if (n_sub == 0) {
+      if (ex_value[1] >= minpeak && n_clusters < 200) cluster_peak[n_clusters++] = ex_index[1];
+    }
+    else {
+      for (unsigned i = 0; i < n_sub && n_clusters < 200; ++i) cluster_peak[n_clusters++] = sub_peak[i];
+    }
+  }
+  uint32_t n_peaks = 0;
+  for (unsigned i = 0; i < n_clusters && n_peaks < OR_PV_MAX_VERTICES; ++i) {
+    const int izmax = cluster_peak[i];
+    const float d1 = zh[izmax] - zh[izmax - 1];
+    const float d2 = zh[izmax] - zh[izmax + 1];
+    const float idz = d1 + d2 > 0 ? 0.5f * (d1 - d2) / (d1 + d2) : 0.0f;
+    zpeaks[n_peaks++] = OR_PV_ZMIN + OR_PV_DZ * ((float) izmax + idz + 0.5f);
+  }
+  return n_peaks;
+}
+
+void oracle_pv_beamline_peak(uint32_t n_events, const float* zhisto, float* zpeaks, uint32_t* n_zpeaks)
+{
+  for (uint32_t e = 0; e < n_events; ++e)
+    n_zpeaks[e] =
+      oracle_pv_peaks_event(zhisto + (size_t) e * OR_PV_NBINS, zpeaks + (size_t) e * OR_PV_MAX_VERTICES);
+}
+
+/* pv_beamline_multi_fitter.cu:5-183, one seed after the other, each track's
+ * weight denominator summed over the seeds as the reference does. */
+void oracle_pv_beamline_multi_fitter(
+  uint32_t n_events, const uint32_t* atomics_velo, const float* pvtracks, const float* zpeaks,
+  const uint32_t* n_zpeaks, uint32_t* vertices, uint32_t* n_vertices)
+{
+  for (uint32_t e = 0; e < n_events; ++e) {
+    const uint32_t n = atomics_velo[e], off = atomics_velo[n_events + e];
+    const float* seeds = zpeaks + (size_t) e * OR_PV_MAX_VERTICES;
+    uint32_t S = n_zpeaks[e];
+    if (S > OR_PV_MAX_VERTICES) S = OR_PV_MAX_VERTICES;
+    uint32_t kept = 0;
+    for (uint32_t s = 0; s < S; ++s) {
+      int converged = 0;
+      float cov[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      float vx = 0.f, vy = 0.f, vz = seeds[s];
+      float chi2tot = 0.f;
+      for (unsigned iter = 0; iter < OR_PV_MAX_FIT_ITER && !converged; ++iter) {
+        float h00 = 0.f, h11 = 0.f, h20 = 0.f, h21 = 0.f, h22 = 0.f;
+        float gx = 0.f, gy = 0.f, gz = 0.f;
+        chi2tot = 0.f;
+        unsigned nsel = 0;
+        for (uint32_t k = 0; k < n; ++k) {
+          const float* trk = pvtracks + 7 * ((size_t) off + k);
+          const float z = trk[0], x = trk[1], y = trk[2], tx = trk[3], ty = trk[4], w00 = trk[5], w11 = trk[6];
+          if (OR_PV_ZMIN > z || z > OR_PV_ZMAX) continue;
+          const float dz = vz - z;
+          const float rx = vx - (x + tx * dz);
+          const float ry = vy - (y + ty * dz);
+          const float chi2 = rx * rx * w00 + ry * ry * w11;
+          if (!(chi2 < OR_PV_MAX_DELTA_CHI2)) continue;
+          ++nsel;
+          const float chi2_cut = 0.1f + 0.01f * (float) OR_PV_MAX_FIT_ITER / (float) (iter + 1);
+          float weight = oracle_pv_exp(-chi2 * 0.5f);
+          float denom = oracle_pv_exp(-chi2_cut * 0.5f);
+          for (uint32_t o = 0; o < S; ++o) {
+            const float dzo = seeds[o] - z;
+            const float ox = 0.f - (x + tx * dzo);
+            const float oy = 0.f - (y + ty * dzo);
+            const float chi2o = ox * ox * w00 + oy * oy * w11;
+            denom += oracle_pv_exp(-chi2o * 0.5f);
+          }
+          weight = weight / denom;
+          if (weight < OR_PV_MIN_WEIGHT) continue;
+          const float hx = rx * w00;
+          const float hy = ry * w11;
+          const float hz = -tx * rx * w00 - ty * ry * w11;
+          gx = gx + hx * weight;
+          gy = gy + hy * weight;
+          gz = gz + hz * weight;
+          h00 += weight * w00;
+          h11 += weight * w11;
+          h20 += weight * (-tx * w00);
+          h21 += weight * (-ty * w11);
+          h22 += weight * (-tx * -tx * w00 + -ty * -ty * w11);
+          chi2tot += weight * chi2;
+        }
+        if (nsel >= 2) {
+          const float det = h00 * (h22 * h11 - h21 * h21) + h20 * (-h11 * h20);
+          const float inv_det = 1.f / det;
+          cov[0] = (h22 * h11 - h21 * h21) * inv_det;
+          cov[1] = -(-h20 * h21) * inv_det;
+          cov[2] = (h22 * h00 - h20 * h20) * inv_det;
+          cov[3] = (-h20 * h11) * inv_det;
+          cov[4] = -(h21 * h00) * inv_det;
+          cov[5] = (h11 * h00) * inv_det;
+          const float dx = -1.f * (cov[0] * gx + cov[1] * gy + cov[3] * gz);
+          const float dy = -1.f * (cov[1] * gx + cov[2] * gy + cov[4] * gz);
+          const float dzv = -1.f * (cov[3] * gx + cov[4] * gy + cov[5] * gz);
+          chi2tot += dx * gx + dy * gy + dzv * gz;
+          vx = vx + dx;
+          vy = vy + dy;
+          vz = vz + dzv;
+          converged = fabsf(dzv) < 0.001f;
+        }
+        else {
+          break;
+        }
+      }
+      const float rho2 = (vx - 0.f) * (vx - 0.f) + (vy - 0.f) * (vy - 0.f);
+      if (n >= OR_PV_MIN_TRACKS_PER_VERTEX && rho2 < OR_PV_MAX_VERTEX_RHO2) {
+        uint32_t* v = vertices + ((size_t) e * OR_PV_MAX_VERTICES + kept) * OR_PV_VERTEX_WORDS;
+        const float f[4] = {vx, vy, vz, chi2tot};
+        memcpy(v, f, sizeof f);
+        v[4] = 0;
+        v[5] = n;
+        memcpy(v + 6, cov, sizeof cov);
+        v[12] = 0;
+        ++kept;
+      }
+    }
+    n_vertices[e] = kept;
+  }
+}

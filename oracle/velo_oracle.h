@@ -158,6 +158,27 @@ uint32_t oracle_global_event_cut(
   uint32_t n_events, const char* ut_raw, const uint32_t* ut_offsets, const char* scifi_raw,
   const uint32_t* scifi_offsets, uint32_t* event_list);
 
+/* ---- beamline primary vertices (cuda/PV/beamlinePV/src/*.cu,
+ *      BeamlinePVConstants.cuh:3-17).  PV tracks: 7 floats per track
+ *      (z, x, y, tx, ty, W_00, W_11; TrackBeamLineVertexFinder.cuh:10-31);
+ *      histograms: 2400 floats per event; peaks: 32 per event; vertices:
+ *      13 words per vertex, 32 per event (PV::Vertex: x, y, z, chi2, ndof,
+ *      n_tracks, cov00, cov10, cov11, cov20, cov21, cov22, nTracks).
+ *      Histogram shares are added in track order, vertices kept in seed
+ *      order (the reference's float and index atomics have no order). ---- */
+#define OR_PV_NBINS 2400u
+#define OR_PV_MAX_VERTICES 32u
+#define OR_PV_VERTEX_WORDS 13u
+float oracle_pv_exp(float x);
+float oracle_pv_gauss_integral(float x);
+void oracle_pv_beamline_extrapolate(
+  uint32_t n_events, const uint32_t* atomics_velo, const char* kalman_states, float* pvtracks);
+void oracle_pv_beamline_histo(uint32_t n_events, const uint32_t* atomics_velo, const float* pvtracks, float* zhisto);
+void oracle_pv_beamline_peak(uint32_t n_events, const float* zhisto, float* zpeaks, uint32_t* n_zpeaks);
+void oracle_pv_beamline_multi_fitter(
+  uint32_t n_events, const uint32_t* atomics_velo, const float* pvtracks, const float* zpeaks,
+  const uint32_t* n_zpeaks, uint32_t* vertices, uint32_t* n_vertices);
+
 /* ---- textbook 8-connected component labelling (Clustering.cpp:3-280
  *      semantics): LHCbIDs of all clusters of one event, plus centroid
  *      sums so coordinates can be formed; used to pin a6 ---- */

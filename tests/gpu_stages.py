@@ -209,3 +209,48 @@ def global_event_cut(ut_events, ut_offsets, scifi_events, scifi_offsets, block=0
     sync()
     k = int(host(cnt)[0])
     return host(lst).view(np.uint32)[:k].copy(), k
+
+
+def pv_extrapolate(n, atomics, kalman_states, tuning=(0, 0)):
+    T = int(atomics[2 * n])
+    da, ds = dev(np.asarray(atomics, np.uint32).view(np.int32)), dev(kalman_states)
+    out = torch.full((max(T, 1) * 7,), float("nan"), dtype=torch.float32, device="cuda")
+    a = nat.PvExtrapolateArgs(ds.data_ptr(), da.data_ptr(), None, out.data_ptr(), n, tuning_of(tuning))
+    nat.launch("velo_pv_beamline_extrapolate_launch", a)
+    sync()
+    return host(out)[:7 * T].reshape(-1, 7)
+
+
+def pv_histo(n, atomics, pvtracks, tuning=(0, 0)):
+    da = dev(np.asarray(atomics, np.uint32).view(np.int32))
+    dt = dev(np.ascontiguousarray(pvtracks, np.float32).reshape(-1) if len(pvtracks) else np.zeros(7, np.float32))
+    out = torch.full((max(n, 1) * 2400,), float("nan"), dtype=torch.float32, device="cuda")
+    a = nat.PvHistoArgs(da.data_ptr(), None, dt.data_ptr(), out.data_ptr(), n, tuning_of(tuning))
+    nat.launch("velo_pv_beamline_histo_launch", a)
+    sync()
+    return host(out)[:n * 2400].reshape(n, 2400)
+
+
+def pv_peak(n, zhisto, tuning=(0, 0)):
+    dh = dev(np.ascontiguousarray(zhisto, np.float32).reshape(-1))
+    zp = torch.full((max(n, 1) * 32,), float("nan"), dtype=torch.float32, device="cuda")
+    cnt = garbage_u32(n)
+    a = nat.PvPeakArgs(dh.data_ptr(), zp.data_ptr(), cnt.data_ptr(), n, tuning_of(tuning))
+    nat.launch("velo_pv_beamline_peak_launch", a)
+    sync()
+    return host(zp)[:n * 32].reshape(n, 32), host(cnt).view(np.uint32)[:n]
+
+
+def pv_multi_fitter(n, atomics, pvtracks, zpeaks, n_zpeaks, tuning=(0, 0)):
+    from oracle.pyoracle import PV_VERTEX_DTYPE
+    da = dev(np.asarray(atomics, np.uint32).view(np.int32))
+    dt = dev(np.ascontiguousarray(pvtracks, np.float32).reshape(-1) if len(pvtracks) else np.zeros(7, np.float32))
+    dz = dev(np.ascontiguousarray(zpeaks, np.float32).reshape(-1))
+    dn = dev(np.asarray(n_zpeaks, np.uint32).view(np.int32))
+    v = torch.zeros((max(n, 1) * 32 * 13,), dtype=torch.int32, device="cuda")
+    cnt = garbage_u32(n)
+    a = nat.PvMultiFitterArgs(da.data_ptr(), None, dt.data_ptr(), dz.data_ptr(), dn.data_ptr(), v.data_ptr(),
+                              cnt.data_ptr(), n, tuning_of(tuning))
+    nat.launch("velo_pv_beamline_multi_fitter_launch", a)
+    sync()
+    return host(v).view(PV_VERTEX_DTYPE)[:n * 32].reshape(n, 32), host(cnt).view(np.uint32)[:n]

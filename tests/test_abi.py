@@ -16,7 +16,8 @@ def declared(header):
 
 @pytest.mark.parametrize("header,lib", [("velo_hlt.h", "libvelo_hlt.so"), ("velo_gen.h", "libvelo_gen.so"),
                                         ("velo_sequence.h", "libvelo_hlt.so"), ("velo_checker.h", "libvelo_hlt.so"),
-                                        ("velo_ghost.h", "libvelo_hlt.so"), ("velo_mdf.h", "libvelo_hlt.so")])
+                                        ("velo_ghost.h", "libvelo_hlt.so"), ("velo_mdf.h", "libvelo_hlt.so"),
+                                        ("velo_pv.h", "libvelo_hlt.so")])
 def test_library_exports_declared_symbols(header, lib):
     import ctypes
     names = declared(header)
@@ -29,7 +30,7 @@ def test_library_exports_declared_symbols(header, lib):
 def test_python_structs_match_header_layout():
     """Field order of the ctypes mirrors equals the header's structs."""
     from cuda_hlt_amd import native as nat
-    text = (ROOT / "include" / "velo_hlt.h").read_text()
+    text = (ROOT / "include" / "velo_hlt.h").read_text() + (ROOT / "include" / "velo_pv.h").read_text()
     for m in re.finditer(r"typedef struct \{([^{}]*)\} (velo_\w+_args);", text):
         body, name = m.groups()
         fields = re.findall(r"\b(\w+);", re.sub(r"/\*.*?\*/", "", body, flags=re.S))
