@@ -214,7 +214,9 @@ def theoretical_waves_per_cu(wg_size: int, vgprs: int, agprs: int, lds_bytes: in
 def lds_occupancy(path: str | Path, trace: str | Path | None = None) -> dict:
     """Per kernel (mean per dispatch): LDS bytes moved, LDS throughput,
     bank-conflict share of active cycles, mean resident waves per CU and
-    their share of the wave slots, and the theoretical occupancy."""
+    their share of the wave slots.  (rocprofv3's per-dispatch LDS field
+    holds the static allocation only, so the occupancy the resources allow
+    is left to theoretical_waves_per_cu with the launch's dynamic LDS.)"""
     f = _find(path, "counter_collection.csv")
     agent = read_agent(path)
     cus = int(agent["Cu_Count"])
@@ -253,15 +255,12 @@ def lds_occupancy(path: str | Path, trace: str | Path | None = None) -> dict:
         n = len(rs)
         mean = {f: sum(r[f] for r in rs) / n for f in ("ns", "gui", "lds_bytes", "conflict", "waves")}
         us = durations.get(k, mean["ns"] / 1e3)
-        theo = theoretical_waves_per_cu(rs[0]["wg"], rs[0]["vgpr"], rs[0]["agpr"], rs[0]["lds"], agent)
         out[k] = {"dispatches": n, "duration_us": round(us, 2), "lds_bytes": round(mean["lds_bytes"]),
                   "lds_TBps": round(mean["lds_bytes"] / (us * 1e-6) / 1e12, 3) if us else None,
                   "lds_bytes_per_cu_cycle": round(mean["lds_bytes"] / (mean["gui"] * cus), 2) if mean["gui"] else None,
                   "lds_bank_conflict_pct": round(mean["conflict"], 2),
                   "waves_per_cu": round(mean["waves"], 2), "occupancy_pct": round(100 * mean["waves"] / slots_cu, 1),
-                  "theoretical_waves_per_cu": theo["waves"], "theoretical_limit": theo["limit"],
-                  "workgroup_size": rs[0]["wg"], "vgprs": rs[0]["vgpr"], "agprs": rs[0]["agpr"],
-                  "lds_per_workgroup": rs[0]["lds"]}
+                  "workgroup_size": rs[0]["wg"]}
     return {"cu_count": cus, "wave_slots_per_cu": slots_cu, "num_xcc": xcc, "kernels": out}
 
 
@@ -271,13 +270,12 @@ def lds_markdown(t: dict, title: str, command: str) -> str:
         "(SQ_INSTS_LDS_LOAD_BANDWIDTH + SQ_INSTS_LDS_STORE_BANDWIDTH); conflicts = SQ_LDS_BANK_CONFLICT / "
         "(GRBM_GUI_ACTIVE x CUs); resident waves per CU = 4 x SQ_WAVE_CYCLES / (GRBM_GUI_ACTIVE x CUs).", "",
         "| kernel | dispatches | us | LDS bytes | LDS TB/s | LDS B/CU/cycle | bank conflict % | waves/CU | "
-        "occupancy % | resources allow | limited by | WG size | VGPR | AGPR | LDS/WG |",
-        "|---|---|---|---|---|---|---|---|---|---|---|---|---|---|---|"]
+        "occupancy % | WG size |",
+        "|---|---|---|---|---|---|---|---|---|---|"]
     for k, v in sorted(t["kernels"].items(), key=lambda kv: -kv[1]["duration_us"]):
         lines.append(f"| `{k}` | {v['dispatches']} | {v['duration_us']} | {v['lds_bytes']} | {v['lds_TBps']} | "
                      f"{v['lds_bytes_per_cu_cycle']} | {v['lds_bank_conflict_pct']} | {v['waves_per_cu']} | "
-                     f"{v['occupancy_pct']} | {v['theoretical_waves_per_cu']} | {v['theoretical_limit']} | "
-                     f"{v['workgroup_size']} | {v['vgprs']} | {v['agprs']} | {v['lds_per_workgroup']} |")
+                     f"{v['occupancy_pct']} | {v['workgroup_size']} |")
     return "\n".join(lines) + "\n"
 
 

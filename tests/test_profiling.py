@@ -129,5 +129,4 @@ def test_lds_occupancy(tmp_path):
     assert s["lds_bytes"] == pytest.approx(4e9) and s["lds_TBps"] == pytest.approx(4.0)
     assert s["lds_bank_conflict_pct"] == pytest.approx(5.0)
     assert s["waves_per_cu"] == pytest.approx(16.0) and s["occupancy_pct"] == pytest.approx(50.0)
-    assert s["theoretical_waves_per_cu"] == 14 and s["theoretical_limit"] == "lds"
     assert "search_by_triplet_kernel" in pf.lds_markdown(t, "t", "c")
