@@ -96,7 +96,8 @@ def test_sequence_with_pv_matches_oracle(gpu, geometry, module_zs):
     try:
         s.run(n, host_events=ev.events, host_offsets=ev.offsets, pv_beamline=True)
         out = s.host_tracks()
-        _eq(out.kalman, kal)
+        T = int(r.atomics[2 * n])
+        _eq(out.kalman[:49 * T], kal[:49 * T])
         _eq(out.pv_counts, ref["n_vertices"])
         for e in range(n):
             _eq(out.pv_vertices[e, :out.pv_counts[e]], ref["vertices"][e, :ref["n_vertices"][e]])
