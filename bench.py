@@ -388,6 +388,12 @@ def main():
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                 "traffic": measured_traffic(args.traffic, dominant, B)}
+    # decode + clustering together: R + 16 H + 416 bytes per event over the two steps' time
+    dc_ms = per_step.get("velo_estimate_input_size_t", 0.0) + per_step.get("velo_masked_clustering_t", 0.0)
+    dc_bytes = 4 * S + 2504 * B + 16 * H + 416 * B
+    dc = {"bytes": dc_bytes, "ms": round(dc_ms, 4),
+          "GBps": round(dc_bytes / (dc_ms * 1e-3) / 1e9, 1) if dc_ms > 0 else None,
+          "frac": round(dc_bytes / (dc_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5) if dc_ms > 0 else None}
 
     result = {
         "metric": METRIC, "value": round(value, 1), "unit": "events/s", "n_gpus": 1, "steps": args.steps,
@@ -399,6 +405,7 @@ def main():
         "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 3),
                        "p99": round(1e3 * float(np.percentile(lat, 99)), 3)},
         "roofline": roofline,
+        "decode_clustering_roofline": dc,
         "efficiency": {"category": checker.HEADLINE, "value": round(chk.efficiency(), 5),
                        "ghost_rate": round(chk.ghost_rate, 5), "events": B, "tracks": chk.tracks},
         "kernels": kernels,

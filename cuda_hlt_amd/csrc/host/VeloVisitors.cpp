@@ -25,9 +25,9 @@ void tune(S& state, const velo_tuning& t, const RuntimeOptions& ro, WorkSlot slo
   if (x.grid_dim) {
     if (!ro.dev_work_counters) throw std::invalid_argument("persistent launch without work counters");
     x.dev_work_counter = ro.dev_work_counters + 2 * slot;
-    if (!x.dev_work_order) x.dev_work_order = ro.dev_work_order;
-    // a work order lists input events; after the cut, slots are selected events
-    if (ro.run_global_event_cut) x.dev_work_order = nullptr;
+    // a caller's work order lists input events; after the cut, slots are
+    // selected events, so only an order computed on the device applies
+    if (!x.dev_work_order && !ro.run_global_event_cut) x.dev_work_order = ro.dev_work_order;
   }
   state.handler.set_tuning(x);
 }
@@ -173,6 +173,35 @@ void SequenceVisitor::visit<global_event_cut_t>(
              event);
 }
 
+// ---- velo_event_order_t (no reference counterpart) --------------------------------
+template<>
+void SequenceVisitor::set_arguments_size<velo_event_order_t>(
+  velo_event_order_t::arguments_t arguments,
+  const RuntimeOptions&,
+  const Constants&,
+  const HostBuffers& host_buffers)
+{
+  arguments.set_size<dev_event_order>(host_buffers.host_number_of_selected_events[0]);
+}
+
+template<>
+void SequenceVisitor::visit<velo_event_order_t>(
+  velo_event_order_t& state,
+  const velo_event_order_t::arguments_t& arguments,
+  const RuntimeOptions& runtime_options,
+  const Constants&,
+  HostBuffers& host_buffers,
+  hipStream_t& stream,
+  hipEvent_t&)
+{
+  state.set_opts(dim3(host_buffers.host_number_of_selected_events[0]), dim3(1024), stream);
+  state.set_arguments(
+    arguments.offset<dev_velo_raw_input_offsets>(),
+    runtime_options.run_global_event_cut ? arguments.offset<dev_event_list>() : nullptr,
+    arguments.offset<dev_event_order>());
+  state.invoke();
+}
+
 // ---- velo_estimate_input_size_t ----------------------------------------------
 template<>
 void SequenceVisitor::set_arguments_size<velo_estimate_input_size_t>(
@@ -186,7 +215,6 @@ void SequenceVisitor::set_arguments_size<velo_estimate_input_size_t>(
   arguments.set_size<dev_module_cluster_num>(n * VELO_N_MODULES);
   arguments.set_size<dev_module_candidate_num>(n);
   arguments.set_size<dev_cluster_candidates>(n * VELO_MAX_CANDIDATES_EVENT);
-  arguments.set_size<dev_event_order>(n);
   arguments.set_size<dev_velo_event_status>(n);
 }
 
@@ -397,7 +425,18 @@ void SequenceVisitor::visit<velo_search_by_triplet_t>(
     arguments.offset<dev_rel_indices>(),
     constants.dev_velo_module_zs,
     arguments.offset<dev_velo_event_status>());
-  tune(state, runtime_options.tuning.search_by_triplet, runtime_options, kSlotSearch);
+  // No search shape given: persistent over the largest-first order, as many
+  // workgroups as fit on the device at once (the tail of one workgroup per
+  // event otherwise leaves most of the last round idle).
+  velo_tuning t = runtime_options.tuning.search_by_triplet;
+  const velo_tuning none {};
+  if (std::memcmp(&t, &none, sizeof(t)) == 0) {
+    static thread_local uint32_t resident = 0;
+    if (!resident) resident = velo_search_by_triplet_resident_workgroups(0);
+    t.grid_dim = resident;
+    t.dev_work_order = arguments.offset<dev_event_order>();
+  }
+  tune(state, t, runtime_options, kSlotSearch);
   state.invoke();
 }
 

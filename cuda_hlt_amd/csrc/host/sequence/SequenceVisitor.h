@@ -47,6 +47,7 @@ struct SequenceVisitor {
 
 DECLARE_VISITOR(init_event_list_t)
 DECLARE_VISITOR(global_event_cut_t)
+DECLARE_VISITOR(velo_event_order_t)
 DECLARE_VISITOR(velo_estimate_input_size_t)
 DECLARE_VISITOR(prefix_sum_velo_clusters_t)
 DECLARE_VISITOR(velo_masked_clustering_t)

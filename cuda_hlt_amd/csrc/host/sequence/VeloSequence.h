@@ -36,6 +36,12 @@ ALGORITHM(
     dev_number_of_selected_events,
     dev_event_list))
 
+// no reference counterpart: fills dev_event_order (declared, unused there)
+ALGORITHM(
+  velo_event_order_launch,
+  velo_event_order_t,
+  ARGUMENTS(dev_velo_raw_input_offsets, dev_event_list, dev_event_order))
+
 ALGORITHM(
   velo_estimate_input_size_launch,
   velo_estimate_input_size_t,
@@ -98,7 +104,8 @@ ALGORITHM(
     dev_h0_candidates,
     dev_h2_candidates,
     dev_rel_indices,
-    dev_velo_event_status))
+    dev_velo_event_status,
+    dev_event_order))
 
 ALGORITHM(
   velo_weak_tracks_adder_launch,
@@ -184,6 +191,7 @@ ALGORITHM(
 SEQUENCE_T(
   init_event_list_t,
   global_event_cut_t,
+  velo_event_order_t,
   velo_estimate_input_size_t,
   prefix_sum_velo_clusters_t,
   velo_masked_clustering_t,

@@ -268,6 +268,43 @@ __global__ void init_event_list_kernel(uint32_t* event_list, uint32_t n)
   if (i < n) event_list[i] = i;
 }
 
+// Largest-first processing order of the selected events: one workgroup
+// counts the events per size bucket (64 bytes of raw input each), scans the
+// buckets from the largest down and places every event at its bucket's next
+// position.  Events of one bucket come in no particular order; only the
+// scheduling of persistent launches depends on the order, never an output.
+constexpr uint32_t kOrderBuckets = 1024;
+constexpr uint32_t kOrderBlock = 1024;
+
+__device__ inline uint32_t order_bucket(const uint32_t* offsets, const uint32_t* event_list, uint32_t i)
+{
+  const uint32_t ev = event_list ? event_list[i] : i;
+  const uint32_t size = offsets[ev + 1] - offsets[ev];
+  return min(size >> 6, kOrderBuckets - 1);
+}
+
+__global__ void __launch_bounds__(kOrderBlock)
+  event_order_kernel(const uint32_t* offsets, const uint32_t* event_list, uint32_t* order, uint32_t n)
+{
+  __shared__ uint32_t next[kOrderBuckets];
+  __shared__ uint32_t wave_total[kOrderBlock / kWave];
+  const uint32_t t = threadIdx.x;
+  next[t] = 0;
+  __syncthreads();
+  for (uint32_t i = t; i < n; i += kOrderBlock) atomicAdd(&next[order_bucket(offsets, event_list, i)], 1u);
+  __syncthreads();
+  // exclusive scan from the largest bucket down: thread t holds bucket 1023 - t
+  const uint32_t count = next[kOrderBuckets - 1 - t];
+  const uint32_t incl = wave_inclusive_scan(count);
+  if (t % kWave == kWave - 1) wave_total[t / kWave] = incl;
+  __syncthreads();
+  uint32_t before = 0;
+  for (uint32_t w = 0; w < t / kWave; ++w) before += wave_total[w];
+  next[kOrderBuckets - 1 - t] = before + incl - count;
+  __syncthreads();
+  for (uint32_t i = t; i < n; i += kOrderBlock) order[atomicAdd(&next[order_bucket(offsets, event_list, i)], 1u)] = i;
+}
+
 // ---------------------------------------------------------------------------
 // Event staging and module iteration (K1 and K5)
 // ---------------------------------------------------------------------------
@@ -763,6 +800,15 @@ int velo_init_event_list_launch(const velo_init_event_list_args* a, void* stream
   const uint32_t block = 256;
   hipLaunchKernelGGL(init_event_list_kernel, dim3((a->n_events + block - 1) / block), dim3(block), 0,
                      (hipStream_t) stream, a->dev_event_list, a->n_events);
+  return (int) hipGetLastError();
+}
+
+int velo_event_order_launch(const velo_event_order_args* a, void* stream)
+{
+  if (!a || (a->n_events && (!a->dev_raw_input_offsets || !a->dev_event_order))) return (int) hipErrorInvalidValue;
+  if (a->n_events == 0) return 0;
+  hipLaunchKernelGGL(event_order_kernel, dim3(1), dim3(kOrderBlock), 0, (hipStream_t) stream,
+                     a->dev_raw_input_offsets, a->dev_event_list, a->dev_event_order, a->n_events);
   return (int) hipGetLastError();
 }
 

@@ -240,21 +240,182 @@ __device__ inline uint32_t pv_find_peaks(const float* zh, PvPeakScratch& w, floa
   return n_peaks;
 }
 
+// The same search with the work spread over the wave.  The edge pass of
+// the reference decomposes exactly: every empty -> filled transition opens
+// an edge, and the next filled -> empty transition keeps the pair when the
+// sum of the bins after the opening transition up to and including the
+// closing one (in bin order, from 0) exceeds kMinTracksInSeed, else drops
+// it.  So each lane sums one pair's bins in that order, and each lane then
+// runs the extrema and subcluster pass of one kept pair (proto-cluster);
+// the peaks come out in proto-cluster order through a wave prefix sum.
+// Every float operation is the sequential search's, in its order, so the
+// results are identical.  When more than 99 pairs are kept (the sequential
+// search would then reach its 200-edge limit) or transitions exceed the
+// table, lane 0 runs the sequential search instead.
+constexpr uint32_t kMaxTransitions = 512;
+constexpr uint32_t kPeakLanes = 32; // lanes of the extrema pass
+
+struct PvPeakParallel {
+  uint16_t trans[kMaxTransitions];
+  uint16_t kept[kMaxTransitions];  // kept pairs: begin, end, begin, end, ...
+  uint16_t ex_index[kPeakLanes][kMaxExtrema];
+  float ex_value[kPeakLanes][kMaxExtrema];
+  float ex_integral[kPeakLanes][kMaxExtrema];
+  uint32_t n_trans, n_kept;
+};
+
+// Extrema of the proto-cluster [ibegin, iend] into lane slot `w` (the
+// reference's loop, pv_beamline_peak.cu:43-96); returns their number.
+__device__ inline int pv_extrema(const float* zh, int ibegin, int iend, uint16_t* ex_index, float* ex_value,
+                                 float* ex_integral)
+{
+  const float mindip = kMinDipDensity * kPvDz;
+  const float minpeak = kMinDensity * kPvDz;
+  int n_ex = 0;
+  auto push = [&](int i, float v, float in) {
+    if (n_ex < (int) kMaxExtrema) {
+      ex_index[n_ex] = (uint16_t) i;
+      ex_value[n_ex] = v;
+      ex_integral[n_ex] = in;
+      ++n_ex;
+    }
+  };
+  bool rising = true;
+  float integral = zh[ibegin];
+  push(ibegin, zh[ibegin], integral);
+  for (int i = ibegin; i < iend; ++i) {
+    const float value = zh[i];
+    const bool stillrising = zh[i + 1] > value;
+    if (rising && !stillrising && value >= minpeak) {
+      const int n = n_ex;
+      if (n >= 2) {
+        const float dv1 = ex_value[n - 2] - ex_value[n - 1];
+        const float dv2 = value - ex_value[n - 1];
+        if (dv1 > mindip && dv2 > mindip) {
+          push(i, value, integral + 0.5f * value);
+        }
+        else if (dv1 > dv2) {
+          n_ex = max(n_ex - 1, 0);
+        }
+        else {
+          n_ex = max(n_ex - 2, 0);
+          push(i, value, integral + 0.5f * value);
+        }
+      }
+      else {
+        push(i, value, integral + 0.5f * value);
+      }
+    }
+    else if (rising != stillrising) {
+      push(i, value, integral + 0.5f * value);
+    }
+    rising = stillrising;
+    integral += value;
+  }
+  push(iend, zh[iend], integral);
+  return n_ex;
+}
+
+__device__ inline float pv_cluster_mean(const float* zh, int izmax)
+{
+  const float d1 = zh[izmax] - zh[izmax - 1];
+  const float d2 = zh[izmax] - zh[izmax + 1];
+  const float idz = d1 + d2 > 0 ? 0.5f * (d1 - d2) / (d1 + d2) : 0.0f;
+  return kPvZmin + kPvDz * ((float) izmax + idz + 0.5f);
+}
+
 __global__ void __launch_bounds__(64) pv_peak_kernel(velo_pv_beamline_peak_args a)
 {
   __shared__ float zh[kPvBins + 1];
-  __shared__ PvPeakScratch work;
+  __shared__ PvPeakParallel w;
+  __shared__ PvPeakScratch fallback;
   const uint32_t N = a.n_events;
   const uint32_t lane = threadIdx.x;
+  const float threshold = kPvDz * (1.f / (10.f * kMaxTrackZ0Err));
   for_each_event(a.tuning, N, [&](const uint32_t e) {
     const float* in = a.dev_zhisto + (size_t) e * kPvBins;
     for (int b = lane; b < kPvBins; b += kWave) zh[b] = in[b];
     if (lane == 0) zh[kPvBins] = 0.f;
     __syncthreads();
-    if (lane == 0) {
-      float* out = a.dev_zpeaks + (size_t) e * kMaxPv;
-      a.dev_number_of_zpeaks[e] = pv_find_peaks(zh, work, out);
+
+    // transitions of the empty flag (the flag before bin 1 counts as empty)
+    uint32_t n_trans = 0;
+    for (int base = 1; base < kPvBins; base += kWave) {
+      const int i = base + (int) lane;
+      bool t = false;
+      if (i < kPvBins) {
+        const bool prev = i == 1 ? true : zh[i - 1] < threshold;
+        t = (zh[i] < threshold) != prev;
+      }
+      const uint64_t m = ballot(t);
+      const uint32_t at = n_trans + count_below(m);
+      if (t && at < kMaxTransitions) w.trans[at] = (uint16_t) i;
+      n_trans += (uint32_t) __popcll(m);
     }
+    __syncthreads();
+
+    // pairs (opening, closing transition) kept by their bin sums
+    const uint32_t n_pairs = min(n_trans, kMaxTransitions) / 2;
+    uint32_t n_kept = 0;
+    for (uint32_t base = 0; base < n_pairs; base += kWave) {
+      const uint32_t k = base + lane;
+      bool keep = false;
+      int t0 = 0, t1 = 0;
+      if (k < n_pairs) {
+        t0 = w.trans[2 * k];
+        t1 = w.trans[2 * k + 1];
+        float integral = 0;
+        for (int i = t0 + 1; i <= t1; ++i) integral += zh[i];
+        keep = integral > kMinTracksInSeed;
+      }
+      const uint64_t m = ballot(keep);
+      const uint32_t at = n_kept + count_below(m);
+      if (keep) {
+        w.kept[2 * at] = (uint16_t) t0;
+        w.kept[2 * at + 1] = (uint16_t) t1;
+      }
+      n_kept += (uint32_t) __popcll(m);
+    }
+    __syncthreads();
+
+    float* out = a.dev_zpeaks + (size_t) e * kMaxPv;
+    if (n_trans > kMaxTransitions || n_kept > 99) {
+      if (lane == 0) a.dev_number_of_zpeaks[e] = pv_find_peaks(zh, fallback, out);
+      __syncthreads();
+      return;
+    }
+    // extrema pass, one lane per kept pair; peaks in pair order
+    uint32_t n_peaks = 0;
+    const float minpeak = kMinDensity * kPvDz;
+    for (uint32_t base = 0; base < n_kept; base += kPeakLanes) {
+      const uint32_t p = base + lane;
+      uint32_t count = 0;
+      int n = 0;
+      if (lane < kPeakLanes && p < n_kept) {
+        n = pv_extrema(zh, w.kept[2 * p], w.kept[2 * p + 1], w.ex_index[lane], w.ex_value[lane],
+                       w.ex_integral[lane]);
+        if (n > 3)
+          for (int i = 1; i < n / 2 + 1; ++i)
+            count += w.ex_integral[lane][2 * i] - w.ex_integral[lane][2 * i - 2] > kMinTracksInSeed ? 1u : 0u;
+        count = min(count, kMaxSubclusters);
+        if (count == 0 && w.ex_value[lane][1] >= minpeak) count = 1;
+      }
+      const uint32_t incl = wave_inclusive_scan(count);
+      uint32_t at = n_peaks + incl - count;
+      if (count) {
+        uint32_t sub = 0;
+        if (n > 3)
+          for (int i = 1; i < n / 2 + 1 && sub < kMaxSubclusters; ++i)
+            if (w.ex_integral[lane][2 * i] - w.ex_integral[lane][2 * i - 2] > kMinTracksInSeed) {
+              if (at < kMaxPv) out[at] = pv_cluster_mean(zh, w.ex_index[lane][2 * i - 1]);
+              ++at;
+              ++sub;
+            }
+        if (sub == 0 && at < kMaxPv) out[at] = pv_cluster_mean(zh, w.ex_index[lane][1]);
+      }
+      n_peaks += __shfl(incl, kWave - 1, kWave);
+    }
+    if (lane == 0) a.dev_number_of_zpeaks[e] = min(n_peaks, kMaxPv);
     __syncthreads();
   });
 }

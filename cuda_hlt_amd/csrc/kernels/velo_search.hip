@@ -771,6 +771,18 @@ int velo_search_by_triplet_launch(const velo_search_by_triplet_args* a, void* st
   return (int) hipGetLastError();
 }
 
+uint32_t velo_search_by_triplet_resident_workgroups(uint32_t lds_tile)
+{
+  const uint32_t tile = lds_tile ? lds_tile : kDefaultWindowHits;
+  int device = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&device) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, search_by_triplet_kernel, kWave,
+                                                   4 * sizeof(float) * tile) != hipSuccess)
+    return 0;
+  return (uint32_t) (cus * per_cu);
+}
+
 int velo_weak_tracks_adder_launch(const velo_weak_tracks_adder_args* a, void* stream)
 {
   if (!a || !a->dev_velo_cluster_container || !a->dev_module_cluster_start || !a->dev_tracks ||

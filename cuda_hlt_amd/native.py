@@ -101,6 +101,7 @@ KalmanFitArgs = _args("KalmanFitArgs", [
     "dev_atomics_velo", "dev_velo_track_hit_number", "dev_velo_track_hits", "dev_velo_states",
     "dev_velo_kalman_beamline_states"])
 
+EventOrderArgs = _args("EventOrderArgs", ["dev_raw_input_offsets", "dev_event_list", "dev_event_order"])
 GlobalEventCutArgs = type("GlobalEventCutArgs", (C.Structure,), {"_fields_": [
     ("dev_ut_raw_input", P), ("dev_ut_raw_input_offsets", P), ("dev_scifi_raw_input", P),
     ("dev_scifi_raw_input_offsets", P), ("dev_number_of_selected_events", P), ("dev_event_list", P),
@@ -117,6 +118,7 @@ PvMultiFitterArgs = _args("PvMultiFitterArgs", [
 LAUNCHERS = {
     "velo_init_event_list_launch": InitEventListArgs,
     "velo_global_event_cut_launch": GlobalEventCutArgs,
+    "velo_event_order_launch": EventOrderArgs,
     "velo_estimate_input_size_launch": EstimateInputSizeArgs,
     "velo_prefix_sum_launch": PrefixSumArgs,
     "velo_masked_clustering_launch": MaskedClusteringArgs,
@@ -163,6 +165,8 @@ def lib():
         L.velo_constants_free.restype = C.c_int
         L.velo_sp_tables.argtypes = [P, P, P, P]
         L.velo_hlt_version.restype = C.c_char_p
+        L.velo_search_by_triplet_resident_workgroups.argtypes = [U32]
+        L.velo_search_by_triplet_resident_workgroups.restype = U32
         L.velo_pv_exp.argtypes = [C.c_float]
         L.velo_pv_exp.restype = C.c_float
         L.velo_error_string.argtypes = [C.c_int]

@@ -110,6 +110,21 @@ typedef struct {
 } velo_init_event_list_args;
 int velo_init_event_list_launch(const velo_init_event_list_args* a, void* stream);
 
+/* ---- event_order: no reference counterpart; fills the reference's
+ * dev_event_order argument (ArgumentsCommon.cuh:16, declared but unused
+ * there).  Position k of dev_event_order receives a selected-event index,
+ * largest raw event first (in buckets of 64 bytes; order within a bucket
+ * unspecified); persistent launches take events in this order.  Outputs do
+ * not depend on it.  dev_event_list NULL = identity. */
+typedef struct {
+  const uint32_t* dev_raw_input_offsets;
+  const uint32_t* dev_event_list;
+  uint32_t* dev_event_order; /* N */
+  uint32_t n_events;
+  velo_tuning tuning;
+} velo_event_order_args;
+int velo_event_order_launch(const velo_event_order_args* a, void* stream);
+
 /* ---- global_event_cut (GlobalEventCut.cu:3-55, GlobalEventCut.cuh:9) ----
  * Keeps an event when its SciFi cluster estimate (bank bytes without the
  * 4-byte source word, halved, minus 2, in 32-bit unsigned arithmetic) and
@@ -246,6 +261,10 @@ typedef struct {
   velo_tuning tuning;
 } velo_search_by_triplet_args;
 int velo_search_by_triplet_launch(const velo_search_by_triplet_args* a, void* stream);
+/* Workgroups of search_by_triplet that fit on the current device at once
+ * with an LDS window of lds_tile hits (0 = the default): the persistent
+ * grid the sequence uses when no search shape is given.  0 on error. */
+uint32_t velo_search_by_triplet_resident_workgroups(uint32_t lds_tile);
 
 /* ---- K9: weak_tracks_adder (WeakTracksAdder.cu:117-157) ----
  * Accepted weak tracks are appended in weak-track order. */

@@ -254,3 +254,15 @@ def pv_multi_fitter(n, atomics, pvtracks, zpeaks, n_zpeaks, tuning=(0, 0)):
     nat.launch("velo_pv_beamline_multi_fitter_launch", a)
     sync()
     return host(v).view(PV_VERTEX_DTYPE)[:n * 32].reshape(n, 32), host(cnt).view(np.uint32)[:n]
+
+
+def event_order(offsets, event_list=None):
+    n = len(offsets) - 1 if event_list is None else len(event_list)
+    do = dev(np.asarray(offsets, np.uint32).view(np.int32))
+    dl = None if event_list is None else dev(np.asarray(event_list, np.uint32).view(np.int32))
+    out = garbage_u32(n)
+    a = nat.EventOrderArgs(do.data_ptr(), None if dl is None else dl.data_ptr(), out.data_ptr(), n,
+                           nat.Tuning(0, 0, 0, 0, None, None, None))
+    nat.launch("velo_event_order_launch", a)
+    sync()
+    return host(out).view(np.uint32)[:n]
