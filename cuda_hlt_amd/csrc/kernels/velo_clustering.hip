@@ -132,34 +132,35 @@ __device__ inline uint32_t neighbour_pixels_sorted(uint32_t w, uint32_t i, P wor
   return pixels;
 }
 
-// load_window for a sorted bank: three address ranges, one per SP column
+// load_window for a sorted bank: one address range from (col - 1, row - 2)
+// to (col + 1, row + 1), rows outside the window skipped
 template<typename P>
 __device__ inline void load_window_sorted(int32_t sp_row, int32_t sp_col, P words, uint32_t n, uint32_t pa[3])
 {
-  pa[0] = pa[1] = pa[2] = 0;
+  uint32_t p0 = 0, p1 = 0, p2 = 0;
   const int32_t row_lo = sp_row - 2;
   const int32_t r0 = row_lo < 0 ? 0 : row_lo;
   const int32_t r1 = sp_row + 1 > 63 ? 63 : sp_row + 1;
-#pragma unroll
-  for (int32_t rc = 0; rc < 3; ++rc) {
-    const int32_t col = sp_col - 1 + rc;
-    if (col < 0 || col > 511) continue;
-    const uint32_t key_lo = ((uint32_t) col << 6) | (uint32_t) r0;
-    const uint32_t key_hi = ((uint32_t) col << 6) | (uint32_t) r1;
-    uint32_t bits_acc = 0;
-    for (uint32_t j = lower_bound_address(words, n, key_lo); j < n; ++j) {
-      const uint32_t o = words[j];
-      const uint32_t oa = sp_address(o);
-      if (oa > key_hi) break;
-      if (o & 0x80000000u) continue;
-      const int32_t rr = (int32_t) (oa & 0x3Fu) - row_lo;
-      const uint32_t osp = o & 0xFFu;
-      bits_acc |= ((osp & 0x0Fu) << (4 * rr)) | ((osp & 0xF0u) << (12 + 4 * rr));
-    }
-    if (rc == 0) pa[0] = bits_acc;
-    else if (rc == 1) pa[1] = bits_acc;
-    else pa[2] = bits_acc;
+  const int32_t c0 = sp_col - 1 < 0 ? 0 : sp_col - 1;
+  const int32_t c1 = sp_col + 1 > 511 ? 511 : sp_col + 1;
+  const uint32_t key_lo = ((uint32_t) c0 << 6) | (uint32_t) r0;
+  const uint32_t key_hi = ((uint32_t) c1 << 6) | (uint32_t) r1;
+  for (uint32_t j = lower_bound_address(words, n, key_lo); j < n; ++j) {
+    const uint32_t o = words[j];
+    const uint32_t oa = sp_address(o);
+    if (oa > key_hi) break;
+    const int32_t rr = (int32_t) (oa & 0x3Fu) - row_lo;
+    if ((o & 0x80000000u) || (uint32_t) rr >= 4u) continue;
+    const int32_t rc = (int32_t) (oa >> 6) - (sp_col - 1); // 0..2 inside the key range
+    const uint32_t osp = o & 0xFFu;
+    const uint32_t bits = ((osp & 0x0Fu) << (4 * rr)) | ((osp & 0xF0u) << (12 + 4 * rr));
+    p0 |= rc == 0 ? bits : 0u;
+    p1 |= rc == 1 ? bits : 0u;
+    p2 |= rc == 2 ? bits : 0u;
   }
+  pa[0] = p0;
+  pa[1] = p1;
+  pa[2] = p2;
 }
 
 struct Hit {
@@ -206,23 +207,29 @@ __device__ inline bool cluster_candidate(
   const uint32_t row = (uint32_t) (sp_row * 4) + (k % 4u);
   const uint32_t col = (uint32_t) (sp_col * 2) + (k >= 4u ? 1u : 0u);
   const uint32_t start = (1u << (row - (uint32_t) row_lower_limit)) << (16u * (col % 2u));
+  // The reference grows a frontier with the masks of current_mask /
+  // mask_ltr / mask_rtl, which together are the 8-neighbourhood of a pixel
+  // in the 6-column x 16-row window.  After step s its cluster is every
+  // window pixel within s king moves of the candidate along window pixels;
+  // it stops after the first step that adds nothing, or after 12 steps.
+  // The same ball is grown here by dilating the whole cluster (rows, then
+  // pixel columns) and keeping the window pixels.
   uint32_t c0 = 0, c1 = start, c2 = 0;
-  uint32_t w0 = 0, w1 = start, w2 = 0;
-  pa1 &= ~c1;
   for (int it = 0; it < kMaxClusteringIterations; ++it) {
-    const uint32_t m0 = current_mask(w0) | mask_rtl(w1);
-    const uint32_t m1 = current_mask(w1) | mask_rtl(w2) | mask_ltr(w0);
-    const uint32_t m2 = current_mask(w2) | mask_ltr(w1);
-    w0 = pa0 & m0;
-    w1 = pa1 & m1;
-    w2 = pa2 & m2;
-    if (!(w0 | w1 | w2)) break;
-    c0 |= w0;
-    c1 |= w1;
-    c2 |= w2;
-    pa0 &= ~c0;
-    pa1 &= ~c1;
-    pa2 &= ~c2;
+    const uint32_t v0 = c0 | ((c0 << 1) & 0xFFFEFFFFu) | ((c0 >> 1) & 0xFFFF7FFFu);
+    const uint32_t v1 = c1 | ((c1 << 1) & 0xFFFEFFFFu) | ((c1 >> 1) & 0xFFFF7FFFu);
+    const uint32_t v2 = c2 | ((c2 << 1) & 0xFFFEFFFFu) | ((c2 >> 1) & 0xFFFF7FFFu);
+    // (v1:v0) >> 16 moves pixel column 1 into column 0 and column 2 into 1
+    const uint32_t a01 = (uint32_t) ((((uint64_t) v1 << 32) | v0) >> 16);
+    const uint32_t a12 = (uint32_t) ((((uint64_t) v2 << 32) | v1) >> 16);
+    const uint32_t d0 = (v0 | (v0 << 16) | a01) & pa0;
+    const uint32_t d1 = ((v1 | a01 | a12) & pa1) | start;
+    const uint32_t d2 = (v2 | a12 | (v2 >> 16)) & pa2;
+    const bool grew = ((d0 ^ c0) | (d1 ^ c1) | (d2 ^ c2)) != 0u;
+    c0 = d0;
+    c1 = d1;
+    c2 = d2;
+    if (!grew) break;
   }
   if (c2) return false;
   const int n = __popc(c0) + __popc(c1);
@@ -244,15 +251,18 @@ __device__ inline bool cluster_candidate(
   // centroid: sums of absolute pixel columns and rows
   const int x = __popc(c0 & 0xFFFFu) * col_lower_limit + __popc(c0 >> 16) * (col_lower_limit + 1) +
                 __popc(c1 & 0xFFFFu) * (col_lower_limit + 2) + __popc(c1 >> 16) * (col_lower_limit + 3);
-  int y = 0;
-  for (int r = 0; r < 16; ++r) {
-    const uint32_t m = (1u << r) | (1u << (r + 16));
-    y += (__popc(c0 & m) + __popc(c1 & m)) * (row_lower_limit + r);
-  }
-  const uint32_t cx = (uint32_t) (x / n);
-  const uint32_t cy = (uint32_t) (y / n);
+  // sum over pixels of (row_lower_limit + r): the r sum taken bit by bit of r
+  const int y = row_lower_limit * n + (__popc(c0 & 0xAAAAAAAAu) + __popc(c1 & 0xAAAAAAAAu)) +
+                2 * (__popc(c0 & 0xCCCCCCCCu) + __popc(c1 & 0xCCCCCCCCu)) +
+                4 * (__popc(c0 & 0xF0F0F0F0u) + __popc(c1 & 0xF0F0F0F0u)) +
+                8 * (__popc(c0 & 0xFF00FF00u) + __popc(c1 & 0xFF00FF00u));
   const float fxq = (float) x / (float) n;
   const float fyq = (float) y / (float) n;
+  // x, y >= 0 (no pixel lies left of or below the sensor), n <= 64 and the
+  // quotients stay below 1024: the rounded float quotient is within 1e-4 of
+  // x / n and exact when n divides x, so its floor is the integer quotient
+  const uint32_t cx = (uint32_t) fxq;
+  const uint32_t cy = (uint32_t) fyq;
   const float fx = fxq - (float) cx;
   const float fy = fyq - (float) cy;
   *out = make_hit(geometry, sensor, cx, cy, fx, fy);
@@ -314,6 +324,7 @@ __global__ void __launch_bounds__(kOrderBlock)
 // banks are taken as one list in bank order, one SP per lane.
 constexpr uint32_t kDefaultEventWords = 6144;   // estimate: 24 KB of LDS per workgroup
 constexpr uint32_t kDefaultClusterWords = 8192; // clustering: 32 KB
+constexpr uint32_t kClusterEntries = kWave; // candidate entries per wave
 
 __device__ inline uint32_t sel4(uint32_t k, uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3)
 {
@@ -545,8 +556,17 @@ __global__ void __launch_bounds__(1024) estimate_input_size_kernel(velo_estimate
 // ---------------------------------------------------------------------------
 // K5: masked clustering
 // ---------------------------------------------------------------------------
+// Per wave iteration over up to 64 SPs of a module:
+//  1. each lane classifies its SP: isolated (pattern table) or the candidate
+//     byte of a non-isolated SP;
+//  2. the candidates (low nibble first) are packed into the wave's entry
+//     list, so one growth pass serves up to 64 candidates whatever lane they
+//     came from;
+//  3. a wave prefix over the per-SP cluster counts gives each SP its slots,
+//     in bank, SP, cluster order.
 template<typename P>
-__device__ __forceinline__ void cluster_event(P ev, const velo_masked_clustering_args& a, uint32_t e, uint32_t* block_words)
+__device__ __forceinline__ void cluster_event(
+  P ev, const velo_masked_clustering_args& a, uint32_t e, uint32_t* block_words, uint32_t* entries)
 {
   const uint32_t waves = blockDim.x / kWave;
   const uint32_t wave = threadIdx.x / kWave;
@@ -566,77 +586,97 @@ __device__ __forceinline__ void cluster_event(P ev, const velo_masked_clustering
     check_sorted(ev, mb);
     const uint32_t mstart = starts[m];
     uint32_t pos = 0;
-    for (uint32_t jb = 0; jb < mb.total; jb += kWave) {
+    for (uint32_t jb = 0; jb < mb.total;) {
       const uint32_t j = jb + lane;
-      Hit h0 {}, h1 {};
-      uint32_t cnt = 0;
-      if (j < mb.total) {
-        uint32_t k;
-        const uint32_t i = bank_slot(mb, j, &k);
+      uint32_t i = 0, k = 0, w = 0, c = 0, idx = 0;
+      bool live = j < mb.total;
+      if (live) {
+        i = bank_slot(mb, j, &k);
         const uint32_t base = sel4(k, mb.b0, mb.b1, mb.b2, mb.b3);
         const uint32_t n = sel4(k, mb.n0, mb.n1, mb.n2, mb.n3);
-        const uint32_t sensor = 4 * m + k;
-        const uint32_t w = ev[base + i];
-        const uint32_t sp_addr = (w & 0x007FFF00u) >> 8;
-        const uint32_t sp = w & 0xFFu;
-        if (w & 0x80000000u) {
-          // isolated SP: pre-computed pattern (MaskedVeloClustering.cu:105-167)
-          const uint32_t idx = a.dev_velo_sp_patterns[sp];
-          const uint32_t sp_row = sp_addr & 0x3Fu, sp_col = sp_addr >> 6;
-          h0 = make_hit(a.dev_velo_geometry, sensor, sp_col * 2u + ((idx >> 2) & 1u), sp_row * 4u + (idx & 3u),
-                        a.dev_velo_sp_fx[2 * sp], a.dev_velo_sp_fy[2 * sp]);
-          cnt = 1;
-          if (idx & 8u) {
-            h1 = make_hit(a.dev_velo_geometry, sensor, sp_col * 2u + ((idx >> 6) & 1u), sp_row * 4u + ((idx >> 4) & 3u),
-                          a.dev_velo_sp_fx[2 * sp + 1], a.dev_velo_sp_fy[2 * sp + 1]);
-            cnt = 2;
-          }
-        }
-        else {
-          const bool sorted = (mb.sorted >> k) & 1u;
-          const uint32_t c = candidates_from_pixels(sorted ? neighbour_pixels_sorted(w, i, ev + base, n)
-                                                           : neighbour_pixels(w, ev + base, n));
-          if (c) {
-            // the reference decodes the candidate's SP index as uint8_t (:179)
-            const uint32_t ci = i & 0xFFu;
-            const uint32_t cw = ci == i ? w : ev[base + ci];
-            const uint32_t caddr = (cw & 0x007FFF00u) >> 8;
-            const int32_t sp_row = (int32_t) (caddr & 0x3Fu), sp_col = (int32_t) (caddr >> 6);
-            uint32_t pa[3];
-            if (sorted) load_window_sorted(sp_row, sp_col, ev + base, n, pa);
-            else load_window(sp_row, sp_col, ev + base, n, pa);
-            Hit t;
-            if ((c & 0x0Fu) && cluster_candidate(pa[0], pa[1], pa[2], sp_row, sp_col, c_candidate_ks[c & 0x0Fu],
-                                                 a.dev_velo_geometry, sensor, &t)) {
-              h0 = t;
-              cnt = 1;
-            }
-            if ((c & 0xF0u) && cluster_candidate(pa[0], pa[1], pa[2], sp_row, sp_col, c_candidate_ks[c >> 4] + 2u,
-                                                 a.dev_velo_geometry, sensor, &t)) {
-              if (cnt == 0) h0 = t;
-              else h1 = t;
-              ++cnt;
-            }
-          }
-        }
+        w = ev[base + i];
+        if (w & 0x80000000u) idx = a.dev_velo_sp_patterns[w & 0xFFu];
+        else
+          c = candidates_from_pixels(((mb.sorted >> k) & 1u) ? neighbour_pixels_sorted(w, i, ev + base, n)
+                                                             : neighbour_pixels(w, ev + base, n));
       }
+      uint32_t has_lo = (c & 0x0Fu) ? 1u : 0u;
+      uint32_t has_hi = (c & 0xF0u) ? 1u : 0u;
+      uint32_t n_ent;
+      const uint32_t epre = wave_prefix_small(has_lo + has_hi, &n_ent);
+      // this pass takes the SPs whose candidates fit in one wave of entries
+      // (at least 32 SPs); the rest start the next pass
+      const uint32_t end = epre + has_lo + has_hi;
+      if (end > kWave) {
+        live = false;
+        has_lo = has_hi = 0;
+      }
+      const uint32_t taken = (uint32_t) __popcll(ballot(live));
+      n_ent = n_ent > kWave ? __shfl(end, taken - 1u, 64) : n_ent;
+      const bool iso = live && (w & 0x80000000u);
+      if (has_lo) entries[epre] = lane;
+      if (has_hi) entries[epre + has_lo] = lane | 64u;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+      // growth: lane t takes entry t
+      Hit r {};
+      const uint32_t d = lane < n_ent ? entries[lane] : 0u;
+      uint64_t acc = 0;
+      if (n_ent) {
+        const uint32_t src = d & 63u;
+        const uint32_t si = __shfl(i, src, 64), sk = __shfl(k, src, 64), sc = __shfl(c, src, 64);
+        bool good = false;
+        if (lane < n_ent) {
+          const uint32_t kp = (d & 64u) ? c_candidate_ks[sc >> 4] + 2u : c_candidate_ks[sc & 0x0Fu];
+          const uint32_t base = sel4(sk, mb.b0, mb.b1, mb.b2, mb.b3);
+          const uint32_t n = sel4(sk, mb.n0, mb.n1, mb.n2, mb.n3);
+          // the reference decodes the candidate's SP index as uint8_t (:179)
+          const uint32_t cw = ev[base + (si & 0xFFu)];
+          const uint32_t caddr = (cw & 0x007FFF00u) >> 8;
+          const int32_t sp_row = (int32_t) (caddr & 0x3Fu), sp_col = (int32_t) (caddr >> 6);
+          uint32_t pa[3];
+          if ((mb.sorted >> sk) & 1u) load_window_sorted(sp_row, sp_col, ev + base, n, pa);
+          else load_window(sp_row, sp_col, ev + base, n, pa);
+          good = cluster_candidate(pa[0], pa[1], pa[2], sp_row, sp_col, kp, a.dev_velo_geometry, 4 * m + sk, &r);
+        }
+        acc = ballot(good);
+      }
+
+      // clusters per SP and their slots
+      const uint32_t lo_ok = has_lo ? (uint32_t) (acc >> epre) & 1u : 0u;
+      const uint32_t hi_ok = has_hi ? (uint32_t) (acc >> (epre + has_lo)) & 1u : 0u;
+      const uint32_t cnt = iso ? ((idx & 8u) ? 2u : 1u) : lo_ok + hi_ok;
       uint32_t tot;
-      const uint32_t pre = wave_prefix_small(cnt, &tot);
-      if (cnt > 0) {
-        const uint32_t slot = mstart + pos + pre;
-        xs[slot] = h0.x;
-        ys[slot] = h0.y;
-        zs[slot] = h0.z;
-        ids[slot] = h0.id;
+      const uint32_t slot = mstart + pos + wave_prefix_small(cnt, &tot);
+      auto store = [&](uint32_t s, const Hit& h) {
+        xs[s] = h.x;
+        ys[s] = h.y;
+        zs[s] = h.z;
+        ids[s] = h.id;
+      };
+      if (iso) {
+        // isolated SP: pre-computed pattern (MaskedVeloClustering.cu:105-167)
+        const uint32_t sp_addr = (w & 0x007FFF00u) >> 8, sp = w & 0xFFu;
+        const uint32_t sp_row = sp_addr & 0x3Fu, sp_col = sp_addr >> 6, sensor = 4 * m + k;
+        store(slot, make_hit(a.dev_velo_geometry, sensor, sp_col * 2u + ((idx >> 2) & 1u), sp_row * 4u + (idx & 3u),
+                             a.dev_velo_sp_fx[2 * sp], a.dev_velo_sp_fy[2 * sp]));
+        if (idx & 8u)
+          store(slot + 1, make_hit(a.dev_velo_geometry, sensor, sp_col * 2u + ((idx >> 6) & 1u),
+                                   sp_row * 4u + ((idx >> 4) & 3u), a.dev_velo_sp_fx[2 * sp + 1],
+                                   a.dev_velo_sp_fy[2 * sp + 1]));
       }
-      if (cnt > 1) {
-        const uint32_t slot = mstart + pos + pre + 1;
-        xs[slot] = h1.x;
-        ys[slot] = h1.y;
-        zs[slot] = h1.z;
-        ids[slot] = h1.id;
+      if (n_ent) {
+        // a high-nibble cluster follows its SP's low-nibble cluster when that was kept
+        const uint32_t s = __shfl(slot, d & 63u, 64);
+        const uint32_t lo = __shfl(lo_ok, d & 63u, 64);
+        if ((acc >> lane) & 1u) store(s + ((d & 64u) ? lo : 0u), r);
       }
       pos += tot;
+      jb += taken;
+      // the entry list is rewritten by the next pass
+      __builtin_amdgcn_wave_barrier();
     }
     if (lane == 0) a.dev_module_cluster_num[e * VELO_N_MODULES + m] = pos;
     if (pos > VELO_MAX_NUMHITS_IN_MODULE) status |= VELO_STATUS_MODULE_HITS_OVERFLOW;
@@ -651,7 +691,10 @@ __device__ __forceinline__ void cluster_event(P ev, const velo_masked_clustering
   }
 }
 
-__global__ void __launch_bounds__(1024) masked_clustering_kernel(velo_masked_clustering_args a)
+// At most 80 VGPRs: six waves per SIMD (the kernel is VALU-bound and needs
+// the waves to cover LDS latency).
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
+masked_clustering_kernel(velo_masked_clustering_args a)
 {
   extern __shared__ uint32_t lds[];
   __shared__ uint32_t block_words[2];
@@ -660,15 +703,17 @@ __global__ void __launch_bounds__(1024) masked_clustering_kernel(velo_masked_clu
     const uint32_t off = a.dev_raw_input_offsets[ei];
     const uint32_t nwords = (a.dev_raw_input_offsets[ei + 1] - off) / 4u;
     const uint32_t* gev = reinterpret_cast<const uint32_t*>(a.dev_raw_input + off);
+    // each wave's candidate entry list follows the event tile
+    uint32_t* entries = lds + a.tuning.lds_tile + (threadIdx.x / kWave) * kClusterEntries;
     if (threadIdx.x < 2) block_words[threadIdx.x] = 0;
     if (nwords <= a.tuning.lds_tile) {
       stage_words(lds, gev, nwords);
       __syncthreads();
-      cluster_event(static_cast<const uint32_t*>(lds), a, e, block_words);
+      cluster_event(static_cast<const uint32_t*>(lds), a, e, block_words, entries);
     }
     else {
       __syncthreads();
-      cluster_event(gev, a, e, block_words);
+      cluster_event(gev, a, e, block_words, entries);
     }
   });
 }
@@ -840,7 +885,7 @@ int velo_masked_clustering_launch(const velo_masked_clustering_args* in, void* s
   const uint32_t block = block_of(a.tuning);
   if (!block) return (int) hipErrorInvalidValue;
   if (!a.tuning.lds_tile) a.tuning.lds_tile = kDefaultClusterWords;
-  const size_t shmem = (size_t) a.tuning.lds_tile * sizeof(uint32_t);
+  const size_t shmem = ((size_t) a.tuning.lds_tile + (block / kWave) * kClusterEntries) * sizeof(uint32_t);
   hipLaunchKernelGGL(masked_clustering_kernel, dim3(event_grid(a.tuning, a.n_events)), dim3(block), shmem, (hipStream_t) stream, a);
   return (int) hipGetLastError();
 }
