@@ -436,8 +436,12 @@ __device__ inline uint32_t block_or(uint32_t v, uint32_t* shared_word)
 template<typename P>
 __device__ __forceinline__ void estimate_event(
   P ev, const velo_estimate_input_size_args& a, uint32_t e, uint32_t* module_cands, uint32_t* module_cand_offset,
-  uint32_t* block_status)
+  uint32_t* block_status, uint32_t* stash, uint32_t stash_words)
 {
+  // stash: the event's LDS copy (null when the event is read from global
+  // memory).  Pass 1 keeps each non-isolated SP's candidate byte in bits
+  // 23-30 of its word there, bits no reader of the copy looks at, so pass 2
+  // need not compute it again.
   const uint32_t waves = blockDim.x / kWave;
   const uint32_t wave = threadIdx.x / kWave;
   const uint32_t lane = lane_id();
@@ -462,6 +466,7 @@ __device__ __forceinline__ void estimate_event(
         const uint32_t n = sel4(k, mb.n0, mb.n1, mb.n2, mb.n3);
         const uint32_t c = candidates_from_pixels(((mb.sorted >> k) & 1u) ? neighbour_pixels_sorted(w, i, ev + base, n)
                                                                           : neighbour_pixels(w, ev + base, n));
+        if (stash && base + i < stash_words) stash[base + i] = (w & 0x807FFFFFu) | (c << 23);
         const uint32_t f = ((c & 0x0Fu) ? 1u : 0u) + ((c & 0xF0u) ? 1u : 0u);
         est += f;
         ncand += f;
@@ -489,7 +494,7 @@ __device__ __forceinline__ void estimate_event(
   for (uint32_t m = wave; m < VELO_N_MODULES; m += waves) {
     if (module_cands[m] == 0) continue;
     ModuleBanks mb = module_banks(ev, n_banks, m);
-    check_sorted(ev, mb);
+    if (!stash) check_sorted(ev, mb);
     uint32_t pos = module_cand_offset[m];
     for (uint32_t jb = 0; jb < mb.total; jb += kWave) {
       const uint32_t j = jb + lane;
@@ -500,8 +505,9 @@ __device__ __forceinline__ void estimate_event(
         const uint32_t w = ev[base + i];
         const uint32_t n = sel4(k, mb.n0, mb.n1, mb.n2, mb.n3);
         if (!(w & 0x80000000u))
-          c = candidates_from_pixels(((mb.sorted >> k) & 1u) ? neighbour_pixels_sorted(w, i, ev + base, n)
-                                                             : neighbour_pixels(w, ev + base, n));
+          c = stash ? (w >> 23) & 0xFFu
+                    : candidates_from_pixels(((mb.sorted >> k) & 1u) ? neighbour_pixels_sorted(w, i, ev + base, n)
+                                                                     : neighbour_pixels(w, ev + base, n));
       }
       const uint32_t cnt = ((c & 0x0Fu) ? 1u : 0u) + ((c & 0xF0u) ? 1u : 0u);
       uint32_t total;
@@ -544,11 +550,11 @@ __global__ void __launch_bounds__(1024) estimate_input_size_kernel(velo_estimate
     if (nwords <= a.tuning.lds_tile) {
       stage_words(lds, gev, nwords);
       __syncthreads();
-      estimate_event(static_cast<const uint32_t*>(lds), a, e, module_cands, module_cand_offset, &block_status);
+      estimate_event(static_cast<const uint32_t*>(lds), a, e, module_cands, module_cand_offset, &block_status, lds, nwords);
     }
     else {
       __syncthreads();
-      estimate_event(gev, a, e, module_cands, module_cand_offset, &block_status);
+      estimate_event(gev, a, e, module_cands, module_cand_offset, &block_status, nullptr, 0u);
     }
   });
 }
