@@ -24,9 +24,64 @@ constexpr size_t kMaxSortLds = 65536;
 
 __device__ inline float module_phi(uint32_t m, float x, float y) { return (m & 1u) ? phi_odd(x, y) : phi_even(x, y); }
 
+// phi as an unsigned key that orders like the float; -0 and +0 share the
+// key of +0 (they compare equal), so the key of 0 does not keep the sign.
+__device__ inline uint32_t phi_key(float p)
+{
+  const uint32_t b = __float_as_uint(p + 0.0f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ inline uint32_t key_phi_bits(uint32_t k) { return (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k; }
+
+// Sorted position of each of a module's n hits, by phi with ties in slot
+// order (CalculatePhi.cu:47-53): perm[pos] = i.  keys[i] is phi_key of hit
+// i.  Counting the strictly smaller keys gives every hit its position when
+// no two phis are equal; equal phis share a count, which leaves a position
+// unwritten, and the module is then ranked again with the slot tie-break.
+__device__ inline uint32_t keys_below(uint4 q, uint32_t k)
+{
+  return (uint32_t) (q.x < k) + (uint32_t) (q.y < k) + (uint32_t) (q.z < k) + (uint32_t) (q.w < k);
+}
+
+__device__ inline void rank_module(const uint32_t* keys, uint16_t* perm, uint32_t n_any, uint32_t lane)
+{
+  const uint32_t n = __builtin_amdgcn_readfirstlane(n_any); // the same on every lane
+  const uint4* k4 = reinterpret_cast<const uint4*>(keys);
+  for (uint32_t p = lane; p < n; p += kWave) perm[p] = 0xFFFFu;
+  wave_sync();
+#pragma unroll 1
+  for (uint32_t i0 = 0; i0 < n; i0 += kWave) {
+    const uint32_t i = i0 + lane;
+    const uint32_t ki = i < n ? keys[i] : 0u;
+    uint32_t r = 0, j = 0;
+    // four 16-byte reads in flight per step
+    for (; j + 16 <= n; j += 16) {
+      const uint4 q0 = k4[j / 4], q1 = k4[j / 4 + 1], q2 = k4[j / 4 + 2], q3 = k4[j / 4 + 3];
+      r += keys_below(q0, ki) + keys_below(q1, ki) + keys_below(q2, ki) + keys_below(q3, ki);
+    }
+    for (; j + 4 <= n; j += 4) r += keys_below(k4[j / 4], ki);
+    for (; j < n; ++j) r += (uint32_t) (keys[j] < ki);
+    if (i < n) perm[r] = (uint16_t) i;
+  }
+  wave_sync();
+  bool clash = false;
+  for (uint32_t p = lane; p < n; p += kWave) clash |= perm[p] == 0xFFFFu;
+  if (ballot(clash) == 0) return;
+  for (uint32_t i = lane; i < n; i += kWave) {
+    const uint32_t ki = keys[i];
+    uint32_t r = 0;
+    for (uint32_t j = 0; j < n; ++j) {
+      const uint32_t kj = keys[j];
+      r += (kj < ki) || (kj == ki && j < i);
+    }
+    perm[r] = (uint16_t) i;
+  }
+  wave_sync();
+}
+
 __global__ void __launch_bounds__(1024) calculate_phi_and_sort_kernel(velo_calculate_phi_and_sort_args a)
 {
-  extern __shared__ uint32_t lds[];
+  extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
   for_each_event(a.tuning, a.n_events, [&](const uint32_t e) {
     const uint32_t waves = blockDim.x / kWave;
     const uint32_t wave = threadIdx.x / kWave;
@@ -38,7 +93,7 @@ __global__ void __launch_bounds__(1024) calculate_phi_and_sort_kernel(velo_calcu
     uint32_t* ID = Z + total;
     uint32_t* PHI = ID + total;
     uint32_t* TMP = PHI + total;
-    float* phi = reinterpret_cast<float*>(lds) + wave * kSortCap;
+    uint32_t* keys = lds + wave * kSortCap;
     uint16_t* perm = reinterpret_cast<uint16_t*>(lds + waves * kSortCap) + wave * kSortCap;
     // the event's x, y, z, id (slots 0-3), when they fit
     uint32_t* ev = lds + waves * kSortCap + waves * kSortCap / 2;
@@ -74,20 +129,16 @@ __global__ void __launch_bounds__(1024) calculate_phi_and_sort_kernel(velo_calcu
       if (staged && n <= kSortCap) {
         const uint32_t r = s - off;
         for (uint32_t i = lane; i < n; i += kWave)
-          phi[i] = module_phi(m, __uint_as_float(ev[r + i]), __uint_as_float(ev[cap + r + i]));
+          keys[i] = phi_key(module_phi(m, __uint_as_float(ev[r + i]), __uint_as_float(ev[cap + r + i])));
         wave_sync();
-        for (uint32_t i = lane; i < n; i += kWave) {
-          const float p = phi[i];
-          uint32_t pos = 0;
-          for (uint32_t j = 0; j < n; ++j) {
-            const float q = phi[j];
-            pos += (p > q) || (p == q && i > j);
-          }
-          perm[pos] = (uint16_t) i;
-          a.dev_hit_permutation[s + pos] = s + i;
-          PHI[s + pos] = __float_as_uint(p);
+        rank_module(keys, perm, n, lane);
+        for (uint32_t p = lane; p < n; p += kWave) {
+          const uint32_t i = perm[p];
+          uint32_t bits = key_phi_bits(keys[i]);
+          if (bits == 0u) bits = __float_as_uint(module_phi(m, __uint_as_float(ev[r + i]), __uint_as_float(ev[cap + r + i])));
+          a.dev_hit_permutation[s + p] = s + i;
+          PHI[s + p] = bits;
         }
-        wave_sync();
         // SortByPhi.cu:21-33 order: X->5, Y->0, Z->1, ID->2; every read is from LDS
         for (uint32_t p = lane; p < n; p += kWave) {
           const uint32_t q = r + perm[p];
@@ -98,20 +149,17 @@ __global__ void __launch_bounds__(1024) calculate_phi_and_sort_kernel(velo_calcu
         }
       }
       else if (n <= kSortCap) {
-        for (uint32_t i = lane; i < n; i += kWave) phi[i] = module_phi(m, __uint_as_float(X[s + i]), __uint_as_float(Y[s + i]));
+        for (uint32_t i = lane; i < n; i += kWave)
+          keys[i] = phi_key(module_phi(m, __uint_as_float(X[s + i]), __uint_as_float(Y[s + i])));
         wave_sync();
-        for (uint32_t i = lane; i < n; i += kWave) {
-          const float p = phi[i];
-          uint32_t pos = 0;
-          for (uint32_t j = 0; j < n; ++j) {
-            const float q = phi[j];
-            pos += (p > q) || (p == q && i > j);
-          }
-          perm[pos] = (uint16_t) i;
-          a.dev_hit_permutation[s + pos] = s + i;
-          PHI[s + pos] = __float_as_uint(p);
+        rank_module(keys, perm, n, lane);
+        for (uint32_t p = lane; p < n; p += kWave) {
+          const uint32_t i = perm[p];
+          uint32_t bits = key_phi_bits(keys[i]);
+          if (bits == 0u) bits = __float_as_uint(module_phi(m, __uint_as_float(X[s + i]), __uint_as_float(Y[s + i])));
+          a.dev_hit_permutation[s + p] = s + i;
+          PHI[s + p] = bits;
         }
-        wave_sync();
         uint32_t vx[kSortChunks], vy[kSortChunks], vz[kSortChunks], vid[kSortChunks];
   #pragma unroll
         for (uint32_t c = 0; c < kSortChunks; ++c) {

@@ -197,6 +197,48 @@ def test_calculate_phi_and_sort_uneven(stages, chain_uneven):
     _check_sort(stages, chain_uneven, (512, 0))
 
 
+@pytest.mark.parametrize("tile", [0, 1])
+def test_calculate_phi_and_sort_equal_phis(stages, chain, tile):
+    """Hits sharing a phi keep their slot order, and a phi of -0 sorts level
+    with +0 and keeps its sign: in every module of the first events, hits 1-3
+    copy hit 0's x and y, and two hits sit at y = -0 and y = +0; 65 hits of
+    one module share one phi, so the tie spans two waves of ranks.
+    tile 1 sorts from global memory, tile 0 from LDS."""
+    n = 20
+    starts = chain.starts[: n * N_MODULES + 1].copy()
+    nums = chain.nums[: n * N_MODULES].copy()
+    total_all = int(chain.starts[chain.n * N_MODULES])
+    total = int(starts[n * N_MODULES])
+    cont = np.concatenate([chain.cont_raw[k * total_all: k * total_all + total] for k in range(6)]).copy()
+    x = cont[0:total].view(np.float32)
+    y = cont[total: 2 * total].view(np.float32)
+    changed, big = 0, False
+    for e in range(n):
+        for m in range(N_MODULES):
+            s, k = int(starts[e * N_MODULES + m]), int(nums[e * N_MODULES + m])
+            if k >= 6:
+                x[s + 1: s + 4] = x[s]
+                y[s + 1: s + 4] = y[s]
+                x[s + 4] = x[s + 5] = 5.0
+                y[s + 4] = np.float32(-0.0)
+                y[s + 5] = np.float32(0.0)
+                changed += 1
+            if k >= 70 and not big:
+                x[s: s + 65] = x[s]
+                y[s: s + 65] = y[s]
+                big = True
+    assert changed > 100 and big
+    want, want_perm = po.calculate_phi_and_sort(n, starts, nums, cont)
+    got, perm = stages.phi_sort(n, starts, nums, cont, (256, tile))
+    mask = module_mask(n, starts, nums)
+    for s in (0, 1, 2, 4, 5):
+        np.testing.assert_array_equal(got[s * total: (s + 1) * total], want[s * total: (s + 1) * total],
+                                      err_msg=f"slot {s}")
+    np.testing.assert_array_equal(perm[:total][mask], want_perm[:total][mask])
+    phi = want[4 * total: 5 * total][mask]
+    assert np.any(phi == 0x80000000) and np.any(phi == 0)   # both zeros present
+
+
 def _check_fill(stages, c: Chain, tuning=(0, 0)):
     h0, h2 = stages.fill_candidates(c.n, c.cont, c.starts, c.nums, tuning)
     total = int(c.starts[c.n * N_MODULES])
