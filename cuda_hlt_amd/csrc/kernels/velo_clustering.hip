@@ -4,10 +4,12 @@
 // Work layout: one workgroup per event.  The workgroup first copies the raw
 // event into LDS (coalesced); wave w then owns modules w, w+W, w+2W, ...
 // (W = waves per workgroup) and takes the SPs of a module's four banks as
-// one list, one SP per lane.  Each lane resolves its SP's neighbourhood by
-// scanning its bank in LDS; output slots come from a ballot-based wave
-// prefix sum, so clusters are written in bank, SP, cluster order with no
-// atomics and no cross-wave synchronisation.
+// one list, one SP per lane.  estimate_input_size resolves each SP's
+// neighbourhood by scanning its bank in LDS and lists the candidates;
+// masked clustering reads that list (or scans the banks itself when it has
+// none) and grows one candidate per lane.  Output slots come from a
+// ballot-based wave prefix sum, so clusters are written in bank, SP, cluster
+// order with no atomics and no cross-wave synchronisation.
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -133,9 +135,12 @@ __device__ inline uint32_t neighbour_pixels_sorted(uint32_t w, uint32_t i, P wor
 }
 
 // load_window for a sorted bank: one address range from (col - 1, row - 2)
-// to (col + 1, row + 1), rows outside the window skipped
+// to (col + 1, row + 1), rows outside the window skipped.  hint: index of
+// the SP at (sp_row, sp_col); the range starts at most a few SPs before it,
+// so it is found by stepping back (a binary search after four steps).
 template<typename P>
-__device__ inline void load_window_sorted(int32_t sp_row, int32_t sp_col, P words, uint32_t n, uint32_t pa[3])
+__device__ inline void load_window_sorted(
+  int32_t sp_row, int32_t sp_col, P words, uint32_t n, uint32_t hint, uint32_t pa[3])
 {
   uint32_t p0 = 0, p1 = 0, p2 = 0;
   const int32_t row_lo = sp_row - 2;
@@ -145,7 +150,15 @@ __device__ inline void load_window_sorted(int32_t sp_row, int32_t sp_col, P word
   const int32_t c1 = sp_col + 1 > 511 ? 511 : sp_col + 1;
   const uint32_t key_lo = ((uint32_t) c0 << 6) | (uint32_t) r0;
   const uint32_t key_hi = ((uint32_t) c1 << 6) | (uint32_t) r1;
-  for (uint32_t j = lower_bound_address(words, n, key_lo); j < n; ++j) {
+  uint32_t j0 = hint < n ? hint : n;
+  for (int step = 0; j0 > 0 && sp_address(words[j0 - 1]) >= key_lo; ++step) {
+    if (step == 4) {
+      j0 = lower_bound_address(words, j0, key_lo);
+      break;
+    }
+    --j0;
+  }
+  for (uint32_t j = j0; j < n; ++j) {
     const uint32_t o = words[j];
     const uint32_t oa = sp_address(o);
     if (oa > key_hi) break;
@@ -166,6 +179,13 @@ __device__ inline void load_window_sorted(int32_t sp_row, int32_t sp_col, P word
 struct Hit {
   float x, y, z;
   uint32_t id;
+};
+
+// A cluster before the local -> global transform: centroid pixel, its
+// fractional offsets and the sensor; key = cx | cy << 10 | sensor << 18.
+struct HitSeed {
+  uint32_t key;
+  float fx, fy;
 };
 
 // Local -> global, MaskedVeloClustering.cu:122-138 and :431-449.
@@ -199,8 +219,7 @@ __device__ inline Hit make_hit(const char* geometry, uint32_t sensor, uint32_t c
 
 // Cluster grown from candidate pixel k (MaskedVeloClustering.cu:274-461).
 __device__ inline bool cluster_candidate(
-  uint32_t pa0, uint32_t pa1, uint32_t pa2, int32_t sp_row, int32_t sp_col, uint32_t k, const char* geometry,
-  uint32_t sensor, Hit* out)
+  uint32_t pa0, uint32_t pa1, uint32_t pa2, int32_t sp_row, int32_t sp_col, uint32_t k, uint32_t sensor, HitSeed* out)
 {
   const int32_t row_lower_limit = (sp_row - 2) * 4;
   const int32_t col_lower_limit = (sp_col - 1) * 2;
@@ -263,9 +282,9 @@ __device__ inline bool cluster_candidate(
   // x / n and exact when n divides x, so its floor is the integer quotient
   const uint32_t cx = (uint32_t) fxq;
   const uint32_t cy = (uint32_t) fyq;
-  const float fx = fxq - (float) cx;
-  const float fy = fyq - (float) cy;
-  *out = make_hit(geometry, sensor, cx, cy, fx, fy);
+  out->fx = fxq - (float) cx;
+  out->fy = fyq - (float) cy;
+  out->key = cx | (cy << 10) | (sensor << 18);
   return true;
 }
 
@@ -325,6 +344,7 @@ __global__ void __launch_bounds__(kOrderBlock)
 constexpr uint32_t kDefaultEventWords = 6144;   // estimate: 24 KB of LDS per workgroup
 constexpr uint32_t kDefaultClusterWords = 8192; // clustering: 32 KB
 constexpr uint32_t kClusterEntries = kWave; // candidate entries per wave
+constexpr uint32_t kClusterSeeds = 6 * kWave; // staged clusters per wave: key, fx, fy for 128
 
 __device__ inline uint32_t sel4(uint32_t k, uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3)
 {
@@ -563,16 +583,20 @@ __global__ void __launch_bounds__(1024) estimate_input_size_kernel(velo_estimate
 // K5: masked clustering
 // ---------------------------------------------------------------------------
 // Per wave iteration over up to 64 SPs of a module:
-//  1. each lane classifies its SP: isolated (pattern table) or the candidate
-//     byte of a non-isolated SP;
-//  2. the candidates (low nibble first) are packed into the wave's entry
-//     list, so one growth pass serves up to 64 candidates whatever lane they
-//     came from;
+//  1. each lane classifies its SP: isolated (pattern table) or not;
+//  2. the candidates of the non-isolated SPs (low nibble first) form the
+//     wave's entries, one per lane, so one growth pass serves up to 64
+//     candidates whatever SP they came from.  They are read from the event's
+//     candidate list (written by estimate_input_size in bank, SP, nibble
+//     order) when it is given and staged, else derived from the raw banks;
 //  3. a wave prefix over the per-SP cluster counts gives each SP its slots,
 //     in bank, SP, cluster order.
+// cl / cl_mod: the event's candidate list in LDS and the first entry of each
+// module (53 values), or null.
 template<typename P>
 __device__ __forceinline__ void cluster_event(
-  P ev, const velo_masked_clustering_args& a, uint32_t e, uint32_t* block_words, uint32_t* entries)
+  P ev, const velo_masked_clustering_args& a, uint32_t e, uint32_t* block_words, uint32_t* entries,
+  uint32_t* seeds, const uint32_t* cl, const uint32_t* cl_mod)
 {
   const uint32_t waves = blockDim.x / kWave;
   const uint32_t wave = threadIdx.x / kWave;
@@ -591,7 +615,10 @@ __device__ __forceinline__ void cluster_event(
     ModuleBanks mb = module_banks(ev, n_banks, m);
     check_sorted(ev, mb);
     const uint32_t mstart = starts[m];
+    const uint32_t mend = starts[m + 1]; // the module's estimated slots end here
     uint32_t pos = 0;
+    uint32_t cptr = cl ? cl_mod[m] : 0u;            // next list entry of the module
+    const uint32_t cend = cl ? cl_mod[m + 1] : 0u;
     for (uint32_t jb = 0; jb < mb.total;) {
       const uint32_t j = jb + lane;
       uint32_t i = 0, k = 0, w = 0, c = 0, idx = 0;
@@ -602,40 +629,83 @@ __device__ __forceinline__ void cluster_event(
         const uint32_t n = sel4(k, mb.n0, mb.n1, mb.n2, mb.n3);
         w = ev[base + i];
         if (w & 0x80000000u) idx = a.dev_velo_sp_patterns[w & 0xFFu];
-        else
+        else if (!cl)
           c = candidates_from_pixels(((mb.sorted >> k) & 1u) ? neighbour_pixels_sorted(w, i, ev + base, n)
                                                              : neighbour_pixels(w, ev + base, n));
       }
-      uint32_t has_lo = (c & 0x0Fu) ? 1u : 0u;
-      uint32_t has_hi = (c & 0xF0u) ? 1u : 0u;
-      uint32_t n_ent;
-      const uint32_t epre = wave_prefix_small(has_lo + has_hi, &n_ent);
-      // this pass takes the SPs whose candidates fit in one wave of entries
-      // (at least 32 SPs); the rest start the next pass
-      const uint32_t end = epre + has_lo + has_hi;
-      if (end > kWave) {
-        live = false;
-        has_lo = has_hi = 0;
+      // per SP: first entry and number of entries; per entry: its SP
+      uint32_t first = 0, n_own = 0, taken, n_ent;
+      uint32_t src = 0, si = 0, sk = 0, kp = 0; // per entry: its SP's lane, SP index, bank, pixel
+      if (!cl) {
+        uint32_t has_lo = (c & 0x0Fu) ? 1u : 0u;
+        uint32_t has_hi = (c & 0xF0u) ? 1u : 0u;
+        first = wave_prefix_small(has_lo + has_hi, &n_ent);
+        // this pass takes the SPs whose candidates fit in one wave of
+        // entries (at least 32 SPs); the rest start the next pass
+        const uint32_t end = first + has_lo + has_hi;
+        if (end > kWave) {
+          live = false;
+          has_lo = has_hi = 0;
+        }
+        n_own = has_lo + has_hi;
+        taken = (uint32_t) __popcll(ballot(live));
+        n_ent = n_ent > kWave ? __shfl(end, taken - 1u, 64) : n_ent;
+        if (has_lo) entries[first] = lane;
+        if (has_hi) entries[first + has_lo] = lane | 64u;
+        wave_sync();
+        const uint32_t d = lane < n_ent ? entries[lane] : 0u;
+        src = d & 63u;
+        const uint32_t sc = __shfl(c, src, 64);
+        si = __shfl(i, src, 64);
+        sk = __shfl(k, src, 64);
+        kp = (d & 64u) ? c_candidate_ks[sc >> 4] + 2u : c_candidate_ks[sc & 0x0Fu];
       }
-      const uint32_t taken = (uint32_t) __popcll(ballot(live));
-      n_ent = n_ent > kWave ? __shfl(end, taken - 1u, 64) : n_ent;
+      else {
+        // entries of SPs [jb, jb + 64) from the list; when 64 entries do not
+        // reach past the pass, its SPs end before the SP of entry 64
+        const bool ok = cptr + lane < cend;
+        const uint32_t ent = ok ? cl[cptr + lane] : 0u;
+        si = ent >> 11;
+        sk = (ent >> 3) & 3u;
+        kp = ent & 7u;
+        const uint32_t epos = ok ? sel4(sk, 0u, mb.p1, mb.p2, mb.p3) + si : 0xFFFFFFFFu;
+        uint32_t limit = jb + kWave;
+        if (cptr + kWave < cend) {
+          const uint32_t e64 = cl[cptr + kWave];
+          const uint32_t p64 = sel4((e64 >> 3) & 3u, 0u, mb.p1, mb.p2, mb.p3) + (e64 >> 11);
+          limit = p64 < limit ? p64 : limit;
+        }
+        limit = limit < mb.total ? limit : mb.total;
+        limit = limit > jb ? limit : jb + 1; // a list that does not match the banks still ends
+        taken = limit - jb;
+        live = lane < taken;
+        const bool mine = epos < limit;
+        n_ent = (uint32_t) __popcll(ballot(mine));
+        src = mine ? epos - jb : 0u;
+        // the SP of each entry records its first entry and count
+        entries[lane] = 0u;
+        wave_sync();
+        const uint32_t prev = __shfl_up(src, 1, 64);
+        const bool lead = mine && (lane == 0 || prev != src);
+        const uint32_t next = __shfl_down(src, 1, 64);
+        const bool pair = lead && lane + 1 < n_ent && next == src;
+        if (lead) entries[src] = (lane + 1) | (pair ? 0x100u : 0u);
+        wave_sync();
+        if (live) {
+          const uint32_t info = entries[lane];
+          first = (info & 0xFFu) - 1u;
+          n_own = info ? 1u + (info >> 8) : 0u;
+        }
+        cptr += n_ent;
+      }
       const bool iso = live && (w & 0x80000000u);
-      if (has_lo) entries[epre] = lane;
-      if (has_hi) entries[epre + has_lo] = lane | 64u;
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
       // growth: lane t takes entry t
-      Hit r {};
-      const uint32_t d = lane < n_ent ? entries[lane] : 0u;
+      HitSeed r {};
       uint64_t acc = 0;
       if (n_ent) {
-        const uint32_t src = d & 63u;
-        const uint32_t si = __shfl(i, src, 64), sk = __shfl(k, src, 64), sc = __shfl(c, src, 64);
         bool good = false;
         if (lane < n_ent) {
-          const uint32_t kp = (d & 64u) ? c_candidate_ks[sc >> 4] + 2u : c_candidate_ks[sc & 0x0Fu];
           const uint32_t base = sel4(sk, mb.b0, mb.b1, mb.b2, mb.b3);
           const uint32_t n = sel4(sk, mb.n0, mb.n1, mb.n2, mb.n3);
           // the reference decodes the candidate's SP index as uint8_t (:179)
@@ -643,46 +713,59 @@ __device__ __forceinline__ void cluster_event(
           const uint32_t caddr = (cw & 0x007FFF00u) >> 8;
           const int32_t sp_row = (int32_t) (caddr & 0x3Fu), sp_col = (int32_t) (caddr >> 6);
           uint32_t pa[3];
-          if ((mb.sorted >> sk) & 1u) load_window_sorted(sp_row, sp_col, ev + base, n, pa);
+          if ((mb.sorted >> sk) & 1u) load_window_sorted(sp_row, sp_col, ev + base, n, si & 0xFFu, pa);
           else load_window(sp_row, sp_col, ev + base, n, pa);
-          good = cluster_candidate(pa[0], pa[1], pa[2], sp_row, sp_col, kp, a.dev_velo_geometry, 4 * m + sk, &r);
+          good = cluster_candidate(pa[0], pa[1], pa[2], sp_row, sp_col, kp, 4 * m + sk, &r);
         }
         acc = ballot(good);
       }
 
       // clusters per SP and their slots
-      const uint32_t lo_ok = has_lo ? (uint32_t) (acc >> epre) & 1u : 0u;
-      const uint32_t hi_ok = has_hi ? (uint32_t) (acc >> (epre + has_lo)) & 1u : 0u;
-      const uint32_t cnt = iso ? ((idx & 8u) ? 2u : 1u) : lo_ok + hi_ok;
+      const uint32_t own_ok = n_own ? (uint32_t) __popcll((acc >> first) & (n_own > 1 ? 3u : 1u)) : 0u;
+      const uint32_t cnt = iso ? ((idx & 8u) ? 2u : 1u) : own_ok;
+      // every cluster of the pass is staged at its slot relative to the
+      // pass (below 128), then transformed and stored one per lane
       uint32_t tot;
-      const uint32_t slot = mstart + pos + wave_prefix_small(cnt, &tot);
-      auto store = [&](uint32_t s, const Hit& h) {
-        xs[s] = h.x;
-        ys[s] = h.y;
-        zs[s] = h.z;
-        ids[s] = h.id;
+      const uint32_t rel = wave_prefix_small(cnt, &tot);
+      auto stage = [&](uint32_t t, uint32_t key, float fx, float fy) {
+        seeds[t] = key;
+        seeds[2 * kWave + t] = __float_as_uint(fx);
+        seeds[4 * kWave + t] = __float_as_uint(fy);
       };
       if (iso) {
         // isolated SP: pre-computed pattern (MaskedVeloClustering.cu:105-167)
         const uint32_t sp_addr = (w & 0x007FFF00u) >> 8, sp = w & 0xFFu;
-        const uint32_t sp_row = sp_addr & 0x3Fu, sp_col = sp_addr >> 6, sensor = 4 * m + k;
-        store(slot, make_hit(a.dev_velo_geometry, sensor, sp_col * 2u + ((idx >> 2) & 1u), sp_row * 4u + (idx & 3u),
-                             a.dev_velo_sp_fx[2 * sp], a.dev_velo_sp_fy[2 * sp]));
+        const uint32_t sp_row = sp_addr & 0x3Fu, sp_col = sp_addr >> 6, sensor = (4 * m + k) << 18;
+        stage(rel, (sp_col * 2u + ((idx >> 2) & 1u)) | ((sp_row * 4u + (idx & 3u)) << 10) | sensor,
+              a.dev_velo_sp_fx[2 * sp], a.dev_velo_sp_fy[2 * sp]);
         if (idx & 8u)
-          store(slot + 1, make_hit(a.dev_velo_geometry, sensor, sp_col * 2u + ((idx >> 6) & 1u),
-                                   sp_row * 4u + ((idx >> 4) & 3u), a.dev_velo_sp_fx[2 * sp + 1],
-                                   a.dev_velo_sp_fy[2 * sp + 1]));
+          stage(rel + 1, (sp_col * 2u + ((idx >> 6) & 1u)) | ((sp_row * 4u + ((idx >> 4) & 3u)) << 10) | sensor,
+                a.dev_velo_sp_fx[2 * sp + 1], a.dev_velo_sp_fy[2 * sp + 1]);
       }
       if (n_ent) {
-        // a high-nibble cluster follows its SP's low-nibble cluster when that was kept
-        const uint32_t s = __shfl(slot, d & 63u, 64);
-        const uint32_t lo = __shfl(lo_ok, d & 63u, 64);
-        if ((acc >> lane) & 1u) store(s + ((d & 64u) ? lo : 0u), r);
+        // an SP's second cluster follows its first when that one was kept
+        const uint32_t s = __shfl(rel, src, 64);
+        const uint32_t f = __shfl(first, src, 64);
+        const uint32_t after = lane > f ? (uint32_t) (acc >> f) & 1u : 0u;
+        if ((acc >> lane) & 1u) stage(s + after, r.key, r.fx, r.fy);
+      }
+      wave_sync();
+      for (uint32_t t = lane; t < tot; t += kWave) {
+        const uint32_t key = seeds[t];
+        const Hit h = make_hit(a.dev_velo_geometry, key >> 18, key & 0x3FFu, (key >> 10) & 0xFFu,
+                               __uint_as_float(seeds[2 * kWave + t]), __uint_as_float(seeds[4 * kWave + t]));
+        const uint32_t slot = mstart + pos + t;
+        if (slot < mend) {
+          xs[slot] = h.x;
+          ys[slot] = h.y;
+          zs[slot] = h.z;
+          ids[slot] = h.id;
+        }
       }
       pos += tot;
       jb += taken;
       // the entry list is rewritten by the next pass
-      __builtin_amdgcn_wave_barrier();
+      wave_sync();
     }
     if (lane == 0) a.dev_module_cluster_num[e * VELO_N_MODULES + m] = pos;
     if (pos > VELO_MAX_NUMHITS_IN_MODULE) status |= VELO_STATUS_MODULE_HITS_OVERFLOW;
@@ -711,15 +794,40 @@ masked_clustering_kernel(velo_masked_clustering_args a)
     const uint32_t* gev = reinterpret_cast<const uint32_t*>(a.dev_raw_input + off);
     // each wave's candidate entry list follows the event tile
     uint32_t* entries = lds + a.tuning.lds_tile + (threadIdx.x / kWave) * kClusterEntries;
+    // then each wave's staged clusters of one pass (3 x 128 words)
+    uint32_t* seeds = lds + a.tuning.lds_tile + (blockDim.x / kWave) * kClusterEntries + (threadIdx.x / kWave) * kClusterSeeds;
     if (threadIdx.x < 2) block_words[threadIdx.x] = 0;
+    // the event's candidate list follows the event in LDS when it is given,
+    // complete (below the list's capacity) and fits
+    const bool list = a.dev_cluster_candidates && a.dev_event_candidate_num;
+    const uint32_t ncand = list ? a.dev_event_candidate_num[e] : 0u;
     if (nwords <= a.tuning.lds_tile) {
       stage_words(lds, gev, nwords);
+      uint32_t* cl = nullptr;
+      uint32_t* cl_mod = nullptr;
+      if (list && ncand < VELO_MAX_CANDIDATES_EVENT &&
+          (size_t) nwords + ncand + VELO_N_MODULES + 1 <= a.tuning.lds_tile) {
+        cl = lds + nwords;
+        cl_mod = cl + ncand;
+        stage_words(cl, a.dev_cluster_candidates + (size_t) e * VELO_MAX_CANDIDATES_EVENT, ncand);
+        __syncthreads();
+        if (threadIdx.x <= VELO_N_MODULES) {
+          // first entry of module threadIdx.x (bank field bits 3-10, module = bank / 4)
+          uint32_t lo = 0, hi = ncand;
+          while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (((cl[mid] >> 5) & 0x3Fu) < threadIdx.x) lo = mid + 1;
+            else hi = mid;
+          }
+          cl_mod[threadIdx.x] = lo;
+        }
+      }
       __syncthreads();
-      cluster_event(static_cast<const uint32_t*>(lds), a, e, block_words, entries);
+      cluster_event(static_cast<const uint32_t*>(lds), a, e, block_words, entries, seeds, cl, cl_mod);
     }
     else {
       __syncthreads();
-      cluster_event(gev, a, e, block_words, entries);
+      cluster_event(gev, a, e, block_words, entries, seeds, nullptr, nullptr);
     }
   });
 }
@@ -891,7 +999,8 @@ int velo_masked_clustering_launch(const velo_masked_clustering_args* in, void* s
   const uint32_t block = block_of(a.tuning);
   if (!block) return (int) hipErrorInvalidValue;
   if (!a.tuning.lds_tile) a.tuning.lds_tile = kDefaultClusterWords;
-  const size_t shmem = ((size_t) a.tuning.lds_tile + (block / kWave) * kClusterEntries) * sizeof(uint32_t);
+  const size_t shmem =
+    ((size_t) a.tuning.lds_tile + (block / kWave) * (kClusterEntries + kClusterSeeds)) * sizeof(uint32_t);
   hipLaunchKernelGGL(masked_clustering_kernel, dim3(event_grid(a.tuning, a.n_events)), dim3(block), shmem, (hipStream_t) stream, a);
   return (int) hipGetLastError();
 }

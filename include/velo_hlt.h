@@ -188,9 +188,12 @@ int velo_prefix_sum_launch(const velo_prefix_sum_args* a, void* stream);
 /* ---- K5: masked_velo_clustering (MaskedVeloClustering.cu:48-464) ----
  * Clusters of module m are written to [start_m, start_m + num_m) in bank,
  * SP, cluster order (isolated SP: LUT cluster 0 then 1; otherwise the
- * low-nibble candidate then the high-nibble one).  Candidates are derived
- * from the raw banks; dev_cluster_candidates is accepted for signature
- * parity. */
+ * low-nibble candidate then the high-nibble one).  Candidates are read from
+ * dev_cluster_candidates / dev_event_candidate_num (estimate_input_size's
+ * list, as the reference does) when both are given and the event's list is
+ * below VELO_MAX_CANDIDATES_EVENT and fits in LDS beside the event; else
+ * (NULL, a full list, a large event) they are derived from the raw banks,
+ * with the same result. */
 typedef struct {
   const char* dev_raw_input;
   const uint32_t* dev_raw_input_offsets;

@@ -81,14 +81,20 @@ def prefix_sum(arr: np.ndarray, device_count=None) -> np.ndarray:
     return host(d).view(np.uint32)
 
 
-def clustering(events, offsets, starts, consts: nat.DeviceConstants, tuning=(0, 0)):
+def clustering(events, offsets, starts, consts: nat.DeviceConstants, tuning=(0, 0), cands=None):
+    """cands: (per-event candidate counts, candidate list) from
+    estimate_input_size, or None to derive the candidates from the banks."""
     n = len(offsets) - 1
     total = int(starts[n * N_MODULES])
     raw, off, st = dev(events), dev(offsets.view(np.int32)), dev(starts.view(np.int32))
     num, status = garbage_u32(n * N_MODULES), u32(n)
     cont = garbage_u32(6 * total + 1)
     c = consts.c
-    a = nat.MaskedClusteringArgs(raw.data_ptr(), off.data_ptr(), st.data_ptr(), num.data_ptr(), None, None,
+    cnum_p = cand_p = None
+    if cands is not None:
+        cnum_d, cand_d = dev(cands[0].view(np.int32)), dev(cands[1].view(np.int32))
+        cnum_p, cand_p = cnum_d.data_ptr(), cand_d.data_ptr()
+    a = nat.MaskedClusteringArgs(raw.data_ptr(), off.data_ptr(), st.data_ptr(), num.data_ptr(), cnum_p, cand_p,
                                  cont.data_ptr(), None, None, c.dev_velo_geometry, c.dev_velo_sp_patterns,
                                  c.dev_velo_sp_fx, c.dev_velo_sp_fy, status.data_ptr(), n, tuning_of(tuning))
     nat.launch("velo_masked_clustering_launch", a)

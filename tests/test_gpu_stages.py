@@ -144,8 +144,8 @@ def test_prefix_sum_sizes(stages, size):
     np.testing.assert_array_equal(got.astype(np.uint64), want)
 
 
-def _check_clustering(stages, consts, c: Chain, tuning=(0, 0)):
-    num, cont, st = stages.clustering(c.ev.events, c.ev.offsets, c.starts, consts, tuning)
+def _check_clustering(stages, consts, c: Chain, tuning=(0, 0), cands=None):
+    num, cont, st = stages.clustering(c.ev.events, c.ev.offsets, c.starts, consts, tuning, cands)
     n = c.n
     total = int(c.starts[n * N_MODULES])
     np.testing.assert_array_equal(num[: n * N_MODULES], c.nums)
@@ -170,6 +170,24 @@ def test_masked_clustering_unsorted_banks(stages, consts, chain_scrambled):
 @pytest.mark.parametrize("block,tile", [(64, 32), (512, 512)])
 def test_masked_clustering_launch_shapes(stages, consts, chain, block, tile):
     _check_clustering(stages, consts, chain, (block, tile))
+
+
+@pytest.mark.parametrize("name", ["chain", "chain_uneven", "chain_scrambled"])
+@pytest.mark.parametrize("block,tile", [(512, 0), (128, 2048)])
+def test_masked_clustering_from_candidate_list(stages, consts, request, name, block, tile):
+    """Candidates read from estimate_input_size's list give what the banks
+    give (tile 2048: the larger events do not fit beside their list and take
+    the bank path)."""
+    c = request.getfixturevalue(name)
+    _check_clustering(stages, consts, c, (block, tile), (c.cnum, c.cand))
+
+
+def test_masked_clustering_full_candidate_list(stages, consts, chain):
+    """An event whose list is at capacity may have lost candidates: it takes
+    the bank path, so nothing is lost."""
+    cnum = chain.cnum.copy()
+    cnum[::7] = 3000
+    _check_clustering(stages, consts, chain, (0, 0), (cnum, chain.cand))
 
 
 def _check_sort(stages, c: Chain, tuning=(0, 0)):
