@@ -121,8 +121,18 @@ __device__ inline uint32_t neighbour_pixels_sorted(uint32_t w, uint32_t i, P wor
       pixels |= ((osp & 0x01u) | ((osp & 0x10u) << 2)) << 4;
     }
   }
-  // right column: right-bottom (+63), right (+64), top-right (+65)
-  for (uint32_t j = lower_bound_address(words, n, addr + 63u); j < n; ++j) {
+  // right column: right-bottom (+63), right (+64), top-right (+65); the
+  // first SP at or above +63 is a few SPs after i (a binary search after
+  // four steps)
+  uint32_t j0 = i + 1;
+  for (int step = 0; j0 < n && sp_address(words[j0]) < addr + 63u; ++step) {
+    if (step == 4) {
+      j0 += lower_bound_address(words + j0, n - j0, addr + 63u);
+      break;
+    }
+    ++j0;
+  }
+  for (uint32_t j = j0; j < n; ++j) {
     const uint32_t o = words[j];
     const uint32_t oa = sp_address(o);
     if (oa > addr + 65u) break;
