@@ -176,12 +176,19 @@ __device__ __forceinline__ void consolidate_event(
     static_cast<const uint8_t*>(a.dev_tracks) + (size_t) e * VELO_MAX_TRACKS * VELO_TRACK_HITS_BYTES;
   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
     const uint16_t* t = reinterpret_cast<const uint16_t*>(tracks + (size_t) i * VELO_TRACK_HITS_BYTES);
-    const uint32_t nh = t[0];
+    // the whole row (count and 26 hit slots) is read at once, so the hit
+    // loops below wait for one memory round trip instead of one per hit
+    uint32_t hh[VELO_MAX_TRACK_SIZE];
+#pragma unroll
+    for (uint32_t h = 0; h < VELO_MAX_TRACK_SIZE; ++h) hh[h] = t[1 + h];
+    const uint32_t nh = t[0] < VELO_MAX_TRACK_SIZE ? t[0] : VELO_MAX_TRACK_SIZE;
     const uint32_t ho = a.dev_velo_track_hit_number[toff + i];
     float s0 = 0.0f, sx = 0.0f, sz = 0.0f, sxz = 0.0f, sz2 = 0.0f;
     float u0 = 0.0f, uy = 0.0f, uz = 0.0f, uyz = 0.0f, uz2 = 0.0f;
-    for (uint32_t h = 0; h < nh; ++h) {
-      const uint32_t hi = t[1 + h];
+#pragma unroll
+    for (uint32_t h = 0; h < VELO_MAX_TRACK_SIZE; ++h) {
+      if (h >= nh) break;
+      const uint32_t hi = hh[h];
       const float x = hv.x[hi], y = hv.y[hi], z = hv.z[hi];
       cx[ho + h] = x;
       cy[ho + h] = y;
@@ -211,7 +218,7 @@ __device__ __forceinline__ void consolidate_event(
     const float ty = (uyz * u0 - uy * uz) * denu;
     float y0 = (uy * uz2 - uyz * uz) * denu;
     const float zb = -(x0 * tx + y0 * ty) / (tx * tx + ty * ty);
-    const float z_first = hv.z[t[1]];
+    const float z_first = hv.z[hh[0]];
     const bool bwd = zb > z_first;
     const float m00 = s0;
     const float m11 = u0;
@@ -223,8 +230,10 @@ __device__ __forceinline__ void consolidate_event(
     const float den31 = 1.0f / (m33 * m11 - m31 * m31);
     float ch = 0.0f;
     int ndof = -4;
-    for (uint32_t h = 0; h < nh; ++h) {
-      const uint32_t hi = t[1 + h];
+#pragma unroll
+    for (uint32_t h = 0; h < VELO_MAX_TRACK_SIZE; ++h) {
+      if (h >= nh) break;
+      const uint32_t hi = hh[h];
       const float z = hv.z[hi];
       const float x = x0 + tx * z;
       const float y = y0 + ty * z;
