@@ -22,7 +22,7 @@
 
 namespace velo {
 
-constexpr uint32_t kUsedBits = 16384;
+constexpr uint32_t kUsedBits = 8192;
 constexpr uint32_t kUsedWords = kUsedBits / 32;
 constexpr uint32_t kTtfTrackletFlag = 0x80000000u;
 
@@ -277,6 +277,20 @@ struct SeedScratch {
   uint64_t best[kWave];
 };
 
+// h1 hits staged for a seeding batch with their h0 and h2 windows (first |
+// last << 16 as fill_candidates writes them, read once from global memory).
+struct SeedStage {
+  uint16_t h1[2 * kWave];
+  uint32_t c0[2 * kWave];
+  uint32_t c2[2 * kWave];
+};
+__device__ inline uint32_t window_pair(const int16_t* c, uint32_t h)
+{
+  return *reinterpret_cast<const uint32_t*>(c + 2 * h);
+}
+__device__ inline int window_first(uint32_t w) { return (int) (int16_t) (w & 0xFFFFu); }
+__device__ inline int window_last(uint32_t w) { return (int) (int16_t) (w >> 16); }
+
 // Items of one seeding batch must fit the key's 17-bit item field.
 constexpr uint32_t kMaxSeedItems = 1u << 17;
 
@@ -286,7 +300,7 @@ constexpr uint32_t kMaxSeedItems = 1u << 17;
 // bit patterns order like the floats; among equal scatters the lower item
 // (smaller h0), then the first h2, wins, as in seed_best's loop order.
 __device__ inline uint64_t seed_item(
-  uint32_t h1, int h0, uint32_t item, const EventHits& H, const int16_t* h2c, const UsedSet& used)
+  uint32_t h1, int h0, uint32_t item, const EventHits& H, int f2, int l2, const UsedSet& used)
 {
   if (used.test(h0)) return ~0ull;
   const float h1x = H.x(h1), h1y = H.y(h1), h1z = H.z(h1);
@@ -294,7 +308,6 @@ __device__ inline uint64_t seed_item(
   const float dmax = kMaxSlope * (h0z - h1z);
   if (!(fabsf(h1x - h0x) < dmax && fabsf(h1y - h0y) < dmax)) return ~0ull;
   const float dz10 = h1z - h0z;
-  const int f2 = h2c[2 * h1], l2 = h2c[2 * h1 + 1];
   float best = kFltMax;
   uint32_t bb2 = 0;
   for (int h2 = f2; h2 < l2; ++h2) {
@@ -325,16 +338,16 @@ __device__ inline uint64_t seed_item(
 // The result per h1 equals seed_best's, which runs instead (one h1 per
 // lane) for a batch with more pairs than the key can number.
 __device__ inline uint32_t seed_batch(
-  const uint16_t* stage, uint32_t k, const EventHits& H, const int16_t* h0c, const int16_t* h2c, const UsedSet& used,
+  const SeedStage& stage, uint32_t k, const EventHits& H, const int16_t* h0c, const int16_t* h2c, const UsedSet& used,
   const Ring& ring, SearchState& st, SeedScratch& sc)
 {
   const uint32_t lane = lane_id();
   uint32_t h1 = 0, c = 0;
   int f0 = 0;
   if (lane < k) {
-    h1 = stage[lane];
-    f0 = h0c[2 * h1];
-    const int l0 = h0c[2 * h1 + 1];
+    h1 = stage.h1[lane];
+    f0 = window_first(stage.c0[lane]);
+    const int l0 = window_last(stage.c0[lane]);
     c = l0 > f0 ? (uint32_t) (l0 - f0) : 0u;
   }
   const uint32_t inc = wave_inclusive_scan(c);
@@ -363,9 +376,10 @@ __device__ inline uint32_t seed_batch(
         else hi = mid - 1;
       }
       const uint32_t j = lo;
-      const uint32_t hj = stage[j];
-      const int h0 = h0c[2 * hj] + (int) (i - sc.item_begin[j]);
-      const uint64_t key = seed_item(hj, h0, i, H, h2c, used);
+      const uint32_t hj = stage.h1[j];
+      const int h0 = window_first(stage.c0[j]) + (int) (i - sc.item_begin[j]);
+      const uint32_t c2 = stage.c2[j];
+      const uint64_t key = seed_item(hj, h0, i, H, window_first(c2), window_last(c2), used);
       if (key != ~0ull) atomicMin(&sc.best[j], key);
     }
   }
@@ -388,7 +402,7 @@ __device__ inline uint32_t seed_batch(
 
 // TrackSeeding.cu:7-363: h1 candidates of module_data[2] then [3].
 __device__ inline uint32_t seeding(
-  const ModuleData* md, uint16_t* stage, const EventHits& H, const int16_t* h0c, const int16_t* h2c,
+  const ModuleData* md, SeedStage& stage, const EventHits& H, const int16_t* h0c, const int16_t* h2c,
   const UsedSet& used, const Ring& ring, SearchState& st, SeedScratch& sc)
 {
   const uint32_t lane = lane_id();
@@ -400,9 +414,19 @@ __device__ inline uint32_t seeding(
       const uint32_t r = base + lane;
       const uint32_t h1 = side_start + r;
       bool cand = false;
-      if (r < side_num) cand = !used.test((int) h1) && h0c[2 * h1] != -1 && h2c[2 * h1] != -1;
+      uint32_t c0 = 0, c2 = 0;
+      if (r < side_num) {
+        c0 = window_pair(h0c, h1);
+        c2 = window_pair(h2c, h1);
+        cand = !used.test((int) h1) && window_first(c0) != -1 && window_first(c2) != -1;
+      }
       const uint64_t m = ballot(cand);
-      if (cand) stage[staged + count_below(m)] = (uint16_t) h1;
+      if (cand) {
+        const uint32_t slot = staged + count_below(m);
+        stage.h1[slot] = (uint16_t) h1;
+        stage.c0[slot] = c0;
+        stage.c2[slot] = c2;
+      }
       staged += (uint32_t) __popcll(m);
       wave_sync();
       if (staged >= kWave) {
@@ -410,9 +434,18 @@ __device__ inline uint32_t seeding(
         wave_sync();
         const uint32_t rest = staged - kWave;
         uint16_t carry = 0;
-        if (lane < rest) carry = stage[kWave + lane];
+        uint32_t carry0 = 0, carry2 = 0;
+        if (lane < rest) {
+          carry = stage.h1[kWave + lane];
+          carry0 = stage.c0[kWave + lane];
+          carry2 = stage.c2[kWave + lane];
+        }
         wave_sync();
-        if (lane < rest) stage[lane] = carry;
+        if (lane < rest) {
+          stage.h1[lane] = carry;
+          stage.c0[lane] = carry0;
+          stage.c2[lane] = carry2;
+        }
         staged = rest;
         wave_sync();
       }
@@ -551,7 +584,7 @@ constexpr uint32_t kDefaultWindowHits = 512;
 __global__ void __launch_bounds__(64) search_by_triplet_kernel(velo_search_by_triplet_args a)
 {
   __shared__ uint32_t used_bits[kUsedWords];
-  __shared__ uint16_t stage[2 * kWave];
+  __shared__ SeedStage stage;
   __shared__ SeedScratch seed_scratch;
 
   const uint32_t N = a.n_events;
