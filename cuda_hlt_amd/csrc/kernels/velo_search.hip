@@ -206,8 +206,27 @@ __device__ inline void forward_window(
   *l = last;
 }
 
+// The first kTtfLds entries of a step stay in LDS (double-buffered: the
+// step's entries and those pushed for the next step); the rest go through
+// the global ring.
+constexpr uint32_t kTtfLds = 128;
+
 struct Ring {
   uint32_t* base; // 2 words per entry
+  uint2* cur;     // LDS: entries [0, kTtfLds) of the step being forwarded
+  uint2* next;    // LDS: entries [0, kTtfLds) pushed for the next step
+  __device__ uint2 entry(uint32_t begin, uint32_t idx) const { return idx < kTtfLds ? cur[idx] : get(begin + idx); }
+  __device__ void push(uint32_t begin, uint32_t slot, uint32_t w0, uint32_t w1) const
+  {
+    if (slot < kTtfLds) next[slot] = make_uint2(w0, w1);
+    else put(begin + slot, w0, w1);
+  }
+  __device__ void swap()
+  {
+    uint2* t = cur;
+    cur = next;
+    next = t;
+  }
   __device__ uint2 get(uint32_t pos) const
   {
     const uint32_t p = pos % VELO_TTF_MODULO;
@@ -259,7 +278,7 @@ __device__ inline void push_ttf(bool want, uint32_t w0, uint32_t w1, const Ring&
   const uint64_t m = ballot(want);
   if (!m) return;
   const uint32_t slot = st.next_n + count_below(m);
-  if (want && slot < st.cap) ring.put(st.next_begin + slot, w0, w1);
+  if (want && slot < st.cap) ring.push(st.next_begin, slot, w0, w1);
   const uint32_t cnt = (uint32_t) __popcll(m);
   if (st.next_n + cnt > st.cap) {
     st.status |= VELO_STATUS_TTF_OVERFLOW;
@@ -466,7 +485,7 @@ __device__ inline void forwarding(
     const uint32_t idx = base + lane;
     const bool active = idx < cur_n;
     uint2 ent = make_uint2(0, 0);
-    if (active) ent = ring.get(cur_begin + idx);
+    if (active) ent = ring.entry(cur_begin, idx);
     const bool tracklet = (ent.y & kTtfTrackletFlag) != 0u;
     const uint32_t skipped = (ent.y >> 28) & 7u;
     const uint32_t nh = (ent.y >> 16) & 0x1Fu;
@@ -585,6 +604,7 @@ __global__ void __launch_bounds__(64) search_by_triplet_kernel(velo_search_by_tr
 {
   __shared__ uint32_t used_bits[kUsedWords];
   __shared__ SeedStage stage;
+  __shared__ uint2 ttf_lds[2 * kTtfLds];
   __shared__ SeedScratch seed_scratch;
 
   const uint32_t N = a.n_events;
@@ -614,7 +634,7 @@ __global__ void __launch_bounds__(64) search_by_triplet_kernel(velo_search_by_tr
     const int16_t* h2c = a.dev_h2_candidates + 2 * (size_t) off;
     uint8_t* tracks = static_cast<uint8_t*>(a.dev_tracks) + (size_t) e * VELO_MAX_TRACKS * VELO_TRACK_HITS_BYTES;
     uint8_t* weak = static_cast<uint8_t*>(a.dev_weak_tracks) + (size_t) e * VELO_MAX_WEAK_TRACKS * VELO_TRACKLET_BYTES;
-    Ring ring {a.dev_tracks_to_follow + (size_t) e * VELO_TTF_MODULO * VELO_TTF_ENTRY_WORDS};
+    Ring ring {a.dev_tracks_to_follow + (size_t) e * VELO_TTF_MODULO * VELO_TTF_ENTRY_WORDS, ttf_lds, ttf_lds + kTtfLds};
 
     UsedSet used {used_bits, a.dev_hit_used + off, ev_hits <= kUsedBits};
     if (used.in_lds) {
@@ -631,6 +651,7 @@ __global__ void __launch_bounds__(64) search_by_triplet_kernel(velo_search_by_tr
     load_modules(md, first, starts, nums, off, a.dev_velo_module_zs);
     stage_window(first);
     uint32_t prev_tracklets = seeding(md, stage, H, h0c, h2c, used, ring, st, seed_scratch);
+    ring.swap();
     uint32_t cur_begin = 0, cur_n = st.next_n;
     first -= 2;
     while (first >= 4) {
@@ -646,6 +667,7 @@ __global__ void __launch_bounds__(64) search_by_triplet_kernel(velo_search_by_tr
       prev_tracklets = tl;
       cur_begin = st.next_begin;
       cur_n = st.next_n;
+      ring.swap();
       first -= 2;
       wave_sync();
     }
@@ -654,7 +676,7 @@ __global__ void __launch_bounds__(64) search_by_triplet_kernel(velo_search_by_tr
     for (uint32_t base = 0; base < cur_n; base += kWave) {
       const uint32_t idx = base + lane;
       uint2 ent = make_uint2(0, 0);
-      if (idx < cur_n) ent = ring.get(cur_begin + idx);
+      if (idx < cur_n) ent = ring.entry(cur_begin, idx);
       const bool tl = idx < cur_n && (ent.y & kTtfTrackletFlag);
       append_weak(tl, ent.y & 0xFFFFu, ent.x & 0xFFFFu, ent.x >> 16, weak, st);
     }
