@@ -59,53 +59,69 @@ __device__ inline float launder(float v)
   return v;
 }
 
+// The window is a ring over hit numbers: hit h sits at h mod cap (cap a
+// power of two), so consecutive steps, whose windows share four of their
+// six modules, copy only the hits they do not share.
 struct EventHits {
   const float *gx, *gy, *gz, *gphi; // event-relative global slots
-  uint32_t cap;                     // window capacity (hits)
+  uint32_t cap;                     // window capacity (hits), a power of two
   uint32_t lo, n;                   // window = hits [lo, lo + n)
+  __device__ uint32_t at(uint32_t h, uint32_t r) const { return r < n ? (h & (cap - 1u)) : 0u; }
   __device__ float x(uint32_t h) const
   {
     const uint32_t r = h - lo;
-    const uint32_t rr = r < n ? r : 0u;
-    float v = launder(s_window[rr]);
+    float v = launder(s_window[at(h, r)]);
     if (r >= n) v = gx[h];
     return v;
   }
   __device__ float y(uint32_t h) const
   {
     const uint32_t r = h - lo;
-    const uint32_t rr = r < n ? r : 0u;
-    float v = launder(s_window[cap + rr]);
+    float v = launder(s_window[cap + at(h, r)]);
     if (r >= n) v = gy[h];
     return v;
   }
   __device__ float z(uint32_t h) const
   {
     const uint32_t r = h - lo;
-    const uint32_t rr = r < n ? r : 0u;
-    float v = launder(s_window[2 * cap + rr]);
+    float v = launder(s_window[2 * cap + at(h, r)]);
     if (r >= n) v = gz[h];
     return v;
   }
   __device__ float phi(uint32_t h) const
   {
     const uint32_t r = h - lo;
-    const uint32_t rr = r < n ? r : 0u;
-    float v = launder(s_window[3 * cap + rr]);
+    float v = launder(s_window[3 * cap + at(h, r)]);
     if (r >= n) v = gphi[h];
     return v;
   }
-  // Copies hits [lo_, hi_) into the window when they fit (wave-wide).
+  __device__ void copy(uint32_t from, uint32_t to)
+  {
+    for (uint32_t h = from + lane_id(); h < to; h += kWave) {
+      const uint32_t q = h & (cap - 1u);
+      s_window[q] = gx[h];
+      s_window[cap + q] = gy[h];
+      s_window[2 * cap + q] = gz[h];
+      s_window[3 * cap + q] = gphi[h];
+    }
+  }
+  // Makes hits [lo_, hi_) the window when they fit, copying those not in the
+  // current window (wave-wide).
   __device__ void stage(uint32_t lo_, uint32_t hi_)
   {
-    lo = lo_;
-    n = hi_ - lo_ <= cap ? hi_ - lo_ : 0u;
-    for (uint32_t i = lane_id(); i < n; i += kWave) {
-      s_window[i] = gx[lo + i];
-      s_window[cap + i] = gy[lo + i];
-      s_window[2 * cap + i] = gz[lo + i];
-      s_window[3 * cap + i] = gphi[lo + i];
+    const uint32_t old_lo = lo, old_hi = lo + n;
+    const uint32_t want = hi_ - lo_ <= cap ? hi_ - lo_ : 0u;
+    if (want) {
+      if (n == 0 || hi_ <= old_lo || lo_ >= old_hi) {
+        copy(lo_, hi_);
+      }
+      else {
+        if (lo_ < old_lo) copy(lo_, old_lo);
+        if (hi_ > old_hi) copy(old_hi, hi_);
+      }
     }
+    lo = lo_;
+    n = want;
     wave_sync();
   }
 };
@@ -600,6 +616,16 @@ __device__ inline void forwarding(
 
 constexpr uint32_t kDefaultWindowHits = 512;
 
+// Window capacity for a requested tile: the largest power of two not above
+// it (the window is a ring indexed by hit number mod capacity).
+static uint32_t window_hits(uint32_t lds_tile)
+{
+  const uint32_t t = lds_tile ? lds_tile : kDefaultWindowHits;
+  uint32_t p = 1;
+  while (p <= t / 2) p *= 2;
+  return p;
+}
+
 __global__ void __launch_bounds__(64) search_by_triplet_kernel(velo_search_by_triplet_args a)
 {
   __shared__ uint32_t used_bits[kUsedWords];
@@ -820,7 +846,7 @@ int velo_search_by_triplet_launch(const velo_search_by_triplet_args* a, void* st
   if (a->n_events == 0) return 0;
   if (!schedule_ok(a->tuning)) return (int) hipErrorInvalidValue;
   velo_search_by_triplet_args args = *a;
-  if (!args.tuning.lds_tile) args.tuning.lds_tile = kDefaultWindowHits;
+  args.tuning.lds_tile = window_hits(args.tuning.lds_tile);
   hipLaunchKernelGGL(search_by_triplet_kernel, dim3(event_grid(args.tuning, args.n_events)), dim3(kWave), 4 * sizeof(float) * args.tuning.lds_tile,
                      (hipStream_t) stream, args);
   return (int) hipGetLastError();
@@ -828,7 +854,7 @@ int velo_search_by_triplet_launch(const velo_search_by_triplet_args* a, void* st
 
 uint32_t velo_search_by_triplet_resident_workgroups(uint32_t lds_tile)
 {
-  const uint32_t tile = lds_tile ? lds_tile : kDefaultWindowHits;
+  const uint32_t tile = window_hits(lds_tile);
   int device = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&device) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;

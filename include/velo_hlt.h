@@ -243,8 +243,11 @@ int velo_fill_candidates_launch(const velo_fill_candidates_args* a, void* stream
  * Tracks are numbered in forwarding order; seeding takes h1 candidates of
  * module_data[2] then [3] in index order and, for equal scatter, the
  * lexicographically smallest (h0, h2).  dev_tracks_to_follow holds
- * N * 2000 * VELO_TTF_ENTRY_WORDS words; dev_tracklets and dev_rel_indices
- * are not used (tracklet hits travel inside the tracks-to-follow entries). */
+ * N * 2000 * VELO_TTF_ENTRY_WORDS words (a step's first 128 entries stay
+ * in LDS and are not written there); dev_tracklets and dev_rel_indices are
+ * not used (tracklet hits travel inside the tracks-to-follow entries).
+ * tuning.lds_tile: hits of the LDS window, rounded down to a power of two
+ * (0 = 512). */
 typedef struct {
   const uint32_t* dev_velo_cluster_container;
   const uint32_t* dev_module_cluster_start;
