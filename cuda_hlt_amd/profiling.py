@@ -108,6 +108,15 @@ def read_trace(path: str | Path) -> list[dict]:
     return sorted(out, key=lambda r: -r["total_us"])
 
 
+def read_dispatches(path: str | Path, kernel: str) -> list[float]:
+    """Durations (us) of every dispatch of `kernel`, in dispatch order."""
+    f = _find(path, "kernel_trace.csv")
+    with open(f, newline="") as fh:
+        rows = [r for r in csv.DictReader(fh) if short_name(r["Kernel_Name"]) == kernel]
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    return [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in rows]
+
+
 def _header(title: str, command: str) -> list[str]:
     return [f"# {title}", "", "Command (run from the repository root):", "", "```", command, "```", ""]
 
@@ -120,12 +129,20 @@ def stats_markdown(path, title, command) -> str:
     return "\n".join(lines) + "\n"
 
 
-def trace_markdown(path, title, command) -> str:
+def trace_markdown(path, title, command, per_dispatch=None, first=0) -> str:
+    """per_dispatch: a kernel whose dispatches are also listed one by one,
+    the first `first` of them (the plain sequence's) averaged apart."""
     lines = _header(title, command) + ["| kernel | workgroups | workgroup size | dispatches | average us | total us |",
                                        "|---|---|---|---|---|---|"]
     for r in read_trace(path):
         lines.append(f"| `{r['kernel']}` | {r['workgroups']} | {r['workgroup_size']} | {r['dispatches']} | "
                      f"{r['average_us']:.1f} | {r['total_us']:.1f} |")
+    if per_dispatch:
+        d = read_dispatches(path, per_dispatch)
+        lines += ["", f"`{per_dispatch}` per dispatch, in order (us): " + ", ".join(f"{x:.1f}" for x in d) + "."]
+        if 0 < first <= len(d):
+            lines.append(f"Mean of the first {first} (the plain sequence): {sum(d[:first]) / first:.1f} us; "
+                         f"of the rest: {sum(d[first:]) / max(1, len(d) - first):.1f} us.")
     return "\n".join(lines) + "\n"
 
 
@@ -287,6 +304,9 @@ def main(argv=None):
         p.add_argument("--csv", required=True)
         p.add_argument("--command", required=True)
         p.add_argument("--title", required=True)
+        if name == "trace":
+            p.add_argument("--per-dispatch", default=None, help="kernel whose dispatches are listed one by one")
+            p.add_argument("--first", type=int, default=0, help="dispatches of the plain sequence (averaged apart)")
     p = sub.add_parser("traffic")
     p.add_argument("--fetch", required=True)
     p.add_argument("--write", required=True)
@@ -309,7 +329,7 @@ def main(argv=None):
     elif a.cmd == "counters":
         sys.stdout.write(counters_markdown(a.csv, a.title, a.command))
     elif a.cmd == "trace":
-        sys.stdout.write(trace_markdown(a.csv, a.title, a.command))
+        sys.stdout.write(trace_markdown(a.csv, a.title, a.command, a.per_dispatch, a.first))
     elif a.cmd == "lds":
         t = lds_occupancy(a.csv, a.trace)
         t["command"] = a.command
