@@ -396,6 +396,15 @@ __device__ inline ModuleBanks module_banks(P ev, uint32_t n_banks, uint32_t m)
   bank_of(ev, n_banks, 4 * m + 1, &mb.b1, &mb.n1, &mb.status);
   bank_of(ev, n_banks, 4 * m + 2, &mb.b2, &mb.n2, &mb.status);
   bank_of(ev, n_banks, 4 * m + 3, &mb.b3, &mb.n3, &mb.status);
+  // the same on every lane: kept in scalar registers
+  mb.b0 = __builtin_amdgcn_readfirstlane(mb.b0);
+  mb.b1 = __builtin_amdgcn_readfirstlane(mb.b1);
+  mb.b2 = __builtin_amdgcn_readfirstlane(mb.b2);
+  mb.b3 = __builtin_amdgcn_readfirstlane(mb.b3);
+  mb.n0 = __builtin_amdgcn_readfirstlane(mb.n0);
+  mb.n1 = __builtin_amdgcn_readfirstlane(mb.n1);
+  mb.n2 = __builtin_amdgcn_readfirstlane(mb.n2);
+  mb.n3 = __builtin_amdgcn_readfirstlane(mb.n3);
   mb.p1 = mb.n0;
   mb.p2 = mb.p1 + mb.n1;
   mb.p3 = mb.p2 + mb.n2;
@@ -414,15 +423,27 @@ __device__ inline void check_sorted(P ev, ModuleBanks& mb)
   uint32_t bad = 0;
   for (uint32_t jb = 0; jb < mb.total; jb += kWave) {
     const uint32_t j = jb + lane_id();
+    bool b = false;
     if (j < mb.total) {
       uint32_t k;
       const uint32_t i = bank_slot(mb, j, &k);
       const uint32_t base = sel4(k, mb.b0, mb.b1, mb.b2, mb.b3);
       const uint32_t n = sel4(k, mb.n0, mb.n1, mb.n2, mb.n3);
-      if (i + 1 < n && sp_address(ev[base + i]) >= sp_address(ev[base + i + 1])) bad |= 1u << k;
+      b = i + 1 < n && sp_address(ev[base + i]) >= sp_address(ev[base + i + 1]);
+    }
+    const uint64_t m = ballot(b);
+    if (m) {
+      // the lanes of bank k are the module positions [p_k, p_k+1) - jb
+      const uint32_t bounds[5] = {0u, mb.p1, mb.p2, mb.p3, mb.total};
+      for (uint32_t k = 0; k < 4; ++k) {
+        const uint32_t lo = bounds[k] > jb ? bounds[k] - jb : 0u;
+        const uint32_t hi = bounds[k + 1] > jb ? bounds[k + 1] - jb : 0u;
+        if (lo >= kWave || hi <= lo) continue;
+        const uint64_t upto = hi >= kWave ? ~0ull : ((1ull << hi) - 1ull);
+        if (m & upto & ~((1ull << lo) - 1ull)) bad |= 1u << k;
+      }
     }
   }
-  for (int o = 32; o > 0; o >>= 1) bad |= __shfl_xor(bad, o, 64);
   mb.sorted &= ~bad;
 }
 
