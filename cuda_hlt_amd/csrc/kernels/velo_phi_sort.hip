@@ -79,7 +79,9 @@ __device__ inline void rank_module(const uint32_t* keys, uint16_t* perm, uint32_
   wave_sync();
 }
 
-__global__ void __launch_bounds__(1024) calculate_phi_and_sort_kernel(velo_calculate_phi_and_sort_args a)
+// At most 96 VGPRs: five waves per SIMD, as many as the LDS of a
+// 256-thread workgroup allows.
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5))) calculate_phi_and_sort_kernel(velo_calculate_phi_and_sort_args a)
 {
   extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
   for_each_event(a.tuning, a.n_events, [&](const uint32_t e) {
@@ -149,38 +151,38 @@ __global__ void __launch_bounds__(1024) calculate_phi_and_sort_kernel(velo_calcu
         }
       }
       else if (n <= kSortCap) {
-        for (uint32_t i = lane; i < n; i += kWave)
-          keys[i] = phi_key(module_phi(m, __uint_as_float(X[s + i]), __uint_as_float(Y[s + i])));
-        wave_sync();
-        rank_module(keys, perm, n, lane);
-        for (uint32_t p = lane; p < n; p += kWave) {
-          const uint32_t i = perm[p];
-          uint32_t bits = key_phi_bits(keys[i]);
-          if (bits == 0u) bits = __float_as_uint(module_phi(m, __uint_as_float(X[s + i]), __uint_as_float(Y[s + i])));
-          a.dev_hit_permutation[s + p] = s + i;
-          PHI[s + p] = bits;
-        }
+        // each lane keeps its hits' x, y, z, id and phi in registers (one
+        // coalesced read) and scatters them to their sorted slots
         uint32_t vx[kSortChunks], vy[kSortChunks], vz[kSortChunks], vid[kSortChunks];
+        float vp[kSortChunks];
   #pragma unroll
         for (uint32_t c = 0; c < kSortChunks; ++c) {
-          const uint32_t p = c * kWave + lane;
-          if (p < n) {
-            const uint32_t i = s + perm[p];
-            vx[c] = X[i];
-            vy[c] = Y[i];
-            vz[c] = Z[i];
-            vid[c] = ID[i];
+          const uint32_t i = c * kWave + lane;
+          if (i < n) {
+            vx[c] = X[s + i];
+            vy[c] = Y[s + i];
+            vz[c] = Z[s + i];
+            vid[c] = ID[s + i];
+            vp[c] = module_phi(m, __uint_as_float(vx[c]), __uint_as_float(vy[c]));
+            keys[i] = phi_key(vp[c]);
           }
         }
-        wave_sync(); // every read of the module is done before slots 0-2 are overwritten
+        wave_sync();
+        rank_module(keys, perm, n, lane);
+        // the keys' slots take the inverse permutation: keys[i] = sorted position of hit i
+        for (uint32_t p = lane; p < n; p += kWave) keys[perm[p]] = p;
+        wave_sync();
   #pragma unroll
         for (uint32_t c = 0; c < kSortChunks; ++c) {
-          const uint32_t p = c * kWave + lane;
-          if (p < n) {
-            TMP[s + p] = vx[c];
-            X[s + p] = vy[c];
-            Y[s + p] = vz[c];
-            Z[s + p] = vid[c];
+          const uint32_t i = c * kWave + lane;
+          if (i < n) {
+            const uint32_t r = s + keys[i];
+            a.dev_hit_permutation[r] = s + i;
+            PHI[r] = __float_as_uint(vp[c]);
+            TMP[r] = vx[c];
+            X[r] = vy[c];
+            Y[r] = vz[c];
+            Z[r] = vid[c];
           }
         }
       }
