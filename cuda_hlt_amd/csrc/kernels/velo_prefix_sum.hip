@@ -278,8 +278,11 @@ __device__ __forceinline__ void consolidate_event(
 // at most kFilterBlock threads per workgroup.
 constexpr uint32_t kFilterBlock = 256;
 
+// The plain instance at most 80 VGPRs: six waves per SIMD, as many as its
+// LDS allows for 512-thread workgroups.
 template<bool Filter>
-__global__ void __launch_bounds__(Filter ? kFilterBlock : 1024) consolidate_kernel(velo_consolidate_velo_tracks_args a)
+__global__ void __launch_bounds__(Filter ? kFilterBlock : 1024) __attribute__((amdgpu_waves_per_eu(Filter ? 1 : 6)))
+consolidate_kernel(velo_consolidate_velo_tracks_args a)
 {
   extern __shared__ uint32_t s_hits[];
   const uint32_t N = a.n_events;
