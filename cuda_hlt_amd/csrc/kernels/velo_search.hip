@@ -95,6 +95,34 @@ struct EventHits {
     if (r >= n) v = gphi[h];
     return v;
   }
+  // Hits of the step's six modules (seeding's h0, h1, h2 and forwarding's
+  // candidates): in the window whenever it holds the step (n > 0), so only
+  // the window's presence is tested, once per step on every lane alike.
+  __device__ uint32_t wat(uint32_t h) const { return n ? (h & (cap - 1u)) : 0u; }
+  __device__ float wx(uint32_t h) const
+  {
+    float v = launder(s_window[wat(h)]);
+    if (!n) v = gx[h];
+    return v;
+  }
+  __device__ float wy(uint32_t h) const
+  {
+    float v = launder(s_window[cap + wat(h)]);
+    if (!n) v = gy[h];
+    return v;
+  }
+  __device__ float wz(uint32_t h) const
+  {
+    float v = launder(s_window[2 * cap + wat(h)]);
+    if (!n) v = gz[h];
+    return v;
+  }
+  __device__ float wphi(uint32_t h) const
+  {
+    float v = launder(s_window[3 * cap + wat(h)]);
+    if (!n) v = gphi[h];
+    return v;
+  }
   __device__ void copy(uint32_t from, uint32_t to)
   {
     for (uint32_t h = from + lane_id(); h < to; h += kWave) {
@@ -152,7 +180,7 @@ __device__ inline float seed_best(
   uint32_t h1, const EventHits& H, const int16_t* h0c, const int16_t* h2c, const UsedSet& used, uint32_t* b0,
   uint32_t* b2)
 {
-  const float h1x = H.x(h1), h1y = H.y(h1), h1z = H.z(h1);
+  const float h1x = H.wx(h1), h1y = H.wy(h1), h1z = H.wz(h1);
   const int f0 = h0c[2 * h1], l0 = h0c[2 * h1 + 1];
   const int f2 = h2c[2 * h1], l2 = h2c[2 * h1 + 1];
   float best = kFltMax;
@@ -163,13 +191,13 @@ __device__ inline float seed_best(
   // iteration order, same result.
   for (int h0 = f0; h0 < l0; ++h0) {
     if (used.test(h0)) continue;
-    const float h0x = H.x(h0), h0y = H.y(h0), h0z = H.z(h0);
+    const float h0x = H.wx(h0), h0y = H.wy(h0), h0z = H.wz(h0);
     const float dmax = kMaxSlope * (h0z - h1z);
     if (!(fabsf(h1x - h0x) < dmax && fabsf(h1y - h0y) < dmax)) continue;
     const float dz10 = h1z - h0z;
     for (int h2 = f2; h2 < l2; ++h2) {
       if (used.test(h2)) continue;
-      const float h2x = H.x(h2), h2y = H.y(h2), h2z = H.z(h2);
+      const float h2x = H.wx(h2), h2y = H.wy(h2), h2z = H.wz(h2);
       const float z2_tz = (h2z - h0z) / dz10;
       const float x = h0x + (h1x - h0x) * z2_tz;
       const float y = h0y + (h1y - h0y) * z2_tz;
@@ -196,7 +224,7 @@ __device__ inline float seed_best(
 struct PhiAt {
   const EventHits* H;
   uint32_t base;
-  __device__ float operator()(int i) const { return H->phi(base + (uint32_t) i); }
+  __device__ float operator()(int i) const { return H->wphi(base + (uint32_t) i); }
 };
 
 // TrackForwarding.cuh:28-58
@@ -338,8 +366,8 @@ __device__ inline uint64_t seed_item(
   uint32_t h1, int h0, uint32_t item, const EventHits& H, int f2, int l2, const UsedSet& used)
 {
   if (used.test(h0)) return ~0ull;
-  const float h1x = H.x(h1), h1y = H.y(h1), h1z = H.z(h1);
-  const float h0x = H.x(h0), h0y = H.y(h0), h0z = H.z(h0);
+  const float h1x = H.wx(h1), h1y = H.wy(h1), h1z = H.wz(h1);
+  const float h0x = H.wx(h0), h0y = H.wy(h0), h0z = H.wz(h0);
   const float dmax = kMaxSlope * (h0z - h1z);
   if (!(fabsf(h1x - h0x) < dmax && fabsf(h1y - h0y) < dmax)) return ~0ull;
   const float dz10 = h1z - h0z;
@@ -347,7 +375,7 @@ __device__ inline uint64_t seed_item(
   uint32_t bb2 = 0;
   for (int h2 = f2; h2 < l2; ++h2) {
     if (used.test(h2)) continue;
-    const float h2x = H.x(h2), h2y = H.y(h2), h2z = H.z(h2);
+    const float h2x = H.wx(h2), h2y = H.wy(h2), h2z = H.wz(h2);
     const float z2_tz = (h2z - h0z) / dz10;
     const float x = h0x + (h1x - h0x) * z2_tz;
     const float y = h0y + (h1y - h0y) * z2_tz;
@@ -527,7 +555,7 @@ __device__ inline void forwarding(
       float best = kFltMax;
       for (int j = 0; j < n_all; ++j) {
         const int h2 = j < n_odd ? of + j : ef + j - n_odd;
-        const float h2x = H.x(h2), h2y = H.y(h2), h2z = H.z(h2);
+        const float h2x = H.wx(h2), h2y = H.wy(h2), h2z = H.wz(h2);
         const float dz = h2z - h0z;
         const float predx = tx * dz;
         const float predy = ty * dz;
