@@ -306,6 +306,13 @@ def test_search_by_triplet_uneven(stages, consts, chain_uneven):
     _check_search(stages, consts, chain_uneven)
 
 
+@pytest.mark.parametrize("tile", [64, 200, 256])
+def test_search_by_triplet_small_windows(stages, consts, chain_uneven, tile):
+    """Windows too small for most steps (hits read from global memory), a
+    tile that rounds down to 128, and one that fits part of the steps."""
+    _check_search(stages, consts, chain_uneven, (64, tile))
+
+
 def test_weak_tracks_adder(stages, chain):
     n = chain.n
     tracks_in = chain.tracks_s.view(np.uint8)
