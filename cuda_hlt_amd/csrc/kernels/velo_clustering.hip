@@ -369,16 +369,29 @@ struct ModuleBanks {
   uint32_t sorted; // bit k: bank 4m+k has strictly increasing SP addresses
 };
 
+// Bank b of an event of nwords words.  A bank whose offset table, header
+// or SP words would reach past the event is flagged and skipped, like a
+// mislabelled one (the reference reads on past the event).
 template<typename P>
-__device__ inline void bank_of(P ev, uint32_t n_banks, uint32_t b, uint32_t* base, uint32_t* n, uint32_t* status)
+__device__ inline void bank_of(
+  P ev, uint32_t n_banks, uint32_t nwords, uint32_t b, uint32_t* base, uint32_t* n, uint32_t* status)
 {
   *base = 0;
   *n = 0;
   if (b >= n_banks || b >= VELO_N_SENSORS) return;
+  // an event holds fewer than 2^30 words, so none of these sums wraps
+  if (nwords < 2u || n_banks > nwords - 2u) {
+    *status |= VELO_STATUS_BAD_BANK;
+    return;
+  }
   const uint32_t h = n_banks + 2 + ev[1 + b] / 4u;
+  if (h > nwords - 2u) {
+    *status |= VELO_STATUS_BAD_BANK;
+    return;
+  }
   const uint32_t sensor = ev[h];
   const uint32_t cnt = ev[h + 1];
-  if (sensor != b) {
+  if (sensor != b || cnt > nwords - 2u - h) {
     *status |= VELO_STATUS_BAD_BANK;
     return;
   }
@@ -387,15 +400,39 @@ __device__ inline void bank_of(P ev, uint32_t n_banks, uint32_t b, uint32_t* bas
   *n = cnt;
 }
 
+// The event's banks, one entry per sensor (first SP word, SP count, status
+// bits), found once per event by the whole workgroup.
+struct BankTable {
+  uint32_t base[VELO_N_SENSORS];
+  uint32_t n[VELO_N_SENSORS];
+  uint32_t status[VELO_N_SENSORS];
+};
+
 template<typename P>
-__device__ inline ModuleBanks module_banks(P ev, uint32_t n_banks, uint32_t m)
+__device__ inline void build_bank_table(P ev, uint32_t nwords, BankTable& t)
+{
+  const uint32_t n_banks = nwords ? ev[0] : 0u;
+  for (uint32_t b = threadIdx.x; b < VELO_N_SENSORS; b += blockDim.x) {
+    uint32_t base, n, st = 0;
+    bank_of(ev, n_banks, nwords, b, &base, &n, &st);
+    t.base[b] = base;
+    t.n[b] = n;
+    t.status[b] = st;
+  }
+}
+
+__device__ inline ModuleBanks module_banks(const BankTable& t, uint32_t m)
 {
   ModuleBanks mb;
-  mb.status = 0;
-  bank_of(ev, n_banks, 4 * m + 0, &mb.b0, &mb.n0, &mb.status);
-  bank_of(ev, n_banks, 4 * m + 1, &mb.b1, &mb.n1, &mb.status);
-  bank_of(ev, n_banks, 4 * m + 2, &mb.b2, &mb.n2, &mb.status);
-  bank_of(ev, n_banks, 4 * m + 3, &mb.b3, &mb.n3, &mb.status);
+  mb.b0 = t.base[4 * m + 0];
+  mb.b1 = t.base[4 * m + 1];
+  mb.b2 = t.base[4 * m + 2];
+  mb.b3 = t.base[4 * m + 3];
+  mb.n0 = t.n[4 * m + 0];
+  mb.n1 = t.n[4 * m + 1];
+  mb.n2 = t.n[4 * m + 2];
+  mb.n3 = t.n[4 * m + 3];
+  mb.status = t.status[4 * m + 0] | t.status[4 * m + 1] | t.status[4 * m + 2] | t.status[4 * m + 3];
   // the same on every lane: kept in scalar registers
   mb.b0 = __builtin_amdgcn_readfirstlane(mb.b0);
   mb.b1 = __builtin_amdgcn_readfirstlane(mb.b1);
@@ -486,8 +523,8 @@ __device__ inline uint32_t block_or(uint32_t v, uint32_t* shared_word)
 // ---------------------------------------------------------------------------
 template<typename P>
 __device__ __forceinline__ void estimate_event(
-  P ev, const velo_estimate_input_size_args& a, uint32_t e, uint32_t* module_cands, uint32_t* module_cand_offset,
-  uint32_t* block_status, uint32_t* stash, uint32_t stash_words)
+  P ev, const BankTable& banks, const velo_estimate_input_size_args& a, uint32_t e, uint32_t* module_cands,
+  uint32_t* module_cand_offset, uint32_t* block_status, uint32_t* stash)
 {
   // stash: the event's LDS copy (null when the event is read from global
   // memory).  Pass 1 keeps each non-isolated SP's candidate byte in bits
@@ -496,12 +533,11 @@ __device__ __forceinline__ void estimate_event(
   const uint32_t waves = blockDim.x / kWave;
   const uint32_t wave = threadIdx.x / kWave;
   const uint32_t lane = lane_id();
-  const uint32_t n_banks = ev[0];
   uint32_t status = 0;
 
   // pass 1: estimates and candidate counts per module
   for (uint32_t m = wave; m < VELO_N_MODULES; m += waves) {
-    ModuleBanks mb = module_banks(ev, n_banks, m);
+    ModuleBanks mb = module_banks(banks, m);
     check_sorted(ev, mb);
     status |= mb.status;
     uint32_t est = 0, ncand = 0;
@@ -517,7 +553,7 @@ __device__ __forceinline__ void estimate_event(
         const uint32_t n = sel4(k, mb.n0, mb.n1, mb.n2, mb.n3);
         const uint32_t c = candidates_from_pixels(((mb.sorted >> k) & 1u) ? neighbour_pixels_sorted(w, i, ev + base, n)
                                                                           : neighbour_pixels(w, ev + base, n));
-        if (stash && base + i < stash_words) stash[base + i] = (w & 0x807FFFFFu) | (c << 23);
+        if (stash) stash[base + i] = (w & 0x807FFFFFu) | (c << 23);
         const uint32_t f = ((c & 0x0Fu) ? 1u : 0u) + ((c & 0xF0u) ? 1u : 0u);
         est += f;
         ncand += f;
@@ -544,7 +580,7 @@ __device__ __forceinline__ void estimate_event(
   uint32_t* cands = a.dev_cluster_candidates + (size_t) e * VELO_MAX_CANDIDATES_EVENT;
   for (uint32_t m = wave; m < VELO_N_MODULES; m += waves) {
     if (module_cands[m] == 0) continue;
-    ModuleBanks mb = module_banks(ev, n_banks, m);
+    ModuleBanks mb = module_banks(banks, m);
     if (!stash) check_sorted(ev, mb);
     uint32_t pos = module_cand_offset[m];
     for (uint32_t jb = 0; jb < mb.total; jb += kWave) {
@@ -592,6 +628,7 @@ __global__ void __launch_bounds__(1024) estimate_input_size_kernel(velo_estimate
   __shared__ uint32_t module_cands[VELO_N_MODULES];
   __shared__ uint32_t module_cand_offset[VELO_N_MODULES + 1];
   __shared__ uint32_t block_status;
+  __shared__ BankTable banks;
   for_each_event(a.tuning, a.n_events, [&](const uint32_t e) {
     const uint32_t ei = event_index(a.dev_event_list, e);
     const uint32_t off = a.dev_raw_input_offsets[ei];
@@ -601,11 +638,15 @@ __global__ void __launch_bounds__(1024) estimate_input_size_kernel(velo_estimate
     if (nwords <= a.tuning.lds_tile) {
       stage_words(lds, gev, nwords);
       __syncthreads();
-      estimate_event(static_cast<const uint32_t*>(lds), a, e, module_cands, module_cand_offset, &block_status, lds, nwords);
+      build_bank_table(static_cast<const uint32_t*>(lds), nwords, banks);
+      __syncthreads();
+      estimate_event(static_cast<const uint32_t*>(lds), banks, a, e, module_cands, module_cand_offset, &block_status, lds);
     }
     else {
       __syncthreads();
-      estimate_event(gev, a, e, module_cands, module_cand_offset, &block_status, nullptr, 0u);
+      build_bank_table(gev, nwords, banks);
+      __syncthreads();
+      estimate_event(gev, banks, a, e, module_cands, module_cand_offset, &block_status, nullptr);
     }
   });
 }
@@ -626,13 +667,12 @@ __global__ void __launch_bounds__(1024) estimate_input_size_kernel(velo_estimate
 // module (53 values), or null.
 template<typename P>
 __device__ __forceinline__ void cluster_event(
-  P ev, const velo_masked_clustering_args& a, uint32_t e, uint32_t* block_words, uint32_t* entries,
-  uint32_t* seeds, const uint32_t* cl, const uint32_t* cl_mod)
+  P ev, const BankTable& banks, const velo_masked_clustering_args& a, uint32_t e, uint32_t* block_words,
+  uint32_t* entries, uint32_t* seeds, const uint32_t* cl, const uint32_t* cl_mod)
 {
   const uint32_t waves = blockDim.x / kWave;
   const uint32_t wave = threadIdx.x / kWave;
   const uint32_t lane = lane_id();
-  const uint32_t n_banks = ev[0];
   const uint32_t total = a.dev_module_cluster_start[VELO_N_MODULES * a.n_events];
   float* xs = reinterpret_cast<float*>(a.dev_velo_cluster_container);
   float* ys = xs + total;
@@ -643,7 +683,7 @@ __device__ __forceinline__ void cluster_event(
   uint32_t event_hits = 0;
 
   for (uint32_t m = wave; m < VELO_N_MODULES; m += waves) {
-    ModuleBanks mb = module_banks(ev, n_banks, m);
+    ModuleBanks mb = module_banks(banks, m);
     check_sorted(ev, mb);
     const uint32_t mstart = starts[m];
     const uint32_t mend = starts[m + 1]; // the module's estimated slots end here
@@ -818,6 +858,7 @@ masked_clustering_kernel(velo_masked_clustering_args a)
 {
   extern __shared__ uint32_t lds[];
   __shared__ uint32_t block_words[2];
+  __shared__ BankTable banks;
   for_each_event(a.tuning, a.n_events, [&](const uint32_t e) {
     const uint32_t ei = event_index(a.dev_event_list, e);
     const uint32_t off = a.dev_raw_input_offsets[ei];
@@ -841,24 +882,27 @@ masked_clustering_kernel(velo_masked_clustering_args a)
         cl = lds + nwords;
         cl_mod = cl + ncand;
         stage_words(cl, a.dev_cluster_candidates + (size_t) e * VELO_MAX_CANDIDATES_EVENT, ncand);
-        __syncthreads();
-        if (threadIdx.x <= VELO_N_MODULES) {
-          // first entry of module threadIdx.x (bank field bits 3-10, module = bank / 4)
-          uint32_t lo = 0, hi = ncand;
-          while (lo < hi) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (((cl[mid] >> 5) & 0x3Fu) < threadIdx.x) lo = mid + 1;
-            else hi = mid;
-          }
-          cl_mod[threadIdx.x] = lo;
-        }
       }
       __syncthreads();
-      cluster_event(static_cast<const uint32_t*>(lds), a, e, block_words, entries, seeds, cl, cl_mod);
+      build_bank_table(static_cast<const uint32_t*>(lds), nwords, banks);
+      if (cl && threadIdx.x <= VELO_N_MODULES) {
+        // first entry of module threadIdx.x (bank field bits 3-10, module = bank / 4)
+        uint32_t lo = 0, hi = ncand;
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (((cl[mid] >> 5) & 0x3Fu) < threadIdx.x) lo = mid + 1;
+          else hi = mid;
+        }
+        cl_mod[threadIdx.x] = lo;
+      }
+      __syncthreads();
+      cluster_event(static_cast<const uint32_t*>(lds), banks, a, e, block_words, entries, seeds, cl, cl_mod);
     }
     else {
       __syncthreads();
-      cluster_event(gev, a, e, block_words, entries, seeds, nullptr, nullptr);
+      build_bank_table(gev, nwords, banks);
+      __syncthreads();
+      cluster_event(gev, banks, a, e, block_words, entries, seeds, nullptr, nullptr);
     }
   });
 }

@@ -122,6 +122,8 @@ static inline uint32_t rd32(const char* p)
 
 typedef struct {
   uint32_t n_banks;
+  uint32_t n_words;    /* the event's size in 32-bit words */
+  const char* base;
   const char* offsets; /* n_banks+1 u32, bytes relative to payload */
   const char* payload;
 } raw_event;
@@ -130,24 +132,40 @@ typedef struct {
   uint32_t sensor_index;
   uint32_t sp_count;
   const char* sp_words;
+  int fits; /* header and SP words lie inside the event */
 } raw_bank;
 
-static raw_event make_event(const char* ev)
+static raw_event make_event(const char* ev, uint32_t n_words)
 {
   raw_event r;
-  r.n_banks = rd32(ev);
+  r.n_words = n_words;
+  r.n_banks = n_words ? rd32(ev) : 0u;
+  r.base = ev;
   r.offsets = ev + 4;
   r.payload = ev + 4 + 4 * (r.n_banks + 1);
   return r;
 }
 
+/* Bank b starts at word n_banks + 2 + offset / 4 (the decoder's word
+   addressing).  A bank whose offset table, header or SP words would reach
+   past the event does not fit: it is flagged and skipped like a mislabelled
+   one (the reference reads on past the event). */
 static raw_bank make_bank(const raw_event* e, uint32_t b)
 {
   raw_bank k;
-  const char* p = e->payload + rd32(e->offsets + 4 * b);
+  k.sensor_index = 0xFFFFFFFFu;
+  k.sp_count = 0;
+  k.sp_words = NULL;
+  k.fits = 0;
+  if ((uint64_t) e->n_banks + 2u > e->n_words) return k;
+  const uint64_t h = (uint64_t) e->n_banks + 2u + rd32(e->offsets + 4 * b) / 4u;
+  if (h + 2u > e->n_words) return k;
+  const char* p = e->base + 4 * h;
   k.sensor_index = rd32(p);
   k.sp_count = rd32(p + 4);
   k.sp_words = p + 8;
+  k.fits = h + 2u + k.sp_count <= e->n_words;
+  if (!k.fits) k.sp_count = 0;
   return k;
 }
 
@@ -281,10 +299,10 @@ void oracle_estimate_input_size(
     uint32_t* cand_e = candidates + e * OR_MAX_CANDIDATES_EVENT;
     for (uint32_t m = 0; m < OR_N_MODULES; ++m) est_e[m] = num_e[m] = 0;
     uint32_t n_cand = 0;
-    const raw_event ev = make_event(events + offsets[e]);
+    const raw_event ev = make_event(events + offsets[e], (offsets[e + 1] - offsets[e]) / 4u);
     for (uint32_t b = 0; b < ev.n_banks && b < OR_N_SENSORS; ++b) {
       const raw_bank bank = make_bank(&ev, b);
-      if (bank.sensor_index != b) {
+      if (!bank.fits || bank.sensor_index != b) {
         status[e] |= OR_STATUS_BAD_BANK;
         continue;
       }
@@ -496,10 +514,10 @@ void oracle_masked_clustering(
     const uint32_t* start_e = module_cluster_start + e * OR_N_MODULES;
     uint32_t* num_e = module_cluster_num + e * OR_N_MODULES;
     for (uint32_t m = 0; m < OR_N_MODULES; ++m) num_e[m] = 0;
-    const raw_event ev = make_event(events + offsets[e]);
+    const raw_event ev = make_event(events + offsets[e], (offsets[e + 1] - offsets[e]) / 4u);
     for (uint32_t b = 0; b < ev.n_banks && b < OR_N_SENSORS; ++b) {
       const raw_bank bank = make_bank(&ev, b);
-      if (bank.sensor_index != b) continue; /* flagged by the estimate */
+      if (!bank.fits || bank.sensor_index != b) continue; /* flagged by the estimate */
       const uint32_t module = b >> 2;
       for (uint32_t i = 0; i < bank.sp_count; ++i) {
         const uint32_t w = sp_word_at(&bank, i);
@@ -1293,7 +1311,7 @@ uint32_t oracle_ccl_event(
   static uint32_t stack[COLS * ROWS];
   static uint32_t touched[COLS * ROWS];
   uint32_t n_out = 0;
-  const raw_event ev = make_event(event);
+  const raw_event ev = make_event(event, 0xFFFFFFFFu); /* a well-formed event (test helper) */
   for (uint32_t b = 0; b < ev.n_banks && b < OR_N_SENSORS; ++b) {
     const raw_bank bank = make_bank(&ev, b);
     uint32_t n_touched = 0;

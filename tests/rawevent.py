@@ -111,3 +111,45 @@ def scramble_banks(events: np.ndarray, offsets: np.ndarray, seed: int = 0) -> np
                 j = int(rng.integers(0, n - 1))
                 sps[j + 1] = sps[j]
     return out
+
+
+def malformed(events: np.ndarray, offsets: np.ndarray):
+    """Damaged copies of the first four events, in four ways, plus the
+    same events with every damaged bank emptied (its SP count set to 0),
+    which is what a decoder that skips damaged banks must see:
+    0: the event cut inside the bank stored last;
+    1: bank 7's offset points past the event;
+    2: a bank count the event's offset table cannot hold (every bank damaged);
+    3: bank 100's SP count larger than the event.
+    Returns (events, offsets, clean_events, clean_offsets, damaged) with
+    damaged[e] the set of damaged bank numbers."""
+    out, clean, damaged = [], [], []
+    for e in range(4):
+        w = np.frombuffer(events[offsets[e]:offsets[e + 1]].tobytes(), np.uint32).copy()
+        nb = int(w[0])
+        header = lambda b: nb + 2 + int(w[1 + b]) // 4
+        c = w.copy()
+        if e == 0:
+            last = max(range(nb), key=header)  # the bank stored last
+            end = header(last) + 2 + int(w[header(last) + 1])
+            c[header(last) + 1] = 0
+            w = w[: end - 1] if end - 1 > header(last) + 1 else w[: header(last) + 1]
+            bad = {last}
+        elif e == 1:
+            c[header(7) + 1] = 0
+            w[1 + 7] = 4 * len(w) + 400
+            bad = {7}
+        elif e == 2:
+            w[0] = len(w)
+            c = np.array([0], np.uint32)  # no banks at all
+            bad = set(range(min(nb, 208)))
+        else:
+            w[header(100) + 1] = len(w)
+            c[header(100) + 1] = 0
+            bad = {100}
+        out.append((w.view(np.uint8), None))
+        clean.append((c.view(np.uint8), None))
+        damaged.append(bad)
+    ev, off = concat(out)
+    cev, coff = concat(clean)
+    return ev, off, cev, coff, damaged

@@ -438,3 +438,31 @@ def test_null_argument_is_rejected(stages):
     from cuda_hlt_amd import native as nat
     with pytest.raises(nat.VeloError):
         nat.launch("velo_search_by_triplet_launch", nat.SearchByTripletArgs())
+
+
+@pytest.mark.parametrize("tile", [0, 32])
+def test_damaged_banks(stages, consts, minbias_200, geometry, tile):
+    """Banks whose offset, header or SP words reach past their event are
+    flagged and skipped as the oracle does, the event read from LDS (tile 0)
+    or from global memory (tile 32); every read stays inside the event."""
+    from tests import rawevent
+    ev, off, *_ = rawevent.malformed(minbias_200.events, minbias_200.offsets)
+    n = len(off) - 1
+    oest, onum, ocnum, ocand, ost = po.estimate_input_size(ev, off)
+    est, num, cnum, cand, st = stages.estimate(ev, off, consts, tuning=(256, tile))
+    np.testing.assert_array_equal(est[: n * N_MODULES], oest[: n * N_MODULES])
+    np.testing.assert_array_equal(cnum, ocnum)
+    for e in range(n):
+        k = int(ocnum[e])
+        np.testing.assert_array_equal(cand[e * 3000: e * 3000 + k], ocand[e * 3000: e * 3000 + k])
+    np.testing.assert_array_equal(st & 0x100, ost & 0x100)
+    assert np.all(ost & 0x100)
+    starts = po.prefix_sum(oest)
+    onums, ocont, _ = po.masked_clustering(ev, off, starts, geometry)
+    total = int(starts[n * N_MODULES])
+    mask = module_mask(n, starts, onums)
+    for cands in (None, (ocnum, ocand)):
+        nums, cont, _ = stages.clustering(ev, off, starts, consts, (512, tile), cands)
+        np.testing.assert_array_equal(nums[: n * N_MODULES], onums)
+        for s in range(4):
+            np.testing.assert_array_equal(cont[s * total: (s + 1) * total][mask], ocont[s * total: (s + 1) * total][mask])

@@ -448,3 +448,26 @@ def test_consolidated_states_least_squares(minbias_200, geometry, module_zs):
         assert abs(S[2, t] - tx) < 1e-4 and abs(S[3, t] - ty) < 1e-4
         assert abs(S[11, t] - zb) < max(0.05, 1e-3 * abs(zb))
         assert abs(S[0, t] - (x0 + tx * zb)) < 5e-3 and abs(S[1, t] - (y0 + ty * zb)) < 5e-3
+
+
+def test_damaged_banks_are_flagged_and_skipped(minbias_200):
+    """A bank whose offset, header or SP words reach past its event is
+    flagged (BAD_BANK) and skipped: estimates, candidates and clusters equal
+    those of the same event with that bank emptied."""
+    from tests import rawevent
+    ev, off, cev, coff, damaged = rawevent.malformed(minbias_200.events, minbias_200.offsets)
+    est, num, cnum, cand, st = po.estimate_input_size(ev, off)
+    cest, cnum0, ccnum, ccand, cst = po.estimate_input_size(cev, coff)
+    assert np.all(st & 0x100), st                     # VELO_STATUS_BAD_BANK on every damaged event
+    assert not np.any(cst & 0x100)
+    np.testing.assert_array_equal(est, cest)
+    np.testing.assert_array_equal(cnum, ccnum)
+    np.testing.assert_array_equal(cand, ccand)
+    starts = po.prefix_sum(est)
+    from cuda_hlt_amd import gen
+    g = gen.geometry()
+    nums, cont, _ = po.masked_clustering(ev, off, starts, g)
+    cnums, ccont, _ = po.masked_clustering(cev, coff, starts, g)
+    np.testing.assert_array_equal(nums, cnums)
+    np.testing.assert_array_equal(cont, ccont)
+    assert est[: 2 * 52].sum() > 0 and est[2 * 52: 3 * 52].sum() == 0   # event 2 has no usable bank
