@@ -57,7 +57,7 @@ extern "C" {
 #define VELO_STATUS_TRACKLETS_OVERFLOW 0x020u  /* >2000 tracklets within two steps */
 #define VELO_STATUS_WEAK_OVERFLOW 0x040u       /* >500 weak tracks; extra dropped */
 #define VELO_STATUS_TRACKS_OVERFLOW 0x080u     /* >1200 tracks; extra dropped */
-#define VELO_STATUS_BAD_BANK 0x100u            /* bank index != sensor index; bank skipped */
+#define VELO_STATUS_BAD_BANK 0x100u            /* bank index != sensor index, or the bank reaches past its event; bank skipped */
 
 /* ---- tuning block ----
  * Launch shape of one launcher call.  With grid_dim = 0 a per-event kernel
