@@ -228,7 +228,8 @@ void SequenceVisitor::visit<velo_estimate_input_size_t>(
   hipStream_t& stream,
   hipEvent_t&)
 {
-  state.set_opts(dim3(host_buffers.host_number_of_selected_events[0]), dim3(256), stream);
+  // 512 threads: eight waves share the event's staged words and bank table
+  state.set_opts(dim3(host_buffers.host_number_of_selected_events[0]), dim3(512), stream);
   state.set_arguments(
     arguments.offset<dev_velo_raw_input>(),
     arguments.offset<dev_velo_raw_input_offsets>(),
