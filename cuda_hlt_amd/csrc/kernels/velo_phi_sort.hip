@@ -217,57 +217,66 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5))) 
 }
 
 // fill_candidates: the event's phi slot is copied to LDS when it fits
-// (tuning.lds_tile floats); wave w owns modules 2+w, 2+w+W, ... up to 49.
+// (tuning.lds_tile floats); the waves take 64-hit passes over modules 2-49.
 constexpr uint32_t kDefaultPhiTile = 4096;
 
+// The hits of modules 2-49 are taken as one range, 64 per pass, whatever
+// module they belong to (a module holds 6 to 58 hits on average, so passes
+// per module would leave lanes idle); each lane finds its module from the
+// event's module table.  mstart: 53 event-relative module starts, mnum: 52
+// hit counts.
 template<typename A>
 __device__ __forceinline__ void fill_event(
-  const A& phis, const velo_fill_candidates_args& a, const uint32_t* starts, const uint32_t* nums, uint32_t off,
-  int16_t* h0c, int16_t* h2c)
+  const A& phis, const uint32_t* mstart, const uint32_t* mnum, int16_t* h0c, int16_t* h2c)
 {
   const uint32_t waves = blockDim.x / kWave;
   const uint32_t wave = threadIdx.x / kWave;
   const uint32_t lane = lane_id();
-  for (uint32_t m = 2 + wave; m < 50; m += waves) {
-    const uint32_t n1 = nums[m];
-    const uint32_t m0s = starts[m + 2] - off, m0n = nums[m + 2];
-    const uint32_t m2s = starts[m - 2] - off, m2n = nums[m - 2];
-    const uint32_t s1 = starts[m] - off;
-    for (uint32_t r = lane; r < n1; r += kWave) {
-      const uint32_t h1 = s1 + r;
-      const float phi = phis(h1);
-      int f0 = -1, l0 = -1, f2 = -1, l2 = -1;
-      if (m0n > 0) {
-        const auto w0 = [&](int i) { return phis(m0s + i); };
-        f0 = bs_first_candidate(w0, m0n, phi, kPhiExtrapolation);
-        if (f0 != -1) {
-          const auto w0f = [&](int i) { return phis(m0s + f0 + i); };
-          l0 = bs_second_candidate(w0f, m0n - (uint32_t) f0, phi, kPhiExtrapolation);
-          f0 += (int) m0s;
-          l0 = l0 == 0 ? f0 + 1 : f0 + l0;
-        }
+  const uint32_t lo = mstart[2], hi = mstart[50];
+  uint32_t mb = 2; // module of the pass's first hit (passes of a wave move forward)
+  for (uint32_t base = lo + wave * kWave; base < hi; base += waves * kWave) {
+    while (mb < 49 && mstart[mb + 1] <= base) ++mb;
+    const uint32_t h1 = base + lane;
+    if (h1 >= hi) continue;
+    uint32_t m = mb;
+    while (m < 49 && mstart[m + 1] <= h1) ++m;
+    if (h1 - mstart[m] >= mnum[m]) continue; // an unfilled slot
+    const uint32_t m0s = mstart[m + 2], m0n = mnum[m + 2];
+    const uint32_t m2s = mstart[m - 2], m2n = mnum[m - 2];
+    const float phi = phis(h1);
+    int f0 = -1, l0 = -1, f2 = -1, l2 = -1;
+    if (m0n > 0) {
+      const auto w0 = [&](int i) { return phis(m0s + i); };
+      f0 = bs_first_candidate(w0, m0n, phi, kPhiExtrapolation);
+      if (f0 != -1) {
+        const auto w0f = [&](int i) { return phis(m0s + f0 + i); };
+        l0 = bs_second_candidate(w0f, m0n - (uint32_t) f0, phi, kPhiExtrapolation);
+        f0 += (int) m0s;
+        l0 = l0 == 0 ? f0 + 1 : f0 + l0;
       }
-      if (m2n > 0) {
-        const auto w2 = [&](int i) { return phis(m2s + i); };
-        f2 = bs_first_candidate(w2, m2n, phi, kPhiExtrapolation);
-        if (f2 != -1) {
-          const auto w2f = [&](int i) { return phis(m2s + f2 + i); };
-          l2 = bs_second_candidate(w2f, m2n - (uint32_t) f2, phi, kPhiExtrapolation);
-          f2 += (int) m2s;
-          l2 = l2 == 0 ? f2 + 1 : f2 + l2;
-        }
-      }
-      h0c[2 * h1] = (int16_t) f0;
-      h0c[2 * h1 + 1] = (int16_t) l0;
-      h2c[2 * h1] = (int16_t) f2;
-      h2c[2 * h1 + 1] = (int16_t) l2;
     }
+    if (m2n > 0) {
+      const auto w2 = [&](int i) { return phis(m2s + i); };
+      f2 = bs_first_candidate(w2, m2n, phi, kPhiExtrapolation);
+      if (f2 != -1) {
+        const auto w2f = [&](int i) { return phis(m2s + f2 + i); };
+        l2 = bs_second_candidate(w2f, m2n - (uint32_t) f2, phi, kPhiExtrapolation);
+        f2 += (int) m2s;
+        l2 = l2 == 0 ? f2 + 1 : f2 + l2;
+      }
+    }
+    h0c[2 * h1] = (int16_t) f0;
+    h0c[2 * h1 + 1] = (int16_t) l0;
+    h2c[2 * h1] = (int16_t) f2;
+    h2c[2 * h1 + 1] = (int16_t) l2;
   }
 }
 
 __global__ void __launch_bounds__(1024) fill_candidates_kernel(velo_fill_candidates_args a)
 {
   extern __shared__ float s_phi[];
+  __shared__ uint32_t s_mstart[VELO_N_MODULES + 1];
+  __shared__ uint32_t s_mnum[VELO_N_MODULES];
   for_each_event(a.tuning, a.n_events, [&](const uint32_t e) {
     const uint32_t total = a.dev_module_cluster_start[VELO_N_MODULES * a.n_events];
     const uint32_t* starts = a.dev_module_cluster_start + e * VELO_N_MODULES;
@@ -277,14 +286,17 @@ __global__ void __launch_bounds__(1024) fill_candidates_kernel(velo_fill_candida
     const float* phis = reinterpret_cast<const float*>(a.dev_velo_cluster_container + 4 * (size_t) total + off);
     int16_t* h0c = a.dev_h0_candidates + 2 * (size_t) off;
     int16_t* h2c = a.dev_h2_candidates + 2 * (size_t) off;
+    if (threadIdx.x <= VELO_N_MODULES) s_mstart[threadIdx.x] = starts[threadIdx.x] - off;
+    if (threadIdx.x < VELO_N_MODULES) s_mnum[threadIdx.x] = nums[threadIdx.x];
     if (ev_hits <= a.tuning.lds_tile) {
       for (uint32_t i = threadIdx.x; i < ev_hits; i += blockDim.x) s_phi[i] = phis[i];
       __syncthreads();
       const auto acc = [&](int i) { return s_phi[i]; };
-      fill_event(acc, a, starts, nums, off, h0c, h2c);
+      fill_event(acc, s_mstart, s_mnum, h0c, h2c);
     }
     else {
-      fill_event(ArrayAt {phis}, a, starts, nums, off, h0c, h2c);
+      __syncthreads();
+      fill_event(ArrayAt {phis}, s_mstart, s_mnum, h0c, h2c);
     }
   });
 }
