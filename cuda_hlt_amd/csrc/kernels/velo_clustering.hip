@@ -540,6 +540,7 @@ __device__ __forceinline__ void estimate_event(
     ModuleBanks mb = module_banks(banks, m);
     check_sorted(ev, mb);
     status |= mb.status;
+    if (mb.sorted != 0xFu) status |= VELO_STATUS_UNSORTED_BANK;
     uint32_t est = 0, ncand = 0;
     for (uint32_t j = lane; j < mb.total; j += kWave) {
       uint32_t k;
@@ -681,10 +682,13 @@ __device__ __forceinline__ void cluster_event(
   const uint32_t* starts = a.dev_module_cluster_start + e * VELO_N_MODULES;
   uint32_t status = 0;
   uint32_t event_hits = 0;
+  // estimate_input_size wrote the event's status word, with UNSORTED_BANK
+  // when any bank is out of order; without it every bank is sorted
+  const bool check_order = !a.dev_velo_event_status || (a.dev_velo_event_status[e] & VELO_STATUS_UNSORTED_BANK);
 
   for (uint32_t m = wave; m < VELO_N_MODULES; m += waves) {
     ModuleBanks mb = module_banks(banks, m);
-    check_sorted(ev, mb);
+    if (check_order) check_sorted(ev, mb);
     const uint32_t mstart = starts[m];
     const uint32_t mend = starts[m + 1]; // the module's estimated slots end here
     uint32_t pos = 0;

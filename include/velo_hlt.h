@@ -58,6 +58,8 @@ extern "C" {
 #define VELO_STATUS_WEAK_OVERFLOW 0x040u       /* >500 weak tracks; extra dropped */
 #define VELO_STATUS_TRACKS_OVERFLOW 0x080u     /* >1200 tracks; extra dropped */
 #define VELO_STATUS_BAD_BANK 0x100u            /* bank index != sensor index, or the bank reaches past its event; bank skipped */
+#define VELO_STATUS_UNSORTED_BANK 0x200u       /* a bank's SP addresses do not strictly increase (set by estimate_input_size;
+                                                  masked clustering checks bank order only in events that carry it) */
 
 /* ---- tuning block ----
  * Launch shape of one launcher call.  With grid_dim = 0 a per-event kernel
@@ -208,7 +210,9 @@ typedef struct {
   const uint8_t* dev_velo_sp_patterns;
   const float* dev_velo_sp_fx;
   const float* dev_velo_sp_fy;
-  uint32_t* dev_velo_event_status;
+  uint32_t* dev_velo_event_status;          /* N or NULL; as estimate_input_size left it: bank order is
+                                               checked only in events with VELO_STATUS_UNSORTED_BANK
+                                               (NULL: in every event) */
   uint32_t n_events;
   velo_tuning tuning;
 } velo_masked_clustering_args;

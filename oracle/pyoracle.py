@@ -27,7 +27,7 @@ TRACKLET_DTYPE = np.dtype([("hits", "<u2", (3,))])                   # 6 B
 STATUS_BITS = {
     "SP_INDEX_TRUNCATED": 0x001, "CANDIDATES_OVERFLOW": 0x002, "MODULE_HITS_OVERFLOW": 0x004,
     "EVENT_HITS_OVERFLOW": 0x008, "TTF_OVERFLOW": 0x010, "TRACKLETS_OVERFLOW": 0x020,
-    "WEAK_OVERFLOW": 0x040, "TRACKS_OVERFLOW": 0x080, "BAD_BANK": 0x100,
+    "WEAK_OVERFLOW": 0x040, "TRACKS_OVERFLOW": 0x080, "BAD_BANK": 0x100, "UNSORTED_BANK": 0x200,
 }
 
 _lib = None

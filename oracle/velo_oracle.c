@@ -307,6 +307,9 @@ void oracle_estimate_input_size(
         continue;
       }
       if (bank.sp_count > 256u) status[e] |= OR_STATUS_SP_INDEX_TRUNCATED;
+      for (uint32_t i = 1; i < bank.sp_count; ++i)
+        if (((sp_word_at(&bank, i - 1) >> 8) & 0x7FFFu) >= ((sp_word_at(&bank, i) >> 8) & 0x7FFFu))
+          status[e] |= OR_STATUS_UNSORTED_BANK;
       uint32_t* est_m = est_e + (bank.sensor_index >> 2);
       for (uint32_t i = 0; i < bank.sp_count; ++i) {
         const uint32_t w = sp_word_at(&bank, i);

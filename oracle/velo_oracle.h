@@ -54,6 +54,7 @@ extern "C" {
 #define OR_STATUS_WEAK_OVERFLOW 0x040u       /* >500 weak tracks                */
 #define OR_STATUS_TRACKS_OVERFLOW 0x080u     /* >1200 tracks                    */
 #define OR_STATUS_BAD_BANK 0x100u            /* bank index != sensor index      */
+#define OR_STATUS_UNSORTED_BANK 0x200u       /* SP addresses not increasing     */
 
 #pragma pack(push, 1)
 typedef struct {

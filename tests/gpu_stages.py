@@ -81,13 +81,18 @@ def prefix_sum(arr: np.ndarray, device_count=None) -> np.ndarray:
     return host(d).view(np.uint32)
 
 
-def clustering(events, offsets, starts, consts: nat.DeviceConstants, tuning=(0, 0), cands=None):
+def clustering(events, offsets, starts, consts: nat.DeviceConstants, tuning=(0, 0), cands=None, status0=None):
     """cands: (per-event candidate counts, candidate list) from
-    estimate_input_size, or None to derive the candidates from the banks."""
+    estimate_input_size, or None to derive the candidates from the banks.
+    status0: the event status words estimate_input_size wrote (their
+    UNSORTED_BANK bit tells clustering which events to check for bank
+    order); None for zeros, which is right only when every bank is sorted."""
     n = len(offsets) - 1
     total = int(starts[n * N_MODULES])
     raw, off, st = dev(events), dev(offsets.view(np.int32)), dev(starts.view(np.int32))
     num, status = garbage_u32(n * N_MODULES), u32(n)
+    if status0 is not None:
+        status = dev(np.ascontiguousarray(status0, np.uint32).view(np.int32))
     cont = garbage_u32(6 * total + 1)
     c = consts.c
     cnum_p = cand_p = None

@@ -145,14 +145,14 @@ def test_prefix_sum_sizes(stages, size):
 
 
 def _check_clustering(stages, consts, c: Chain, tuning=(0, 0), cands=None):
-    num, cont, st = stages.clustering(c.ev.events, c.ev.offsets, c.starts, consts, tuning, cands)
+    num, cont, st = stages.clustering(c.ev.events, c.ev.offsets, c.starts, consts, tuning, cands, c.st_est)
     n = c.n
     total = int(c.starts[n * N_MODULES])
     np.testing.assert_array_equal(num[: n * N_MODULES], c.nums)
     for s in range(4):
         np.testing.assert_array_equal(cont[s * total: (s + 1) * total][c.mask],
                                       c.cont_raw[s * total: (s + 1) * total][c.mask], err_msg=f"slot {s}")
-    np.testing.assert_array_equal(st, c.st_clu)
+    np.testing.assert_array_equal(st, c.st_est | c.st_clu)
 
 
 def test_masked_clustering(stages, consts, chain):

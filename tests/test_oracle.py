@@ -450,6 +450,28 @@ def test_consolidated_states_least_squares(minbias_200, geometry, module_zs):
         assert abs(S[0, t] - (x0 + tx * zb)) < 5e-3 and abs(S[1, t] - (y0 + ty * zb)) < 5e-3
 
 
+def test_unsorted_banks_are_flagged(minbias_200):
+    """UNSORTED_BANK is set exactly on the events holding a bank whose SP
+    addresses do not strictly increase, judged here from the raw words."""
+    from tests import rawevent
+    off = minbias_200.offsets
+    ev = rawevent.scramble_banks(minbias_200.events, off, seed=3)
+    st = po.estimate_input_size(ev, off)[-1]
+    st0 = po.estimate_input_size(minbias_200.events, off)[-1]
+    expect = []
+    for e in range(len(off) - 1):
+        w = ev[off[e]:off[e + 1]].view(np.uint32)
+        nb, bad = int(w[0]), False
+        for b in range(nb):
+            s = nb + 2 + int(w[1 + b]) // 4
+            addr = ((w[s + 2: s + 2 + int(w[s + 1])] >> 8) & 0x7FFF).astype(np.int64)
+            bad |= bool(np.any(np.diff(addr) <= 0))
+        expect.append(bad)
+    assert any(expect)
+    np.testing.assert_array_equal((st & po.STATUS_BITS["UNSORTED_BANK"]) != 0, expect)
+    assert not np.any(st0 & po.STATUS_BITS["UNSORTED_BANK"])
+
+
 def test_damaged_banks_are_flagged_and_skipped(minbias_200):
     """A bank whose offset, header or SP words reach past its event is
     flagged (BAD_BANK) and skipped: estimates, candidates and clusters equal
