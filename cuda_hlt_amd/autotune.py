@@ -106,17 +106,16 @@ def tune(seq, n_events, dev_events, dev_offsets, repeats=3, log=print):
     return best, table
 
 
-def batched_reserve_mb(n_events, raw_bytes):
-    """Device arena a batched run needs: the bounds it sizes buffers by
-    (clusters <= raw bytes / 2, 1200 tracks and 1200 * 26 track hits per
-    event) plus the per-event arrays, with 10 % for alignment."""
-    clusters = raw_bytes // 2 + 1
-    tracks = n_events * 1200
-    per_cluster = 24 + 4 + 8 + 1              # container, permutation, windows, used flags
-    per_track = 54 + 4 + 26 * 16 + 49 + 1     # TrackHits, hit count, consolidated hits, state, accepted
-    per_event = 52 * 4 * 2 + 3000 * 4 + 4 * 4 + 2000 * 8 + 500 * 6 + 5 * 4
-    total = clusters * per_cluster + tracks * per_track + n_events * per_event
-    return int(total * 1.1 / 2 ** 20) + 64
+def batched_reserve_mb(n_events, raw_bytes, velo_kalman_fit=False, pv_beamline=False):
+    """Device arena (MiB) a batched run needs: the library's scheduler lays
+    the sequence out on the batched bounds (velo_workspace_bytes: clusters
+    <= raw bytes / 2, 1200 tracks and 1200 * 26 track hits per event) and
+    reports the high-water mark; rounded up to whole MiB."""
+    from .stream import workspace_bytes
+    b = workspace_bytes(n_events, raw_bytes, velo_kalman_fit, pv_beamline)
+    if b == 0:
+        raise ValueError(f"no batched layout for {n_events} events of {raw_bytes} bytes")
+    return -(-b // 2 ** 20)
 
 
 def tune_batched(seq, n_events, dev_events, dev_offsets, events_size, dev_order, repeats=3, log=print):

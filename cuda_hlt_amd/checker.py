@@ -27,6 +27,11 @@ class _Category(C.Structure):
                 ("eff_per_event", C.c_double), ("hit_eff", C.c_double), ("hit_purity", C.c_double)]
 
 
+class _McTruth(C.Structure):
+    _fields_ = [("mcps", C.c_void_p), ("n_mcps", C.c_size_t), ("mc_hits", C.c_void_p), ("n_mc_hits", C.c_size_t),
+                ("n_events", C.c_uint32)]
+
+
 class _Result(C.Structure):
     _fields_ = [("events", C.c_uint64), ("tracks", C.c_uint64), ("ghosts", C.c_uint64),
                 ("ghost_per_event", C.c_double), ("categories", _Category * N_CATEGORIES)]
@@ -44,6 +49,14 @@ def _lib():
         L.velo_checker_run.argtypes = [C.c_uint32] + [C.c_void_p] * 4 + [C.c_void_p, C.c_size_t, C.c_void_p,
                                                                          C.POINTER(_Result), C.c_void_p, C.c_void_p]
         L.velo_checker_run.restype = C.c_int
+        nat.check_mirrors(L.velo_abi_struct_size, {_Category: "velo_checker_category",
+                                                   _Result: "velo_checker_result",
+                                                   _McTruth: "velo_mc_truth"}, nat.LIB_PATH)
+        L.velo_mc_read_folder.argtypes = [C.c_char_p, C.c_uint32, C.c_uint32, C.POINTER(_McTruth), C.c_char_p,
+                                          C.c_size_t]
+        L.velo_mc_read_folder.restype = C.c_int
+        L.velo_mc_free.argtypes = [C.POINTER(_McTruth)]
+        L.velo_mc_free.restype = None
         _bound = True
     return L
 
@@ -114,6 +127,11 @@ def check_arrays(n_events: int, event_track_offsets, track_hit_offsets, hit_ids,
                             track_mcp.ctypes.data, track_weight.ctypes.data)
     if rc != 0:
         raise ValueError("velo_checker_run rejected its input (null pointer or MC particles out of event order)")
+    return result_of(res, track_mcp[:T], track_weight[:T])
+
+
+def result_of(res, track_mcp=None, track_weight=None) -> CheckerResult:
+    """CheckerResult of a velo_checker_result."""
     cats = {}
     for i, name in enumerate(category_names()):
         c = res.categories[i]
@@ -121,7 +139,7 @@ def check_arrays(n_events: int, event_track_offsets, track_hit_offsets, hit_ids,
                       "eff_per_event": c.eff_per_event, "hit_eff": c.hit_eff, "hit_purity": c.hit_purity,
                       "clone_rate": c.clones / (2 * c.found) if c.found else 0.0}
     return CheckerResult(int(res.events), int(res.tracks), int(res.ghosts), res.ghost_per_event, cats,
-                         track_mcp[:T], track_weight[:T])
+                         track_mcp, track_weight)
 
 
 def check(ev: Events, out, accepted=None) -> CheckerResult:
@@ -133,3 +151,20 @@ def check(ev: Events, out, accepted=None) -> CheckerResult:
     ids = np.asarray(out.track_hits)[12 * Ht: 16 * Ht].view(np.uint32)
     return check_arrays(n, np.asarray(out.atomics)[n: 2 * n + 1], np.asarray(out.hit_number)[: T + 1], ids,
                         ev.mcps, ev.mc_hits, accepted)
+
+
+def read_mc_folder(folder, n_events: int = 0, start_event: int = 0):
+    """MC truth of a folder of per-event MC files (the reference's
+    MC_info/tracks layout, MCEvent.cpp:16-103): (mcps, mc_hits) in the
+    generator's MCP_DTYPE layout, particles with VELO hits only."""
+    L = _lib()
+    out = _McTruth()
+    err = C.create_string_buffer(512)
+    if L.velo_mc_read_folder(str(folder).encode(), n_events, start_event, C.byref(out), err, 512) != 0:
+        raise ValueError(err.value.decode())
+    try:
+        mcps = np.frombuffer(C.string_at(out.mcps, out.n_mcps * MCP_DTYPE.itemsize), MCP_DTYPE).copy()
+        hits = np.frombuffer(C.string_at(out.mc_hits, out.n_mc_hits * 4), np.uint32).copy()
+    finally:
+        L.velo_mc_free(C.byref(out))
+    return mcps, hits

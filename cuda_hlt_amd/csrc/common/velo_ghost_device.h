@@ -118,6 +118,8 @@ __device__ __forceinline__ float ghost_logit_infer(const float* w, const float* 
     h1[j] = s > 0.0f ? s : 0.0f;
   }
   float z = w[kGhostB3];
+  // one second-layer unit at a time: its 32 weights in flight, not all 1024
+#pragma unroll 1
   for (uint32_t k = 0; k < VELO_GHOST_HIDDEN; ++k) {
     float s = w[kGhostB2 + k];
 #pragma unroll

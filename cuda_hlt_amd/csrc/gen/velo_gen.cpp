@@ -351,6 +351,14 @@ struct Handle {
 
 extern "C" {
 
+size_t velo_gen_struct_size(const char* name)
+{
+  if (!name) return 0;
+  if (std::strcmp(name, "velo_gen_config") == 0) return sizeof(velo_gen_config);
+  if (std::strcmp(name, "velo_gen_mcp") == 0) return sizeof(velo_gen_mcp);
+  return 0;
+}
+
 void velo_gen_default_config(velo_gen_config* cfg)
 {
   cfg->seed = 0;

@@ -3,6 +3,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <stdexcept>
+#include <string>
 
 hipError_t Stream::initialize(
   unsigned max_number_of_events,
@@ -53,6 +55,50 @@ void Stream::batched_capacities(const RuntimeOptions& ro, HostBuffers& hb)
   hb.host_total_number_of_velo_clusters[0] = (unsigned) clusters;
   hb.host_number_of_reconstructed_velo_tracks[0] = (unsigned) tracks;
   hb.host_accumulated_number_of_hits_in_velo_tracks[0] = (unsigned) track_hits;
+}
+
+size_t Stream::batched_arena_bytes(unsigned n_events, size_t events_bytes, bool run_velo_kalman_fit,
+                                   bool run_pv_beamline)
+{
+  RuntimeOptions ro;
+  ro.number_of_events = n_events;
+  ro.host_velopix_events_size = events_bytes;
+  ro.host_velopix_event_offsets_size = (size_t) n_events + 1;
+  // the input is bound in place in a batched run: any non-null address
+  // makes the sizing take it as external (nothing is dereferenced)
+  static const char resident[4] = {};
+  ro.dev_velopix_events = resident;
+  ro.dev_velopix_event_offsets = reinterpret_cast<const unsigned*>(resident);
+  ro.batched = true;
+  ro.run_pv_beamline = run_pv_beamline;
+  ro.run_velo_kalman_fit = run_velo_kalman_fit || run_pv_beamline;
+  unsigned counters[4] = {};
+  HostBuffers hb;
+  hb.host_number_of_selected_events = counters;
+  hb.host_total_number_of_velo_clusters = counters + 1;
+  hb.host_number_of_reconstructed_velo_tracks = counters + 2;
+  hb.host_accumulated_number_of_hits_in_velo_tracks = counters + 3;
+  batched_capacities(ro, hb);
+  scheduler_t sched;
+  sched.initialize(false, (size_t) 1 << 62, nullptr);
+  SequenceVisitor visitor;
+  Constants constants;
+  Sch::size_sequence(sched, visitor, std::forward_as_tuple(static_cast<const RuntimeOptions&>(ro),
+                                                           static_cast<const Constants&>(constants),
+                                                           static_cast<const HostBuffers&>(hb)));
+  const size_t bytes = sched.memory_manager.total_memory_required;
+  hb.host_number_of_selected_events = nullptr; // not pinned memory: nothing for release() to free
+  hb.host_total_number_of_velo_clusters = hb.host_number_of_reconstructed_velo_tracks = nullptr;
+  hb.host_accumulated_number_of_hits_in_velo_tracks = nullptr;
+  return bytes;
+}
+
+void Stream::run_monte_carlo_test(CheckerInvoker& invoker, unsigned start_event_offset)
+{
+  const unsigned n = host_buffers.host_number_of_selected_events[0];
+  Sch::check_sequence<configured_sequence_t>(
+    sequence_visitor, start_event_offset, n, static_cast<const HostBuffers&>(host_buffers),
+    static_cast<const Constants&>(constants), invoker);
 }
 
 void Stream::drop_graphs()
@@ -141,37 +187,51 @@ hipError_t Stream::run_batched(const RuntimeOptions& ro)
   // totals (the plain sequence reads them between steps), on this stream
   unsigned* hb = host_buffers.host_number_of_selected_events;
   hb[0] = n;
-  auto d2h = [&](void* dst, const void* src, size_t bytes) {
-    hipCheck(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
+  auto d2h = [&](void* dst, const void* src, size_t bytes, const char* what) {
+    host_buffers.copy_to_host(dst, src, bytes, stream, what);
   };
   auto wait = [&]() {
     hipCheck(hipEventRecord(event, stream));
     hipCheck(hipEventSynchronize(event));
   };
-  d2h(hb + 1, output<dev_estimated_input_size>() + (size_t) n * VELO_N_MODULES, sizeof(unsigned));
-  d2h(hb + 2, reinterpret_cast<unsigned*>(output<dev_atomics_velo>()) + 2 * (size_t) n, sizeof(unsigned));
+  d2h(hb + 1, output<dev_estimated_input_size>() + (size_t) n * VELO_N_MODULES, sizeof(unsigned), "cluster total");
+  d2h(hb + 2, reinterpret_cast<unsigned*>(output<dev_atomics_velo>()) + 2 * (size_t) n, sizeof(unsigned),
+      "track total");
   wait();
-  d2h(hb + 3, output<dev_velo_track_hit_number>() + hb[2], sizeof(unsigned));
+  // the totals come from the device: they index and size host copies only
+  // within the bounds the graph was captured for
+  const size_t max_tracks = (size_t) n * VELO_MAX_TRACKS;
+  if (hb[2] > max_tracks)
+    throw std::runtime_error("batched run: track total " + std::to_string(hb[2]) + " exceeds its bound " +
+                             std::to_string(max_tracks));
+  d2h(hb + 3, output<dev_velo_track_hit_number>() + hb[2], sizeof(unsigned), "track hit total");
   wait();
+  if ((size_t) hb[3] > (size_t) hb[2] * VELO_MAX_TRACK_SIZE)
+    throw std::runtime_error("batched run: track hit total " + std::to_string(hb[3]) + " exceeds its bound");
   if (ro.copy_outputs_to_host) {
     const size_t t = hb[2], h = hb[3];
     host_buffers.ensure_outputs(t, h);
-    d2h(host_buffers.host_atomics_velo, output<dev_atomics_velo>(), (2 * (size_t) n + 1) * sizeof(unsigned));
-    d2h(host_buffers.host_velo_track_hit_number, output<dev_velo_track_hit_number>(), (t + 1) * sizeof(unsigned));
-    d2h(host_buffers.host_velo_track_hits, output<dev_velo_track_hits>(), h * 16);
-    d2h(host_buffers.host_velo_states, output<dev_velo_states>(), t * VELO_STATE_BYTES);
-    d2h(host_buffers.host_accepted_velo_tracks, output<dev_accepted_velo_tracks>(), t);
+    d2h(host_buffers.host_atomics_velo, output<dev_atomics_velo>(), (2 * (size_t) n + 1) * sizeof(unsigned),
+        "atomics");
+    d2h(host_buffers.host_velo_track_hit_number, output<dev_velo_track_hit_number>(), (t + 1) * sizeof(unsigned),
+        "track hit number");
+    d2h(host_buffers.host_velo_track_hits, output<dev_velo_track_hits>(), h * 16, "track hits");
+    d2h(host_buffers.host_velo_states, output<dev_velo_states>(), t * VELO_STATE_BYTES, "states");
+    d2h(host_buffers.host_accepted_velo_tracks, output<dev_accepted_velo_tracks>(), t, "accepted");
     if (ro.run_velo_kalman_fit) {
       host_buffers.ensure_kalman(t);
-      d2h(host_buffers.host_kalmanvelo_states, output<dev_velo_kalman_beamline_states>(), t * VELO_STATE_BYTES);
+      d2h(host_buffers.host_kalmanvelo_states, output<dev_velo_kalman_beamline_states>(), t * VELO_STATE_BYTES,
+          "kalman states");
     }
     if (ro.run_pv_beamline) {
       d2h(host_buffers.host_reconstructed_multi_pvs, output<dev_multi_fit_vertices>(),
-          (size_t) n * VELO_PV_MAX_VERTICES * sizeof(velo_pv_vertex));
-      d2h(host_buffers.host_number_of_multivertex, output<dev_number_of_multi_fit_vertices>(), n * sizeof(unsigned));
+          (size_t) n * VELO_PV_MAX_VERTICES * sizeof(velo_pv_vertex), "multi-fit vertices");
+      d2h(host_buffers.host_number_of_multivertex, output<dev_number_of_multi_fit_vertices>(), n * sizeof(unsigned),
+          "vertex counts");
     }
     wait();
   }
+  host_buffers.check_guards();
   return hipSuccess;
 }
 
@@ -209,6 +269,7 @@ hipError_t Stream::run_sequence(const RuntimeOptions& options)
       step_ms.assign(n_steps, 0.f);
       for (size_t i = 0; i < n_steps; ++i) hipCheck(hipEventElapsedTime(&step_ms[i], step_events[i], step_events[i + 1]));
     }
+    host_buffers.check_guards();
   }
   return hipSuccess;
 }

@@ -34,9 +34,9 @@ void tune(S& state, const velo_tuning& t, const RuntimeOptions& ro, WorkSlot slo
 
 // Host read of one total between steps (the plain sequence's three
 // synchronisation points); batched runs size by bounds instead.
-void read_total(unsigned* host, const unsigned* dev, hipStream_t& stream, hipEvent_t& event)
+void read_total(const HostBuffers& hb, unsigned* host, const unsigned* dev, hipStream_t& stream, hipEvent_t& event)
 {
-  hipCheck(hipMemcpyAsync(host, dev, sizeof(unsigned), hipMemcpyDeviceToHost, stream));
+  hb.copy_to_host(host, dev, sizeof(unsigned), stream, "a total");
   hipCheck(hipEventRecord(event, stream));
   hipCheck(hipEventSynchronize(event));
 }
@@ -166,10 +166,9 @@ void SequenceVisitor::visit<global_event_cut_t>(
     arguments.size<dev_ut_raw_input>(),
     arguments.size<dev_scifi_raw_input>());
   state.invoke();
-  hipCheck(hipMemcpyAsync(
-    host_buffers.host_event_list, arguments.offset<dev_event_list>(),
-    (size_t) runtime_options.number_of_events * sizeof(unsigned), hipMemcpyDeviceToHost, stream));
-  read_total(host_buffers.host_number_of_selected_events, arguments.offset<dev_number_of_selected_events>(), stream,
+  host_buffers.copy_to_host(host_buffers.host_event_list, arguments.offset<dev_event_list>(),
+                            (size_t) runtime_options.number_of_events * sizeof(unsigned), stream, "host_event_list");
+  read_total(host_buffers, host_buffers.host_number_of_selected_events, arguments.offset<dev_number_of_selected_events>(), stream,
              event);
 }
 
@@ -273,7 +272,7 @@ void SequenceVisitor::visit<prefix_sum_velo_clusters_t>(
   state.set_arguments(arguments.offset<dev_estimated_input_size>(), arguments.offset<dev_cluster_offset>());
   state.invoke();
   if (!runtime_options.batched)
-    read_total(host_buffers.host_total_number_of_velo_clusters,
+    read_total(host_buffers, host_buffers.host_total_number_of_velo_clusters,
                arguments.offset<dev_estimated_input_size>() + n * VELO_N_MODULES, stream, event);
 }
 
@@ -485,7 +484,7 @@ void SequenceVisitor::visit<copy_and_prefix_sum_single_block_velo_t>(
   state.set_arguments(atomics + 2 * n, atomics, atomics + n, n);
   state.invoke();
   if (!runtime_options.batched)
-    read_total(host_buffers.host_number_of_reconstructed_velo_tracks, atomics + 2 * n, stream, event);
+    read_total(host_buffers, host_buffers.host_number_of_reconstructed_velo_tracks, atomics + 2 * n, stream, event);
 }
 
 // ---- copy_velo_track_hit_number_t ------------------------------------------------
@@ -549,7 +548,7 @@ void SequenceVisitor::visit<prefix_sum_velo_track_hit_number_t>(
                             : nullptr);
   state.invoke();
   if (!runtime_options.batched)
-    read_total(host_buffers.host_accumulated_number_of_hits_in_velo_tracks,
+    read_total(host_buffers, host_buffers.host_accumulated_number_of_hits_in_velo_tracks,
                arguments.offset<dev_velo_track_hit_number>() + t, stream, event);
 }
 
@@ -598,16 +597,16 @@ void SequenceVisitor::visit<consolidate_velo_tracks_t>(
   const size_t t = host_buffers.host_number_of_reconstructed_velo_tracks[0];
   const size_t h = host_buffers.host_accumulated_number_of_hits_in_velo_tracks[0];
   host_buffers.ensure_outputs(t, h);
-  hipCheck(hipMemcpyAsync(host_buffers.host_atomics_velo, arguments.offset<dev_atomics_velo>(),
-                          (2 * (size_t) n + 1) * sizeof(unsigned), hipMemcpyDeviceToHost, stream));
-  hipCheck(hipMemcpyAsync(host_buffers.host_velo_track_hit_number, arguments.offset<dev_velo_track_hit_number>(),
-                          (t + 1) * sizeof(unsigned), hipMemcpyDeviceToHost, stream));
-  hipCheck(hipMemcpyAsync(host_buffers.host_velo_track_hits, arguments.offset<dev_velo_track_hits>(), h * sizeof(Velo::Hit),
-                          hipMemcpyDeviceToHost, stream));
-  hipCheck(hipMemcpyAsync(host_buffers.host_velo_states, arguments.offset<dev_velo_states>(), t * VELO_STATE_BYTES,
-                          hipMemcpyDeviceToHost, stream));
-  hipCheck(hipMemcpyAsync(host_buffers.host_accepted_velo_tracks, arguments.offset<dev_accepted_velo_tracks>(), t,
-                          hipMemcpyDeviceToHost, stream));
+  host_buffers.copy_to_host(host_buffers.host_atomics_velo, arguments.offset<dev_atomics_velo>(),
+                            (2 * (size_t) n + 1) * sizeof(unsigned), stream, "host_atomics_velo");
+  host_buffers.copy_to_host(host_buffers.host_velo_track_hit_number, arguments.offset<dev_velo_track_hit_number>(),
+                            (t + 1) * sizeof(unsigned), stream, "host_velo_track_hit_number");
+  host_buffers.copy_to_host(host_buffers.host_velo_track_hits, arguments.offset<dev_velo_track_hits>(),
+                            h * sizeof(Velo::Hit), stream, "host_velo_track_hits");
+  host_buffers.copy_to_host(host_buffers.host_velo_states, arguments.offset<dev_velo_states>(),
+                            t * VELO_STATE_BYTES, stream, "host_velo_states");
+  host_buffers.copy_to_host(host_buffers.host_accepted_velo_tracks, arguments.offset<dev_accepted_velo_tracks>(),
+                            t, stream, "host_accepted_velo_tracks");
 }
 
 // ---- velo_kalman_fit_t (KalmanFilterVisitor.cu:4-44) -------------------------------
@@ -644,8 +643,8 @@ void SequenceVisitor::visit<velo_kalman_fit_t>(
   if (!runtime_options.copy_outputs_to_host || runtime_options.batched) return;
   const size_t t = host_buffers.host_number_of_reconstructed_velo_tracks[0];
   host_buffers.ensure_kalman(t);
-  hipCheck(hipMemcpyAsync(host_buffers.host_kalmanvelo_states, arguments.offset<dev_velo_kalman_beamline_states>(),
-                          t * VELO_STATE_BYTES, hipMemcpyDeviceToHost, stream));
+  host_buffers.copy_to_host(host_buffers.host_kalmanvelo_states, arguments.offset<dev_velo_kalman_beamline_states>(),
+                            t * VELO_STATE_BYTES, stream, "host_kalmanvelo_states");
 }
 
 // ---- pv_beamline_extrapolate_t (ExtrapolateVisitor.cu:5-33) --------------------
@@ -779,8 +778,8 @@ void SequenceVisitor::visit<pv_beamline_multi_fitter_t>(
     arguments.offset<dev_number_of_multi_fit_vertices>());
   state.invoke();
   if (!runtime_options.copy_outputs_to_host || runtime_options.batched) return;
-  hipCheck(hipMemcpyAsync(host_buffers.host_reconstructed_multi_pvs, arguments.offset<dev_multi_fit_vertices>(),
-                          (size_t) n * VELO_PV_MAX_VERTICES * sizeof(velo_pv_vertex), hipMemcpyDeviceToHost, stream));
-  hipCheck(hipMemcpyAsync(host_buffers.host_number_of_multivertex, arguments.offset<dev_number_of_multi_fit_vertices>(),
-                          (size_t) n * sizeof(unsigned), hipMemcpyDeviceToHost, stream));
+  host_buffers.copy_to_host(host_buffers.host_reconstructed_multi_pvs, arguments.offset<dev_multi_fit_vertices>(),
+                            (size_t) n * VELO_PV_MAX_VERTICES * sizeof(velo_pv_vertex), stream, "host_reconstructed_multi_pvs");
+  host_buffers.copy_to_host(host_buffers.host_number_of_multivertex, arguments.offset<dev_number_of_multi_fit_vertices>(),
+                            (size_t) n * sizeof(unsigned), stream, "host_number_of_multivertex");
 }

@@ -1,6 +1,8 @@
 // C ABI of include/velo_sequence.h over the C++ Stream.
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <vector>
 #include <exception>
 #include <memory>
 
@@ -166,6 +168,66 @@ size_t velo_sequence_struct_size(int which)
   case 2: return sizeof(velo_sequence_output);
   default: return 0;
   }
+}
+
+size_t velo_workspace_bytes(uint32_t n_events, size_t events_bytes, int run_velo_kalman_fit, int run_pv_beamline)
+{
+  try {
+    return Stream::batched_arena_bytes(n_events, events_bytes, run_velo_kalman_fit != 0, run_pv_beamline != 0);
+  }
+  catch (const std::exception&) {
+    return 0;
+  }
+}
+
+int velo_status(const velo_sequence* s, uint32_t* host_status, uint32_t max_events, uint32_t* combined)
+{
+  if (!s || !s->stream) return -1;
+  const Stream& st = *s->stream;
+  const uint32_t n = st.host_buffers.host_number_of_selected_events[0];
+  const uint32_t* dev = st.output<dev_velo_event_status>();
+  if (combined) *combined = 0;
+  if (n == 0) return 0;
+  if (!dev) return -1;
+  std::vector<uint32_t> all(n);
+  if (hipMemcpyAsync(all.data(), dev, (size_t) n * sizeof(uint32_t), hipMemcpyDeviceToHost, st.stream) != hipSuccess ||
+      hipStreamSynchronize(st.stream) != hipSuccess)
+    return -1;
+  if (host_status) std::memcpy(host_status, all.data(), (size_t) std::min(n, max_events) * sizeof(uint32_t));
+  if (combined)
+    for (uint32_t v : all) *combined |= v & VELO_STATUS_ERROR_MASK;
+  return (int) n;
+}
+
+int velo_sequence_check(velo_sequence* s, const velo_mc_truth* truth, uint32_t start_event,
+                        velo_checker_result* result, char* err, size_t err_len)
+{
+  if (!s || !s->stream || !truth || !result) {
+    set_error(err, err_len, "velo_sequence_check: null argument");
+    return -1;
+  }
+  if (!s->last.copy_outputs_to_host) {
+    set_error(err, err_len, "velo_sequence_check: the last run did not copy its outputs to the host");
+    return -1;
+  }
+  CheckerInvoker invoker;
+  invoker.mcps = truth->mcps;
+  invoker.n_mcps = truth->n_mcps;
+  invoker.mc_hits = truth->mc_hits;
+  invoker.n_mc_events = truth->n_events;
+  invoker.velo_result = result;
+  try {
+    s->stream->run_monte_carlo_test(invoker, start_event);
+  }
+  catch (const std::exception& e) {
+    set_error(err, err_len, e.what());
+    return -1;
+  }
+  if (!invoker.velo_checked) {
+    set_error(err, err_len, "velo_sequence_check: no step checked the tracks");
+    return -1;
+  }
+  return 0;
 }
 
 void* velo_sequence_stream(const velo_sequence* s) { return s && s->stream ? (void*) s->stream->stream : nullptr; }

@@ -184,6 +184,46 @@ namespace Sch {
     hook(sizeof...(Is), "end");
   }
 
+  // Lays the sequence out without running it: set_arguments_size and
+  // setup for every step (no visit, nothing launched), so the memory
+  // manager's high-water mark is the arena the run needs.
+  template<typename Scheduler, typename Visitor, size_t... Is, typename... SizeArgs>
+  void size_sequence_impl(
+    Scheduler& scheduler, Visitor& visitor, std::index_sequence<Is...>, std::tuple<SizeArgs...> size_args)
+  {
+    using seq_t = std::remove_reference_t<decltype(scheduler.sequence_tuple)>;
+    auto step = [&](auto idx) {
+      constexpr size_t I = decltype(idx)::value;
+      using T = std::tuple_element_t<I, seq_t>;
+      std::apply(
+        [&](auto&&... a) { visitor.template set_arguments_size<T>(scheduler.template arguments<T>(), a...); },
+        size_args);
+      scheduler.template setup<I, T>();
+    };
+    (step(std::integral_constant<size_t, Is> {}), ...);
+  }
+
+  template<typename Scheduler, typename Visitor, typename... SizeArgs>
+  void size_sequence(Scheduler& scheduler, Visitor& visitor, std::tuple<SizeArgs...> size_args)
+  {
+    using seq_t = std::remove_reference_t<decltype(scheduler.sequence_tuple)>;
+    size_sequence_impl(scheduler, visitor, std::make_index_sequence<std::tuple_size_v<seq_t>> {}, size_args);
+  }
+
+  // Calls visitor.check<T>(args...) for every step T of the sequence, in
+  // order (reference: SchedulerMachinery.cuh:396-420, RunChecker).
+  template<typename Sequence, typename Visitor, size_t... Is, typename... Args>
+  void check_sequence_impl(const Visitor& visitor, std::index_sequence<Is...>, Args&&... args)
+  {
+    (visitor.template check<std::tuple_element_t<Is, Sequence>>(args...), ...);
+  }
+
+  template<typename Sequence, typename Visitor, typename... Args>
+  void check_sequence(const Visitor& visitor, Args&&... args)
+  {
+    check_sequence_impl<Sequence>(visitor, std::make_index_sequence<std::tuple_size_v<Sequence>> {}, args...);
+  }
+
   template<typename Scheduler, typename Visitor, typename Hook, typename... SizeArgs, typename... VisitArgs>
   void run_sequence(
     Scheduler& scheduler, Visitor& visitor, Hook& hook, std::tuple<SizeArgs...> size_args,

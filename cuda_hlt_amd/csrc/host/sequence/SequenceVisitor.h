@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include "CheckerInvoker.h"
 #include "Constants.h"
 #include "HostBuffers.h"
 #include "RuntimeOptions.h"
@@ -28,6 +29,17 @@ struct SequenceVisitor {
     HostBuffers& host_buffers,
     hipStream_t& stream,
     hipEvent_t& event);
+
+  // Checks the output of step T against MC truth after a run (reference:
+  // SequenceVisitor.cuh:42-48); steps without a checker do nothing.
+  template<typename T>
+  void check(
+    const unsigned& start_event_offset,
+    const unsigned& number_of_events_requested,
+    const HostBuffers& host_buffers,
+    const Constants& constants,
+    CheckerInvoker& checker_invoker) const
+  {}
 };
 
 #define DECLARE_VISITOR(_TYPE)                                                                          \
@@ -64,3 +76,12 @@ DECLARE_VISITOR(pv_beamline_extrapolate_t)
 DECLARE_VISITOR(pv_beamline_histo_t)
 DECLARE_VISITOR(pv_beamline_peak_t)
 DECLARE_VISITOR(pv_beamline_multi_fitter_t)
+
+// VeloSequenceCheckers_impl.cuh:8-24: the VELO tracks after consolidation.
+template<>
+void SequenceVisitor::check<consolidate_velo_tracks_t>(
+  const unsigned& start_event_offset,
+  const unsigned& number_of_events_requested,
+  const HostBuffers& host_buffers,
+  const Constants& constants,
+  CheckerInvoker& checker_invoker) const;

@@ -85,6 +85,17 @@ struct Stream {
   // Capacities a batched run sizes its buffers by (no host reads).
   static void batched_capacities(const RuntimeOptions& ro, HostBuffers& hb);
 
+  // Arena bytes a batched run of n_events events holding events_bytes raw
+  // bytes needs: the scheduler lays the sequence out on the batched bounds
+  // (set_arguments_size and setup of every step, nothing launched, no
+  // device call) and reports its high-water mark.
+  static size_t batched_arena_bytes(unsigned n_events, size_t events_bytes, bool run_velo_kalman_fit,
+                                    bool run_pv_beamline);
+
+  // Checks the last run's outputs (copied to the host) against MC truth
+  // with every step's SequenceVisitor::check (Stream.cu:85-105).
+  void run_monte_carlo_test(CheckerInvoker& invoker, unsigned start_event_offset);
+
   template<typename A>
   typename A::type* output() const
   {

@@ -619,7 +619,7 @@ __device__ __forceinline__ void estimate_event(
   if (a.dev_velo_event_status) {
     // first writer of the event's status word: store (no memset needed)
     const uint32_t s = block_or(status, block_status);
-    if (threadIdx.x == 0) a.dev_velo_event_status[e] = s;
+    if (threadIdx.x == 0) a.dev_velo_event_status[e] = s | VELO_STATUS_BANK_ORDER_CHECKED;
   }
 }
 
@@ -682,9 +682,13 @@ __device__ __forceinline__ void cluster_event(
   const uint32_t* starts = a.dev_module_cluster_start + e * VELO_N_MODULES;
   uint32_t status = 0;
   uint32_t event_hits = 0;
-  // estimate_input_size wrote the event's status word, with UNSORTED_BANK
-  // when any bank is out of order; without it every bank is sorted
-  const bool check_order = !a.dev_velo_event_status || (a.dev_velo_event_status[e] & VELO_STATUS_UNSORTED_BANK);
+  // The bank-order check is skipped only where estimate_input_size checked
+  // the event's banks (BANK_ORDER_CHECKED) and found them sorted (no
+  // UNSORTED_BANK); any other status word keeps the check.
+  const bool check_order =
+    !a.dev_velo_event_status ||
+    (a.dev_velo_event_status[e] & (VELO_STATUS_BANK_ORDER_CHECKED | VELO_STATUS_UNSORTED_BANK)) !=
+      VELO_STATUS_BANK_ORDER_CHECKED;
 
   for (uint32_t m = wave; m < VELO_N_MODULES; m += waves) {
     ModuleBanks mb = module_banks(banks, m);

@@ -266,7 +266,12 @@ __device__ __forceinline__ void consolidate_event(
       float f[VELO_GHOST_FEATURES];
       track_features(TrackHitsOf<V> {hv, t}, nh, x0, y0, tx, ty, zb, chi2, f);
       for (uint32_t k = 0; k < VELO_GHOST_FEATURES; ++k) f[k] = (f[k] - model->mean[k]) * model->scale[k];
-      const float p = ghost_sigmoid(ghost_logit_infer(gw, f));
+      // an offset the compiler cannot see through: the network's LDS reads
+      // stay inside the per-track loop instead of being hoisted out of it
+      // into hundreds of registers
+      uint32_t wbase = 0;
+      asm volatile("" : "+s"(wbase));
+      const float p = ghost_sigmoid(ghost_logit_infer(gw + wbase, f));
       accept = p < model->threshold ? 1 : 0;
       if (a.dev_ghost_score) a.dev_ghost_score[ti] = p;
     }

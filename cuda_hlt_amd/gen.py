@@ -50,6 +50,13 @@ def _load():
             f.argtypes = [C.c_void_p]
             f.restype = rt
         lib.velo_gen_free.argtypes = [C.c_void_p]
+        lib.velo_gen_struct_size.argtypes = [C.c_char_p]
+        lib.velo_gen_struct_size.restype = C.c_size_t
+        for st, name in ((GenConfig, b"velo_gen_config"), (MCP_DTYPE, b"velo_gen_mcp")):
+            size = C.sizeof(st) if isinstance(st, type) else st.itemsize
+            if lib.velo_gen_struct_size(name) != size:
+                raise RuntimeError(f"{_LIB_PATH}: {name.decode()} is {lib.velo_gen_struct_size(name)} bytes in the "
+                                   f"library, {size} in the binding (rebuild it)")
         _lib = lib
     return _lib
 

@@ -76,6 +76,10 @@ def _lib():
             f.restype = C.c_int
         L.velo_ghost_partials_size.argtypes = [U32]
         L.velo_ghost_partials_size.restype = C.c_size_t
+        nat.check_mirrors(L.velo_abi_struct_size, {
+            ModelStruct: "velo_ghost_model", TrackFeaturesArgs: "velo_track_features_args",
+            InferArgs: "velo_ghost_infer_args", GradArgs: "velo_ghost_grad_args", AdamArgs: "velo_ghost_adam_args"},
+            nat.LIB_PATH)
         _bound = True
     return L
 

@@ -63,6 +63,12 @@ def _lib():
         L.velo_mdf_free.restype = None
         L.velo_mdf_checksum.argtypes = [C.c_void_p, C.c_size_t]
         L.velo_mdf_checksum.restype = C.c_uint32
+        L.velo_read_folder.argtypes = [C.c_char_p, C.c_uint32, C.c_uint32, C.POINTER(_Events), C.c_char_p,
+                                       C.c_size_t]
+        L.velo_read_folder.restype = C.c_int
+        L.velo_check_vp_events.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32]
+        L.velo_check_vp_events.restype = C.c_int64
+        nat.check_mirrors(L.velo_abi_struct_size, {_Events: "velo_mdf_events"}, nat.LIB_PATH)
         _bound = True
     return L
 
@@ -128,6 +134,30 @@ def read_banks(paths: str | Path | Sequence[str | Path], bank_types: Sequence[in
     if rc != 0:
         raise MdfError(err.value.decode())
     return {int(t): _take(L, outs[k]) for k, t in enumerate(bank_types)}
+
+
+def read_folder(folder: str | Path, n_events: int = 0, start_event: int = 0) -> MdfEvents:
+    """Per-event files of a folder ("N.bin" / "N_M.bin", natural order,
+    round-robin past the last file), as the reference's read_folder
+    (main/src/InputTools.cpp:178-211); n_records = files in the folder."""
+    L = _lib()
+    out = _Events()
+    err = C.create_string_buffer(512)
+    if L.velo_read_folder(str(folder).encode(), n_events, start_event, C.byref(out), err, 512) != 0:
+        raise MdfError(err.value.decode())
+    return _take(L, out)
+
+
+def count_bad_banks(events: np.ndarray, offsets: np.ndarray) -> int:
+    """Banks whose sensor id is not their position (check_velopix_events,
+    CheckVP.cpp:5-52), counted by the library; raises on a bank table that
+    reaches past its event."""
+    ev = np.ascontiguousarray(events, np.uint8)
+    off = np.ascontiguousarray(offsets, np.uint32)
+    n = int(_lib().velo_check_vp_events(ev.ctypes.data, off.ctypes.data, len(off) - 1))
+    if n < 0:
+        raise MdfError("a bank table or bank offset reaches past its event")
+    return n
 
 
 def checksum(data: bytes) -> int:

@@ -35,7 +35,9 @@ STATUS = {
     "SP_INDEX_TRUNCATED": 0x001, "CANDIDATES_OVERFLOW": 0x002, "MODULE_HITS_OVERFLOW": 0x004,
     "EVENT_HITS_OVERFLOW": 0x008, "TTF_OVERFLOW": 0x010, "TRACKLETS_OVERFLOW": 0x020,
     "WEAK_OVERFLOW": 0x040, "TRACKS_OVERFLOW": 0x080, "BAD_BANK": 0x100, "UNSORTED_BANK": 0x200,
+    "BANK_ORDER_CHECKED": 0x80000000,  # not an error: estimate_input_size checked the event's bank order
 }
+STATUS_ERROR_MASK = 0x3FF
 
 P = C.c_void_p
 U32 = C.c_uint32
@@ -138,6 +140,35 @@ LAUNCHERS = {
 
 _lib = None
 
+# C typedef name of every ctypes mirror in this module.
+_C_NAMES = {
+    Tuning: "velo_tuning", Constants: "velo_constants", InitEventListArgs: "velo_init_event_list_args",
+    EventOrderArgs: "velo_event_order_args", GlobalEventCutArgs: "velo_global_event_cut_args",
+    EstimateInputSizeArgs: "velo_estimate_input_size_args", PrefixSumArgs: "velo_prefix_sum_args",
+    MaskedClusteringArgs: "velo_masked_clustering_args", CalculatePhiAndSortArgs: "velo_calculate_phi_and_sort_args",
+    FillCandidatesArgs: "velo_fill_candidates_args", SearchByTripletArgs: "velo_search_by_triplet_args",
+    WeakTracksAdderArgs: "velo_weak_tracks_adder_args",
+    CopyAndPrefixSumArgs: "velo_copy_and_prefix_sum_single_block_args",
+    CopyTrackHitNumberArgs: "velo_copy_velo_track_hit_number_args", KalmanFitArgs: "velo_kalman_fit_args",
+    ConsolidateArgs: "velo_consolidate_velo_tracks_args", PvExtrapolateArgs: "velo_pv_beamline_extrapolate_args",
+    PvHistoArgs: "velo_pv_beamline_histo_args", PvPeakArgs: "velo_pv_beamline_peak_args",
+    PvMultiFitterArgs: "velo_pv_beamline_multi_fitter_args",
+}
+
+
+def check_mirrors(size_fn, mirrors: dict, path) -> None:
+    """Refuse a library whose structs differ in size from these ctypes
+    mirrors ({ctypes type: C typedef name}); size_fn(name) is the library's
+    sizeof query.  A library built from a newer header would otherwise
+    read or write past a mirror."""
+    bad = []
+    for st, name in mirrors.items():
+        n = int(size_fn(name.encode()))
+        if n != C.sizeof(st):
+            bad.append(f"{name}: {n} bytes in the library, {C.sizeof(st)} in {st.__name__}")
+    if bad:
+        raise VeloError(f"{path}: struct layout differs from the binding (rebuild it): " + "; ".join(bad))
+
 
 def lib():
     """Load libvelo_hlt.so (raises when it is not built: no fallback)."""
@@ -171,6 +202,9 @@ def lib():
         L.velo_pv_exp.restype = C.c_float
         L.velo_error_string.argtypes = [C.c_int]
         L.velo_error_string.restype = C.c_char_p
+        L.velo_abi_struct_size.argtypes = [C.c_char_p]
+        L.velo_abi_struct_size.restype = C.c_size_t
+        check_mirrors(L.velo_abi_struct_size, _C_NAMES, LIB_PATH)
         _lib = L
     return _lib
 

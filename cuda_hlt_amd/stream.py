@@ -66,14 +66,24 @@ def _lib():
         L.velo_sequence_enable_step_timing.restype = C.c_int
         L.velo_sequence_step_times.argtypes = [P, C.POINTER(C.c_float), C.POINTER(C.c_char_p), C.c_int]
         L.velo_sequence_step_times.restype = C.c_int
+        L.velo_workspace_bytes.argtypes = [C.c_uint32, C.c_size_t, C.c_int, C.c_int]
+        L.velo_workspace_bytes.restype = C.c_size_t
+        L.velo_status.argtypes = [P, P, C.c_uint32, C.POINTER(C.c_uint32)]
+        L.velo_status.restype = C.c_int
+        L.velo_sequence_check.argtypes = [P, P, C.c_uint32, P, C.c_char_p, C.c_size_t]
+        L.velo_sequence_check.restype = C.c_int
         L.velo_sequence_struct_size.argtypes = [C.c_int]
         L.velo_sequence_struct_size.restype = C.c_size_t
-        for which, st in enumerate((SequenceConfig, SequenceInput, SequenceOutput)):
-            if L.velo_sequence_struct_size(which) != C.sizeof(st):
-                raise nat.VeloError(f"{nat.LIB_PATH}: {st.__name__} is {L.velo_sequence_struct_size(which)} bytes "
-                                    f"in the library, {C.sizeof(st)} here (rebuild it)")
+        nat.check_mirrors(L.velo_abi_struct_size, {
+            SequenceConfig: "velo_sequence_config", SequenceInput: "velo_sequence_input",
+            SequenceOutput: "velo_sequence_output", nat.Tuning: "velo_tuning"}, nat.LIB_PATH)
         _bound = True
     return L
+
+
+def workspace_bytes(n_events: int, events_bytes: int, velo_kalman_fit=False, pv_beamline=False) -> int:
+    """Arena bytes of a batched run (velo_workspace_bytes); no GPU needed."""
+    return int(_lib().velo_workspace_bytes(n_events, events_bytes, int(velo_kalman_fit), int(pv_beamline)))
 
 
 @dataclass
@@ -175,6 +185,34 @@ class VeloSequence:
         if rc != 0:
             raise nat.VeloError(f"velo_sequence_run: {err.value.decode()}")
         return self.output()
+
+    def status(self):
+        """(per-event status words of the last run, OR of their error bits)."""
+        n = int(_lib().velo_status(self.h, None, 0, None))
+        if n < 0:
+            raise nat.VeloError("velo_status failed")
+        out = np.zeros(max(n, 1), np.uint32)
+        combined = C.c_uint32(0)
+        if _lib().velo_status(self.h, out.ctypes.data, n, C.byref(combined)) != n:
+            raise nat.VeloError("velo_status failed")
+        return out[:n], int(combined.value)
+
+    def check(self, mcps, mc_hits, start_event=0):
+        """The last run's tracks against MC truth (MCP_DTYPE particles and
+        their LHCbIDs, events numbered from start_event) through the
+        sequence's checkers; returns a checker.CheckerResult."""
+        from . import checker
+        from .gen import MCP_DTYPE
+        m = np.ascontiguousarray(mcps, MCP_DTYPE)
+        h = np.ascontiguousarray(mc_hits, np.uint32)
+        truth = checker._McTruth(m.ctypes.data if len(m) else None, len(m), h.ctypes.data if len(h) else None,
+                                 len(h), int(m["event"].max()) + 1 if len(m) else 0)
+        res = checker._Result()
+        err = C.create_string_buffer(512)
+        checker._lib()
+        if _lib().velo_sequence_check(self.h, C.byref(truth), start_event, C.byref(res), err, 512) != 0:
+            raise nat.VeloError(err.value.decode())
+        return checker.result_of(res)
 
     def enable_step_timing(self, enable=True):
         nat.check(_lib().velo_sequence_enable_step_timing(self.h, int(enable)), "enable_step_timing")

@@ -61,6 +61,28 @@ typedef struct {
   velo_checker_category categories[VELO_CHECKER_N_CATEGORIES];
 } velo_checker_result;
 
+/* MC truth of the reference's checker input (the MC_info/tracks folder). */
+typedef struct {
+  velo_gen_mcp* mcps; /* particles with VELO hits, in event order; malloc'ed */
+  size_t n_mcps;
+  uint32_t* mc_hits;  /* their VELO LHCbIDs (mcps[i].hit_offset, .n_hits); malloc'ed */
+  size_t n_mc_hits;
+  uint32_t n_events;
+} velo_mc_truth;
+
+/* Reads n_events (0: all) per-event MC files of a folder from start_event
+ * on, in the folder's natural order (CheckerInvoker.cpp:3-44), each file
+ * parsed as MCEvent.cpp:16-103: u32 particle count, then per particle key,
+ * pid, p, pt, eta, phi, eight flag bytes (isLong, isDown, hasVelo, hasUT,
+ * hasSciFi, fromB, fromD, fromS), nPV, and the VELO, UT and SciFi LHCbID
+ * lists (u32 count + ids each).  Only particles with VELO hits are kept,
+ * with their VELO hits; flags map to VELO_MCP_* (electron: |pid| = 11,
+ * backward: eta < 0).  A file whose size differs from what it declares, or
+ * a request past the last file, is an error (-1, message in err). */
+int velo_mc_read_folder(const char* folder, uint32_t n_events, uint32_t start_event, velo_mc_truth* out, char* err,
+                        size_t err_len);
+void velo_mc_free(velo_mc_truth* t);
+
 /* Name of category i (the reference's report label), or NULL. */
 const char* velo_checker_category_name(uint32_t i);
 

@@ -81,6 +81,10 @@ const velo_gen_mcp* velo_gen_mcps(const void* h);
 uint32_t velo_gen_n_mc_hits(const void* h);
 const uint32_t* velo_gen_mc_hits(const void* h);
 void velo_gen_free(void* h);
+/* sizeof velo_gen_config / velo_gen_mcp as the library was built (by
+ * typedef name; 0 for an unknown name), for bindings to check their
+ * mirrors before the library writes into them. */
+size_t velo_gen_struct_size(const char* name);
 
 #ifdef __cplusplus
 }

@@ -58,8 +58,15 @@ extern "C" {
 #define VELO_STATUS_WEAK_OVERFLOW 0x040u       /* >500 weak tracks; extra dropped */
 #define VELO_STATUS_TRACKS_OVERFLOW 0x080u     /* >1200 tracks; extra dropped */
 #define VELO_STATUS_BAD_BANK 0x100u            /* bank index != sensor index, or the bank reaches past its event; bank skipped */
-#define VELO_STATUS_UNSORTED_BANK 0x200u       /* a bank's SP addresses do not strictly increase (set by estimate_input_size;
-                                                  masked clustering checks bank order only in events that carry it) */
+#define VELO_STATUS_UNSORTED_BANK 0x200u       /* a bank's SP addresses do not strictly increase (set by estimate_input_size) */
+/* Not an error: estimate_input_size sets this bit on every event it
+ * processes, after checking the order of the event's banks.  Masked
+ * clustering skips its own bank-order check only in events whose status
+ * word carries this bit and not UNSORTED_BANK; a status word estimate did
+ * not write (zeroed, or a caller's own) makes clustering check every bank. */
+#define VELO_STATUS_BANK_ORDER_CHECKED 0x80000000u
+/* The bits above that report a problem with the event. */
+#define VELO_STATUS_ERROR_MASK 0x3FFu
 
 /* ---- tuning block ----
  * Launch shape of one launcher call.  With grid_dim = 0 a per-event kernel
@@ -358,6 +365,14 @@ int velo_consolidate_velo_tracks_launch(const velo_consolidate_velo_tracks_args*
 /* ---- library information ---- */
 const char* velo_hlt_version(void);
 const char* velo_error_string(int code);
+/* sizeof of a struct declared in this library's headers (velo_hlt.h,
+ * velo_pv.h, velo_ghost.h, velo_checker.h, velo_mdf.h, velo_sequence.h) as
+ * the library was built, by its typedef name; 0 for an unknown name.  A
+ * binding compares its own mirror of every struct the library reads or
+ * writes through a pointer and refuses a library that differs, so that a
+ * library built from a newer header never writes past a caller's struct.
+ * No reference counterpart. */
+size_t velo_abi_struct_size(const char* name);
 
 #ifdef __cplusplus
 }

@@ -60,6 +60,23 @@ int velo_mdf_read_banks(const char* const* paths, uint32_t n_paths, uint32_t max
 
 void velo_mdf_free(velo_mdf_events* ev);
 
+/* Per-event files of one bank type in a folder, into the same layout
+ * (main/src/InputTools.cpp:178-211 read_folder, with list_folder and its
+ * natural order :118-157).  File names are "N.bin" or "N_M.bin", ordered as
+ * the number pairs (0, N) and (N, M); event i of the output is file
+ * (start_event + i) mod (number of files), so a request beyond the folder
+ * starts again from its first file, as the reference does.  n_events = 0
+ * reads each file once.  n_records receives the number of files.  Returns
+ * 0, or -1 with a message in err: missing or empty folder, a file name of
+ * another form, an unreadable file.  Free with velo_mdf_free. */
+int velo_read_folder(const char* folder, uint32_t n_events, uint32_t start_event, velo_mdf_events* out, char* err,
+                     size_t err_len);
+
+/* check_velopix_events (main/src/CheckVP.cpp:5-52): the number of banks,
+ * over all events, whose sensor index differs from their position; -1 when
+ * an event's bank table or a bank offset reaches past the event. */
+int64_t velo_check_vp_events(const char* events, const uint32_t* offsets, uint32_t n_events);
+
 /* The MDF record checksum (genChecksum flag 1) of len bytes. */
 uint32_t velo_mdf_checksum(const void* data, size_t len);
 

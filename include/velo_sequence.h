@@ -14,6 +14,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "velo_checker.h"
 #include "velo_hlt.h"
 #include "velo_pv.h"
 
@@ -130,6 +131,35 @@ void* velo_sequence_stream(const velo_sequence* s); /* the hipStream_t of the se
  * step names, and returns the number of steps. */
 int velo_sequence_enable_step_timing(velo_sequence* s, int enable);
 int velo_sequence_step_times(const velo_sequence* s, float* ms, const char** names, int max_steps);
+
+/* Device arena bytes (velo_sequence_config.reserve_mb covers it) that a
+ * batched run of n_events events holding events_bytes raw bytes needs: the
+ * scheduler lays the sequence out on the batched run's bounds (the
+ * set_arguments_size of every step, SequenceVisitor.cuh:11-22, and the
+ * memory manager's first-fit placement, MemoryManager.cuh:8-170) without
+ * launching anything; the result is its high-water mark.  A plain run of
+ * the same events needs less (it sizes by totals read between steps).  The
+ * reference has no such query: its arena is a fixed command-line
+ * reservation.  No device call, no allocation. */
+size_t velo_workspace_bytes(uint32_t n_events, size_t events_bytes, int run_velo_kalman_fit, int run_pv_beamline);
+
+/* The per-event status words (VELO_STATUS_* of velo_hlt.h) of the last
+ * run: up to max_events of them copied to host_status (may be NULL), and
+ * the OR of the error bits (VELO_STATUS_ERROR_MASK) over all events to
+ * *combined (may be NULL).  Returns the number of events of the last run,
+ * or -1.  Waits for the sequence's stream.  No reference counterpart: the
+ * reference only asserts on its capacities in debug builds. */
+int velo_status(const velo_sequence* s, uint32_t* host_status, uint32_t max_events, uint32_t* combined);
+
+/* Checks the last run's VELO tracks (copy_outputs_to_host set) against MC
+ * truth: every step's SequenceVisitor::check in sequence order
+ * (Stream::run_monte_carlo_test, Stream.cu:85-105; the VELO checker,
+ * VeloSequenceCheckers_impl.cuh:8-24).  The truth covers input events
+ * start_event, start_event + 1, ...; output slots map to input events
+ * through the event list (the global event cut may have dropped some).
+ * Returns 0, or -1 with a message in err. */
+int velo_sequence_check(velo_sequence* s, const velo_mc_truth* truth, uint32_t start_event,
+                        velo_checker_result* result, char* err, size_t err_len);
 
 #ifdef __cplusplus
 }

@@ -28,7 +28,9 @@ STATUS_BITS = {
     "SP_INDEX_TRUNCATED": 0x001, "CANDIDATES_OVERFLOW": 0x002, "MODULE_HITS_OVERFLOW": 0x004,
     "EVENT_HITS_OVERFLOW": 0x008, "TTF_OVERFLOW": 0x010, "TRACKLETS_OVERFLOW": 0x020,
     "WEAK_OVERFLOW": 0x040, "TRACKS_OVERFLOW": 0x080, "BAD_BANK": 0x100, "UNSORTED_BANK": 0x200,
+    "BANK_ORDER_CHECKED": 0x80000000,  # not an error: estimate_input_size checked the event's bank order
 }
+STATUS_ERROR_MASK = 0x3FF
 
 _lib = None
 _P = C.c_void_p

@@ -299,6 +299,7 @@ void oracle_estimate_input_size(
     uint32_t* cand_e = candidates + e * OR_MAX_CANDIDATES_EVENT;
     for (uint32_t m = 0; m < OR_N_MODULES; ++m) est_e[m] = num_e[m] = 0;
     uint32_t n_cand = 0;
+    status[e] |= OR_STATUS_BANK_ORDER_CHECKED;
     const raw_event ev = make_event(events + offsets[e], (offsets[e + 1] - offsets[e]) / 4u);
     for (uint32_t b = 0; b < ev.n_banks && b < OR_N_SENSORS; ++b) {
       const raw_bank bank = make_bank(&ev, b);

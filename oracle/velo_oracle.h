@@ -55,6 +55,8 @@ extern "C" {
 #define OR_STATUS_TRACKS_OVERFLOW 0x080u     /* >1200 tracks                    */
 #define OR_STATUS_BAD_BANK 0x100u            /* bank index != sensor index      */
 #define OR_STATUS_UNSORTED_BANK 0x200u       /* SP addresses not increasing     */
+#define OR_STATUS_BANK_ORDER_CHECKED 0x80000000u /* estimate ran on the event (not an error) */
+#define OR_STATUS_ERROR_MASK 0x3FFu
 
 #pragma pack(push, 1)
 typedef struct {

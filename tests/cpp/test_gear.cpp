@@ -5,6 +5,7 @@
 
 #include "gear/Argument.h"
 #include "gear/ArgumentManager.h"
+#include "gear/CpuHandler.h"
 #include "gear/Scheduler.h"
 
 static int failures = 0;
@@ -117,10 +118,64 @@ void test_scheduler()
   CHECK(s.memory_manager.is_reserved("b_tmp"));
 }
 
+// CPU_ALGORITHM: a host function as a sequence step between device steps.
+static int host_sum(const unsigned* values, unsigned n, unsigned* total)
+{
+  unsigned s = 0;
+  for (unsigned i = 0; i < n; ++i) s += values[i];
+  *total = s;
+  return (int) n;
+}
+CPU_ALGORITHM(host_sum, cpu_sum_t, ARGUMENTS(b_tmp, c_out))
+
+using cpu_seq_t = std::tuple<alg0, cpu_sum_t, alg2>;
+
+struct SumVisitor {
+  unsigned host_values[4] = {1, 2, 3, 4};
+  unsigned total = 0;
+  int returned = -1;
+  std::string steps;
+  template<typename T>
+  void set_arguments_size(typename T::arguments_t arguments)
+  {
+    if constexpr (std::is_same_v<T, alg0>) {
+      arguments.template set_size<a_in>(16);
+      arguments.template set_size<b_tmp>(4);
+    }
+    if constexpr (std::is_same_v<T, alg2>) arguments.template set_size<d_tmp>(2);
+  }
+  template<typename T>
+  void visit(T& state, const typename T::arguments_t& arguments)
+  {
+    steps += T::name;
+    steps += ";";
+    if constexpr (std::is_same_v<T, cpu_sum_t>) {
+      // the step's device arguments are reserved around it like any other's
+      CHECK(arguments.template size<b_tmp>() == 4 * sizeof(unsigned));
+      returned = state.invoke(host_values, 4u, &total);
+    }
+  }
+};
+
+void test_cpu_algorithm()
+{
+  static char arena[1 << 16];
+  Scheduler<cpu_seq_t, std::tuple<>> s;
+  s.initialize(false, sizeof(arena), arena);
+  SumVisitor v;
+  Sch::NoHook hook;
+  Sch::run_sequence(s, v, hook, std::tuple<> {}, std::tuple<> {});
+  CHECK(v.steps == "alg0;cpu_sum_t;alg2;");
+  CHECK(v.total == 10 && v.returned == 4);
+  CHECK(std::string(cpu_sum_t::name) == "cpu_sum_t");
+  CHECK(!s.memory_manager.is_reserved("b_tmp")); // freed after its last use (cpu_sum_t)
+}
+
 int main()
 {
   test_memory_manager();
   test_scheduler();
+  test_cpu_algorithm();
   if (failures) return 1;
   std::printf("ok\n");
   return 0;

@@ -80,3 +80,19 @@ def test_ghost_and_tuning_structs_match_header():
     h = _header_structs("velo_hlt.h")
     assert [f for f, _ in nat.Tuning._fields_] == h["velo_tuning"]
     assert [f for f, _ in nat.Constants._fields_] == h["velo_constants"]
+
+
+def test_workspace_bytes_bounds():
+    """velo_workspace_bytes (no GPU): the batched layout covers at least the
+    bound-sized buffers, grows with events and raw bytes, and the opt-in
+    Kalman and PV steps never need less (their buffers may reuse space that
+    earlier steps freed)."""
+    from cuda_hlt_amd.stream import workspace_bytes
+    n, raw = 1000, 30_000_000
+    b = workspace_bytes(n, raw)
+    tracks = n * 1200
+    assert b >= tracks * (54 + 26 * 16) + (raw // 2) * 24
+    assert workspace_bytes(2 * n, raw) > b and workspace_bytes(n, 2 * raw) > b
+    assert workspace_bytes(n, raw, velo_kalman_fit=True) >= b
+    assert workspace_bytes(n, raw, pv_beamline=True) >= workspace_bytes(n, raw, velo_kalman_fit=True)
+    assert b % 256 == 0

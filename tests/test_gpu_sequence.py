@@ -88,3 +88,33 @@ def test_sequence_with_kalman_fit(sequence, minbias_200, geometry, module_zs):
     np.testing.assert_array_equal(out.kalman[: 49 * T], want[: 49 * T])
     sequence.run(ev.n_events, host_events=ev.events, host_offsets=ev.offsets)
     assert sequence.host_tracks().kalman is None
+
+
+def test_status_of_the_last_run(sequence, minbias_200, geometry, module_zs):
+    """velo_status: the per-event status words equal the oracle's (with
+    estimate's bank-order bit) and no error bit is set on minimum bias."""
+    ev = minbias_200
+    sequence.run(ev.n_events, host_events=ev.events, host_offsets=ev.offsets)
+    words, combined = sequence.status()
+    ref = po.pipeline(ev.events, ev.offsets, geometry, module_zs)
+    np.testing.assert_array_equal(words, ref.status)
+    assert combined == 0 and np.all(words & po.STATUS_BITS["BANK_ORDER_CHECKED"])
+
+
+def test_sequence_check_equals_the_host_checker(sequence, minbias_200):
+    """SequenceVisitor::check<consolidate_velo_tracks_t> through the C ABI
+    (velo_sequence_check) gives the checker's result on the copied-back
+    tracks, also for events past the first of the MC truth."""
+    from cuda_hlt_amd import checker
+    ev = minbias_200
+    sequence.run(ev.n_events, host_events=ev.events, host_offsets=ev.offsets)
+    got = sequence.check(ev.mcps, ev.mc_hits)
+    want = checker.check(ev, sequence.host_tracks())
+    assert (got.events, got.tracks, got.ghosts) == (want.events, want.tracks, want.ghosts)
+    assert got.categories == want.categories
+    sub = ev.subset(50, 120)
+    sequence.run(sub.n_events, host_events=sub.events, host_offsets=sub.offsets)
+    got = sequence.check(ev.mcps, ev.mc_hits, start_event=50)
+    want = checker.check(sub, sequence.host_tracks())
+    assert got.categories == want.categories and got.ghosts == want.ghosts
+

@@ -182,6 +182,22 @@ def test_masked_clustering_from_candidate_list(stages, consts, request, name, bl
     _check_clustering(stages, consts, c, (block, tile), (c.cnum, c.cand))
 
 
+@pytest.mark.parametrize("status_word", [None, 0x1, 0x200])
+def test_masked_clustering_status_not_from_estimate(stages, consts, chain_scrambled, status_word):
+    """A status word estimate_input_size did not write (zeroed, or a caller's
+    own bits without BANK_ORDER_CHECKED) keeps the bank-order check, so
+    unsorted banks still cluster exactly."""
+    c = chain_scrambled
+    st0 = None if status_word is None else np.full(c.n, status_word, np.uint32)
+    num, cont, st = stages.clustering(c.ev.events, c.ev.offsets, c.starts, consts, (0, 0), (c.cnum, c.cand), st0)
+    total = int(c.starts[c.n * N_MODULES])
+    np.testing.assert_array_equal(num[: c.n * N_MODULES], c.nums)
+    for s in range(4):
+        np.testing.assert_array_equal(cont[s * total: (s + 1) * total][c.mask],
+                                      c.cont_raw[s * total: (s + 1) * total][c.mask], err_msg=f"slot {s}")
+    assert st0 is not None or not np.any(st & po.STATUS_BITS["BANK_ORDER_CHECKED"])
+
+
 def test_masked_clustering_full_candidate_list(stages, consts, chain):
     """An event whose list is at capacity may have lost candidates: it takes
     the bank path, so nothing is lost."""
