@@ -34,18 +34,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(S, H, C, T, Ht, N):
+def algorithmic_bytes(S, H, C, T, Ht, N, W=0, A=0):
     """HBM bytes each step must move at least once (per batch).
-    S SP words, H clusters, C candidates, T tracks, Ht hits on tracks, N events."""
+    S SP words, H clusters, C candidates, T tracks, Ht hits on tracks, N
+    events, W weak tracks left by the search, A of them accepted."""
     raw = 4 * S + 2504 * N
     return {
-        "velo_estimate_input_size_t": raw + 4 * C + N * (2 * 52 * 4 + 4),
+        # raw words in; estimates, cluster counts, candidate list and its count, status out
+        "velo_estimate_input_size_t": raw + 4 * C + N * (2 * 52 * 4 + 4 + 4),
         "prefix_sum_velo_clusters_t": 8 * (52 * N),
-        "velo_masked_clustering_t": raw + 16 * H + N * (2 * 52 * 4),
+        # raw words and the candidate list in; clusters (x, y, z, id) and counts out
+        "velo_masked_clustering_t": raw + 4 * C + 4 * N + 16 * H + N * (2 * 52 * 4),
         "velo_calculate_phi_and_sort_t": 16 * H + 4 * H + 16 * H + 4 * H,
         "velo_fill_candidates_t": 4 * H + 8 * H,
         "velo_search_by_triplet_t": 16 * H + 12 * H + 54 * T,
-        "velo_weak_tracks_adder_t": 0,
+        # per weak track its 3 hit numbers, their used flags and x, y, z; an
+        # accepted one appended as a 54-byte track; per event two counters
+        "velo_weak_tracks_adder_t": W * (6 + 3 + 36) + 54 * A + 8 * N,
         "copy_and_prefix_sum_single_block_velo_t": 8 * N,
         "copy_velo_track_hit_number_t": 2 * T + 4 * T,
         "prefix_sum_velo_track_hit_number_t": 8 * T,
@@ -84,51 +89,78 @@ def measured_traffic(path, step, events_per_launch):
     return round(total) if total else None
 
 
-def cpu_baseline(ev, geometry, zs, budget_s=12.0, threads=None):
-    """The oracle (C) on the first events of the workload: single-threaded,
-    then event-parallel over the host's cores (chunks of 32 events per
-    task; the C calls release the GIL)."""
+def host_cpu_info():
+    """Model name, physical cores and hardware threads of the host (lscpu's
+    fields, read from /proc/cpuinfo) and the threads this process may use."""
+    model, cores, threads = "", set(), 0
+    phys = core = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name" and not model:
+                model = v
+            elif k == "processor":
+                threads += 1
+            elif k == "physical id":
+                phys = v
+            elif k == "core id":
+                core = v
+                cores.add((phys, core))
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return {"model": model, "physical_cores": len(cores) or None, "hardware_threads": threads or None,
+            "usable_threads": usable}
+
+
+def cpu_baseline(ev, geometry, zs, budget_s=18.0, threads=None, reps=3):
+    """The oracle (C) on the first events of the workload, as BASELINE.md
+    plans it: the median of `reps` timed passes single-threaded, then the
+    median of `reps` passes event-parallel over the host threads this
+    process may use (at most 16; tasks of 32 events; the C calls release
+    the GIL).  Input is in memory; events/s over each pass."""
     from concurrent.futures import ThreadPoolExecutor
     from oracle import pyoracle as po
-    n = 200
+    # size the single-threaded sample to about budget / (2 reps) seconds in all
+    n = 100
     while True:
         sub = ev.subset(0, min(n, ev.n_events))
         t0 = time.perf_counter()
         po.pipeline_c(sub.events, sub.offsets, geometry, zs)
         dt = time.perf_counter() - t0
-        if dt > budget_s / 4 or n >= ev.n_events:
+        target = budget_s / (2 * reps)
+        if dt > target / 2 or n >= ev.n_events:
             break
-        n = min(ev.n_events, int(n * max(2.0, budget_s / 4 / max(dt, 1e-3))))
-    one = sub.n_events / dt
-    threads = threads or min(16, os.cpu_count() or 1)
-    m = min(ev.n_events, max(sub.n_events, int(one * threads * budget_s / 4)))
+        n = min(ev.n_events, int(n * max(2.0, target / max(dt, 1e-3))))
+    one = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        po.pipeline_c(sub.events, sub.offsets, geometry, zs)
+        one.append(sub.n_events / (time.perf_counter() - t0))
+    one_med = float(np.median(one))
+    info = host_cpu_info()
+    threads = threads or min(16, info["usable_threads"])
+    m = min(ev.n_events, max(sub.n_events, int(one_med * threads * budget_s / (2 * reps))))
     big = ev.subset(0, m)
     chunks = [big.subset(a, min(a + 32, m)) for a in range(0, m, 32)]
-    # passes over the sample until about a second of wall time (tens of
-    # CPU-seconds on the host's threads) has been measured
-    passes = 0
-    t0 = time.perf_counter()
+    multi = []
     with ThreadPoolExecutor(threads) as pool:
-        while True:
+        for _ in range(reps):
+            t0 = time.perf_counter()
             list(pool.map(lambda c: po.pipeline_c(c.events, c.offsets, geometry, zs), chunks))
-            passes += 1
-            dtm = time.perf_counter() - t0
-            if dtm >= budget_s / 12 or passes >= 20:
-                break
-    cpu = ""
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                cpu = line.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
-    return {"value": round(passes * m / dtm, 1), "unit": "events/s", "cores": threads, "kind": "port",
-            "sample": f"first {m} events of the batch x {passes} passes, C oracle in {threads} threads "
-                      f"(32-event tasks), {dtm:.1f} s wall",
-            "single_thread": {"value": round(one, 1), "cores": 1,
-                              "sample": f"first {sub.n_events} events, {dt:.1f} s"},
-            "host_cpu": cpu, "host_threads_available": os.cpu_count()}
+            multi.append(m / (time.perf_counter() - t0))
+    return {"value": round(float(np.median(multi)), 1), "unit": "events/s", "cores": threads, "kind": "port",
+            "sample": f"first {m} events of the timed workload, C oracle (single-threaded restatement of the "
+                      f"reference's clustering, triplet search, scans and consolidation) in {threads} threads "
+                      f"(32-event tasks); median of {reps} passes",
+            "passes": [round(v, 1) for v in multi],
+            "single_thread": {"value": round(one_med, 1), "cores": 1, "passes": [round(v, 1) for v in one],
+                              "sample": f"first {sub.n_events} events, median of {reps} passes"},
+            "host_cpu": info}
 
 
 GHOST_MODEL = ROOT / "cuda_hlt_amd" / "filters" / "velo_ghost_mi355x.npz"
@@ -271,6 +303,98 @@ def extended_run(args, ev, seq, raw, batch_offsets, B, dev):
             "vertices_per_event": round(V / max(n, 1), 3), "kernels": kernels}
 
 
+def oracle_sizes(ev, geometry, zs, sample=500):
+    """Candidate-list entries of the batch (the oracle's estimate, exact)
+    and its weak tracks before and after the weak-track adder (the oracle's
+    search on the first `sample` events, scaled to the batch)."""
+    from oracle import pyoracle as po
+    _, _, cnum, _, _ = po.estimate_input_size(ev.events, ev.offsets)
+    sub = ev.subset(0, min(sample, ev.n_events))
+    n = sub.n_events
+    est, _, _, _, _ = po.estimate_input_size(sub.events, sub.offsets)
+    starts = po.prefix_sum(est)
+    nums, cont, _ = po.masked_clustering(sub.events, sub.offsets, starts, geometry)
+    cont, _ = po.calculate_phi_and_sort(n, starts, nums, cont)
+    h0, h2 = po.fill_candidates(n, cont, starts, nums)
+    tracks, weak, used, atomics, _ = po.search_by_triplet(n, cont, starts, nums, h0, h2, zs)
+    _, atomics_w, _ = po.weak_tracks_adder(n, cont, starts, tracks, weak, used, atomics)
+    w = int(sum(atomics[n + 4 * e] for e in range(n)))
+    a = int(atomics_w[:n].sum() - atomics[:n].sum())
+    scale = ev.n_events / n
+    return {"candidates": int(cnum.sum()), "weak_tracks": round(w * scale), "weak_accepted": round(a * scale)}
+
+
+def verify_slice(ev, host, geometry, zs, n_check):
+    """The first n_check events of a timed batch's output against the
+    oracle, bit for bit (per-event track counts, hit offsets, hits,
+    states): outputs of an event do not depend on the other events."""
+    from oracle import pyoracle as po
+    if n_check <= 0:
+        return "not checked"
+    sub = ev.subset(0, n_check)
+    ref = po.pipeline(sub.events, sub.offsets, geometry, zs)
+    n, N = n_check, host.n_events
+    T_all, Ht_all = int(host.atomics[2 * N]), int(host.hit_number[int(host.atomics[2 * N])])
+    T = int(host.atomics[N + n])
+    Ht = int(host.hit_number[T])
+    ok = np.array_equal(host.atomics[:n], ref.atomics[:n]) and T == int(ref.atomics[2 * n])
+    ok = ok and np.array_equal(host.hit_number[: T + 1], ref.hit_number[: T + 1])
+    hits = host.track_hits[: 16 * Ht_all].view(np.uint32).reshape(4, Ht_all)[:, :Ht]
+    ok = ok and np.array_equal(hits, ref.track_hits[: 16 * Ht].view(np.uint32).reshape(4, Ht))
+    states = [host.states[4 * T_all * k: 4 * T_all * k + 4 * T] for k in range(12)]
+    states.append(host.states[48 * T_all: 48 * T_all + T])
+    ref_states = [ref.states[4 * T * k: 4 * T * (k + 1)] for k in range(12)] + [ref.states[48 * T: 49 * T]]
+    ok = ok and all(np.array_equal(a, b) for a, b in zip(states, ref_states))
+    if not ok:
+        raise SystemExit(f"bench: the first {n_check} events of a timed batch differ from the oracle")
+    return f"first {n_check} events bit-exact with the oracle ({T} tracks)"
+
+
+def pcie_run(args, ev, geometry, zs, B, tuning):
+    """The reference's data path: raw events in (pinned) host memory copied
+    in by init_event_list, the consolidated tracks copied back
+    (InitEventListVisitor.cu:43-78, ConsolidateVeloVisitor.cu:47-66), one
+    launch per batch.  Not the headline value."""
+    import torch
+    from cuda_hlt_amd.stream import VeloSequence
+    steps = args.pcie_steps
+    if steps <= 0:
+        return None
+    host = []
+    for b in range(steps + 1):
+        sub = ev.subset(b * B, (b + 1) * B)
+        e = torch.from_numpy(sub.events).pin_memory()
+        o = torch.from_numpy(sub.offsets.view(np.int32)).pin_memory()
+        host.append((e, o))
+    seq = VeloSequence(geometry, zs, max_events=B, reserve_mb=int(os.environ.get("VELO_RESERVE_MB", "8192")))
+
+    def run(b):
+        e, o = host[b]
+        return seq.run(B, host_events=e.numpy(), host_offsets=o.numpy().view(np.uint32), copy_outputs=True,
+                       tuning=tuning)
+
+    run(steps)
+    torch.cuda.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    h2d = d2h = 0
+    for b in range(steps):
+        t1 = time.perf_counter()
+        o = run(b)
+        lat.append(time.perf_counter() - t1)
+        h2d += host[b][0].numel() + 4 * (B + 1)
+        d2h += 4 * (2 * B + 1) + 4 * (o.total_tracks + 1) + 16 * o.total_track_hits + 49 * o.total_tracks + \
+            o.total_tracks
+    elapsed = time.perf_counter() - t0
+    seq.close()
+    return {"value": round(steps * B / elapsed, 1), "unit": "events/s", "steps": steps,
+            "ms_per_step": round(1e3 * elapsed / steps, 3),
+            "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 3),
+                           "p99": round(1e3 * float(np.percentile(lat, 99)), 3)},
+            "h2d_bytes_per_step": h2d // steps, "d2h_bytes_per_step": d2h // steps,
+            "workload": f"plain sequence on {B}-event batches in pinned host memory, outputs copied back"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -283,14 +407,17 @@ def main():
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="first run one int32 add-one kernel over this many MiB (reads and writes exactly that "
                          "much): the known-size reference for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE")
-    ap.add_argument("--batched-events", type=int, default=0,
-                    help="events of the batched run (uneven nu ~ U(2,11), 50000 measured before); 0 skips "
-                         "it.  Off by default: a batched run aborted once on the GPU without a shown cause")
+    ap.add_argument("--batched-events", type=int, default=50000,
+                    help="events of the batched run (uneven nu ~ U(2,11)); 0 skips it")
     ap.add_argument("--batched-steps", type=int, default=5)
     ap.add_argument("--batched-tuning", default=None,
                     help="launch shapes of the batched run (cuda_hlt_amd.autotune JSON)")
     ap.add_argument("--ghost-model", default=None,
                     help="ghost filter for the batched run (.npz from python -m cuda_hlt_amd.ghost)")
+    ap.add_argument("--verify-events", type=int, default=1000,
+                    help="events of timed batch 0 compared bit for bit with the oracle")
+    ap.add_argument("--pcie-steps", type=int, default=5,
+                    help="launches of the host-input run (input copied in, outputs copied back); 0 skips it")
     ap.add_argument("--no-extended", action="store_true",
                     help="skip the run with the global event cut, velo_kalman_fit and beamline PV finding")
     ap.add_argument("--traffic", default=str(ROOT / "profiles" / "traffic_minbias_10k.json"),
@@ -364,12 +491,16 @@ def main():
     from cuda_hlt_amd import checker
     from cuda_hlt_amd import native as nat
     o0 = run(0, copy_outputs=True)
-    chk = checker.check(ev.subset(0, B), seq.host_tracks())
+    host0 = seq.host_tracks()
+    chk = checker.check(ev.subset(0, B), host0)
     host_nums = nat.d2h(o0.dev_module_cluster_num, 4 * B * 52).view(np.uint32)
     H = int(host_nums.sum())
     S = int(ev.sp_counts()[:B].sum())
-    C_cands = 0
-    ab = algorithmic_bytes(S, H, C_cands, o0.total_tracks, o0.total_track_hits, B)
+    sizes = oracle_sizes(ev.subset(0, B), geometry, zs)
+    verified = verify_slice(ev, host0, geometry, zs, args.verify_events)
+    log(f"[bench] batch 0: {verified}")
+    ab = algorithmic_bytes(S, H, sizes["candidates"], o0.total_tracks, o0.total_track_hits, B,
+                           sizes["weak_tracks"], sizes["weak_accepted"])
     per_step = {k: v / args.steps for k, v in steps_ms.items()}
     dominant = max((k for k in per_step if k != "end"), key=lambda k: per_step[k])
     kernels = {}
@@ -409,7 +540,10 @@ def main():
         "efficiency": {"category": checker.HEADLINE, "value": round(chk.efficiency(), 5),
                        "ghost_rate": round(chk.ghost_rate, 5), "events": B, "tracks": chk.tracks},
         "kernels": kernels,
-        "batch": {"sp_words": S, "clusters": H, "tracks": o0.total_tracks, "track_hits": o0.total_track_hits},
+        "batch": {"sp_words": S, "clusters": H, "tracks": o0.total_tracks, "track_hits": o0.total_track_hits,
+                  "candidates": sizes["candidates"], "weak_tracks_est": sizes["weak_tracks"],
+                  "weak_accepted_est": sizes["weak_accepted"]},
+        "verified": verified,
     }
     if not args.no_extended:
         try:
@@ -418,6 +552,7 @@ def main():
             result["extended"] = {"error": repr(e)}
     seq.close()
     del raw, batch_offsets
+    result["pcie_inclusive"] = pcie_run(args, ev, geometry, zs, B, tuning)
     if args.batched_events:
         result["batched"] = batched_run(args, geometry, zs)
     if not args.no_cpu_baseline:

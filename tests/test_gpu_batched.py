@@ -8,7 +8,7 @@ import torch
 
 from oracle import pyoracle as po
 
-pytestmark = [pytest.mark.gpu, pytest.mark.skip(reason="the batched sequence run aborted the test process once on the GPU (SIGABRT on the main thread, no GPU fault and no runtime message reported) and its cause is not shown yet; it stays off the GPU until it is")]
+pytestmark = pytest.mark.gpu
 
 
 def lpt_order(offsets):
@@ -107,3 +107,23 @@ def test_batched_needs_resident_input(seq, minbias_200):
     from cuda_hlt_amd import native as nat
     with pytest.raises(nat.VeloError, match="device-resident"):
         seq.run(minbias_200.n_events, host_events=minbias_200.events, host_offsets=minbias_200.offsets, batched=True)
+
+
+def test_batched_run_fits_the_workspace_it_asks_for(gpu, geometry, module_zs):
+    """An arena of exactly velo_workspace_bytes (rounded up to a MiB) holds
+    a batched run, whose scheduler reaches the same high-water mark."""
+    from cuda_hlt_amd import gen
+    from cuda_hlt_amd.stream import VeloSequence, workspace_bytes
+    ev = gen.uneven(300, seed=8)
+    need = workspace_bytes(ev.n_events, ev.events.nbytes)
+    s = VeloSequence(geometry, module_zs, max_events=ev.n_events, reserve_mb=-(-need // 2 ** 20))
+    try:
+        raw = torch.from_numpy(ev.events).cuda()
+        off = torch.from_numpy(ev.offsets.view(np.int32)).cuda()
+        o = s.run(ev.n_events, dev_events=raw.data_ptr(), dev_offsets=off.data_ptr(), events_size=ev.events.nbytes,
+                  batched=True)
+        assert o.memory_required == need
+        ref = po.pipeline(ev.events, ev.offsets, geometry, module_zs)
+        compare(s.host_tracks(), ref, ev.n_events)
+    finally:
+        s.close()
