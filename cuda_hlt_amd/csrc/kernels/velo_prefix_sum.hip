@@ -8,6 +8,8 @@
 // written per element.
 #include <hip/hip_fp16.h>
 
+#include <mutex>
+
 #include "velo_device.h"
 #include "velo_ghost_device.h"
 
@@ -339,7 +341,54 @@ consolidate_kernel(velo_consolidate_velo_tracks_args a)
 
 using namespace velo;
 
+// Scratch of velo_prefix_sum_noalloc_launch: the per-tile sums of its scans
+// (256 KiB on every GPU the library runs on, held until the process ends).
+constexpr uint32_t kNoAllocMaxTiles = 65536;
+static_assert((uint64_t) kNoAllocMaxTiles * kScanTile == VELO_PREFIX_SUM_NOALLOC_MAX_ELEMENTS,
+              "allocation-free scan bound");
+static __device__ __align__(256) uint32_t scratch[kNoAllocMaxTiles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+uint32_t* g_scratch[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device: host code never passes the array's name itself to a kernel.
+hipError_t scratch_on_current_device(uint32_t** out)
+{
+  int dev = 0;
+  hipError_t rc = hipGetDevice(&dev);
+  if (rc != hipSuccess) return rc;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (rc != hipSuccess) return rc;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch[dev] = static_cast<uint32_t*>(p);
+  }
+  *out = g_scratch[dev];
+  return hipSuccess;
+}
+} // namespace
+
 extern "C" {
+
+int velo_prefix_sum_noalloc_launch(uint32_t* dev_array, uint32_t array_size, const uint32_t* dev_count, void* stream)
+{
+  if (!dev_array || array_size > VELO_PREFIX_SUM_NOALLOC_MAX_ELEMENTS) return (int) hipErrorInvalidValue;
+  uint32_t* aux = nullptr;
+  const hipError_t rc = scratch_on_current_device(&aux);
+  if (rc != hipSuccess) return (int) rc;
+  velo_prefix_sum_args a {};
+  a.dev_array = dev_array;
+  a.dev_auxiliary_array = aux;
+  a.array_size = array_size;
+  a.tuning.dev_count = dev_count;
+  return velo_prefix_sum_launch(&a, stream);
+}
 
 uint32_t velo_prefix_sum_aux_size(uint32_t n)
 {

@@ -188,6 +188,8 @@ def lib():
             f = getattr(L, name)
             f.argtypes = [C.POINTER(st), P]
             f.restype = C.c_int
+        L.velo_prefix_sum_noalloc_launch.argtypes = [P, U32, P, P]
+        L.velo_prefix_sum_noalloc_launch.restype = C.c_int
         L.velo_prefix_sum_aux_size.argtypes = [U32]
         L.velo_prefix_sum_aux_size.restype = U32
         L.velo_constants_init.argtypes = [C.c_char_p, C.c_size_t, P, C.POINTER(Constants)]

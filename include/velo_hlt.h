@@ -194,6 +194,23 @@ typedef struct {
 uint32_t velo_prefix_sum_aux_size(uint32_t array_size);
 int velo_prefix_sum_launch(const velo_prefix_sum_args* a, void* stream);
 
+/* Allocation-free variant of velo_prefix_sum_launch.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The per-tile sums live in a device array inside the library
+ * (256 KiB on each GPU it runs on, kept until the process ends) instead of
+ * a caller's dev_auxiliary_array; nothing is allocated and nothing needs
+ * to be zeroed.  Same scan as velo_prefix_sum_launch: dev_array[0,
+ * array_size) scanned in place, the total written to dev_array[array_size];
+ * dev_count (may be NULL) gives the element count on the device, array_size
+ * being then the capacity.  array_size above
+ * VELO_PREFIX_SUM_NOALLOC_MAX_ELEMENTS returns hipErrorInvalidValue.  Every
+ * call on one GPU uses that GPU's one array: calls that overlap on one GPU
+ * (on different streams, or from different threads without ordering) race
+ * and give wrong sums; order them on one stream. */
+#define VELO_PREFIX_SUM_NOALLOC_MAX_ELEMENTS 134217728u /* 65536 tiles of 2048 */
+int velo_prefix_sum_noalloc_launch(uint32_t* dev_array, uint32_t array_size, const uint32_t* dev_count, void* stream);
+
 /* ---- K5: masked_velo_clustering (MaskedVeloClustering.cu:48-464) ----
  * Clusters of module m are written to [start_m, start_m + num_m) in bank,
  * SP, cluster order (isolated SP: LUT cluster 0 then 1; otherwise the
