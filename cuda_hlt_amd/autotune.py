@@ -73,13 +73,15 @@ def _outputs(seq, n):
             nat.d2h(o.dev_states, 49 * T).tobytes())
 
 
-def tune(seq, n_events, dev_events, dev_offsets, repeats=3, log=print):
+def tune(seq, n_events, dev_events, dev_offsets, repeats=3, log=print, outputs=_outputs):
     """Per step: workgroup size x LDS tile (SPACE), then the persistent grid
     at the best shape (BATCHED_GRIDS, 0 = one workgroup per event).
-    Returns {algorithm: (block_dim, lds_tile, grid_dim)} and the times."""
+    Returns {algorithm: (block_dim, lds_tile, grid_dim)} and the times.
+    outputs(seq, n) returns the run's consolidated output as comparable
+    bytes (the device buffers by default)."""
     seq.enable_step_timing(True)
     seq.run(n_events, dev_events=dev_events, dev_offsets=dev_offsets, copy_outputs=False)
-    reference = _outputs(seq, n_events)
+    reference = outputs(seq, n_events)
     best, table = {}, {}
 
     def time_cfg(alg, cfg):
@@ -101,7 +103,7 @@ def tune(seq, n_events, dev_events, dev_offsets, repeats=3, log=print):
         log(f"[autotune] {alg}: best block={cfg[0]} tile={cfg[1]} grid={cfg[2]} {times[cfg]:.3f} ms "
             f"(first shape {times.get(first, float('nan')):.3f})")
     seq.run(n_events, dev_events=dev_events, dev_offsets=dev_offsets, copy_outputs=False, tuning=best)
-    if _outputs(seq, n_events) != reference:
+    if outputs(seq, n_events) != reference:
         raise RuntimeError("tuned launch shapes changed the output")
     return best, table
 
@@ -118,7 +120,8 @@ def batched_reserve_mb(n_events, raw_bytes, velo_kalman_fit=False, pv_beamline=F
     return -(-b // 2 ** 20)
 
 
-def tune_batched(seq, n_events, dev_events, dev_offsets, events_size, dev_order, repeats=3, log=print):
+def tune_batched(seq, n_events, dev_events, dev_offsets, events_size, dev_order, repeats=3, log=print,
+                 outputs=_outputs):
     """Launch shapes for uneven batches: per step, workgroup size x LDS tile,
     then the persistent grid, timed with HIP events in plain runs that use
     the batched path's largest-first order; checks the batched (graph) run
@@ -127,7 +130,7 @@ def tune_batched(seq, n_events, dev_events, dev_offsets, events_size, dev_order,
     kw = dict(dev_events=dev_events, dev_offsets=dev_offsets, events_size=events_size, copy_outputs=False,
               dev_work_order=dev_order)
     seq.run(n_events, **kw)
-    reference = _outputs(seq, n_events)
+    reference = outputs(seq, n_events)
     best = dict(BATCHED_DEFAULT)
     table = {}
 
@@ -149,7 +152,7 @@ def tune_batched(seq, n_events, dev_events, dev_offsets, events_size, dev_order,
         table[alg] = {f"{b}x{t}x{g}": round(v, 4) for (b, t, g), v in times.items()}
         log(f"[autotune] {alg}: best block={cfg[0]} tile={cfg[1]} grid={cfg[2]} {times[cfg]:.3f} ms")
     seq.run(n_events, tuning=best, batched=True, **kw)
-    if _outputs(seq, n_events) != reference:
+    if outputs(seq, n_events) != reference:
         raise RuntimeError("tuned batched run changed the output")
     return best, table
 

@@ -32,6 +32,12 @@ class _McTruth(C.Structure):
                 ("n_events", C.c_uint32)]
 
 
+class _ClusteringResult(C.Structure):
+    _fields_ = [("reconstructible", C.c_uint64), ("reconstructed", C.c_uint64), ("correct", C.c_uint64),
+                ("clones", C.c_uint64), ("ghosts", C.c_uint64), ("efficiency", C.c_float),
+                ("clone_fraction", C.c_float), ("ghost_fraction", C.c_float)]
+
+
 class _Result(C.Structure):
     _fields_ = [("events", C.c_uint64), ("tracks", C.c_uint64), ("ghosts", C.c_uint64),
                 ("ghost_per_event", C.c_double), ("categories", _Category * N_CATEGORIES)]
@@ -51,7 +57,14 @@ def _lib():
         L.velo_checker_run.restype = C.c_int
         nat.check_mirrors(L.velo_abi_struct_size, {_Category: "velo_checker_category",
                                                    _Result: "velo_checker_result",
-                                                   _McTruth: "velo_mc_truth"}, nat.LIB_PATH)
+                                                   _McTruth: "velo_mc_truth",
+                                                   _ClusteringResult: "velo_clustering_check_result"}, nat.LIB_PATH)
+        L.velo_x86_clustering.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_size_t,
+                                          C.POINTER(C.c_size_t)]
+        L.velo_x86_clustering.restype = C.c_int
+        L.velo_clustering_check.argtypes = [C.c_uint32] + [C.c_void_p] * 4 + [C.c_int, C.c_float,
+                                                                             C.POINTER(_ClusteringResult)]
+        L.velo_clustering_check.restype = C.c_int
         L.velo_mc_read_folder.argtypes = [C.c_char_p, C.c_uint32, C.c_uint32, C.POINTER(_McTruth), C.c_char_p,
                                           C.c_size_t]
         L.velo_mc_read_folder.restype = C.c_int
@@ -168,3 +181,78 @@ def read_mc_folder(folder, n_events: int = 0, start_event: int = 0):
     finally:
         L.velo_mc_free(C.byref(out))
     return mcps, hits
+
+
+# ---- clustering checker (ClusteringChecker.cpp:32-135) ----------------------
+
+@dataclass
+class ClusteringResult:
+    reconstructible: int
+    reconstructed: int
+    correct: int
+    clones: int
+    ghosts: int
+    efficiency: float
+    clone_fraction: float
+    ghost_fraction: float
+
+    def report(self) -> str:
+        return (f"Clustering: efficiency {100 * self.efficiency:.3f}% ({self.correct}/{self.reconstructible}), "
+                f"clones {100 * self.clone_fraction:.3f}%, ghosts {100 * self.ghost_fraction:.3f}%")
+
+
+def x86_clustering(events, offsets):
+    """The reference's x86 clustering of raw VP events (velo_x86_clustering):
+    (ids, id_offsets), the cluster LHCb IDs of event e at
+    ids[id_offsets[e]:id_offsets[e + 1]]."""
+    L = _lib()
+    ev = np.ascontiguousarray(events)
+    off = np.ascontiguousarray(offsets, np.uint32)
+    n = len(off) - 1
+    id_off = np.zeros(n + 1, np.uint32)
+    cap = max(ev.nbytes // 2, 1)  # at most two clusters per 4-byte SP word
+    ids = np.empty(cap, np.uint32)
+    total = C.c_size_t(0)
+    rc = L.velo_x86_clustering(ev.ctypes.data, off.ctypes.data, n, ids.ctypes.data, id_off.ctypes.data, cap,
+                               C.byref(total))
+    if rc != 0:
+        raise ValueError(f"velo_x86_clustering failed ({rc}, {total.value} clusters)")
+    return ids[: total.value].copy(), id_off
+
+
+def check_clustering(found_offsets, found_ids, expected_offsets, expected_ids, just_check_ids=True,
+                     allowed_distance_error=1.0) -> ClusteringResult:
+    """The reference's clustering report of found vs expected cluster IDs."""
+    L = _lib()
+    fo = np.ascontiguousarray(found_offsets, np.uint32)
+    fi = np.ascontiguousarray(found_ids, np.uint32)
+    xo = np.ascontiguousarray(expected_offsets, np.uint32)
+    xi = np.ascontiguousarray(expected_ids, np.uint32)
+    if len(fo) != len(xo):
+        raise ValueError("found and expected clusters cover different numbers of events")
+    r = _ClusteringResult()
+    rc = L.velo_clustering_check(len(fo) - 1, fo.ctypes.data, fi.ctypes.data if len(fi) else None, xo.ctypes.data,
+                                 xi.ctypes.data if len(xi) else None, int(just_check_ids),
+                                 float(allowed_distance_error), C.byref(r))
+    if rc != 0:
+        raise ValueError("velo_clustering_check rejected its input")
+    return ClusteringResult(int(r.reconstructible), int(r.reconstructed), int(r.correct), int(r.clones),
+                            int(r.ghosts), float(r.efficiency), float(r.clone_fraction), float(r.ghost_fraction))
+
+
+def container_ids(container, starts, nums, n_events, slot=3):
+    """Cluster LHCb IDs per event from a cluster container (u32[6 * total];
+    slot 3: the IDs in clustering order, slot 2: in phi order) and the
+    module starts / counts of masked clustering: (ids, id_offsets)."""
+    c = np.asarray(container).view(np.uint32)
+    st = np.asarray(starts).view(np.uint32)
+    nu = np.asarray(nums).view(np.uint32)
+    total = int(st[52 * n_events])
+    col = c[slot * total: (slot + 1) * total]
+    out, off = [], [0]
+    for e in range(n_events):
+        for m in range(52):
+            s0 = int(st[52 * e + m])
+            out.append(col[s0: s0 + int(nu[52 * e + m])])
+        off.append(off[-1] + int(nu[52 * e: 52 * e + 52].sum()))
+    return (np.concatenate(out) if out else np.zeros(0, np.uint32)), np.asarray(off, np.uint32)

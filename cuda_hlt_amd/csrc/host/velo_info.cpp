@@ -55,6 +55,7 @@ size_t velo_abi_struct_size(const char* name)
     VELO_ABI_ENTRY(velo_checker_category),
     VELO_ABI_ENTRY(velo_checker_result),
     VELO_ABI_ENTRY(velo_mc_truth),
+    VELO_ABI_ENTRY(velo_clustering_check_result),
     VELO_ABI_ENTRY(velo_mdf_events),
     VELO_ABI_ENTRY(velo_sequence_config),
     VELO_ABI_ENTRY(velo_sequence_input),

@@ -101,6 +101,44 @@ int velo_checker_run(uint32_t n_events, const uint32_t* event_track_offsets, con
                      const uint32_t* mc_hits, velo_checker_result* result, int32_t* track_mcp,
                      float* track_weight);
 
+/* ---- clustering checker ----
+ * Replaces checker/clustering/src/ClusteringChecker.cpp:32-135 (checkClustering)
+ * and the x86 clustering it takes as the expected clusters
+ * (x86/velo/clustering/src/Clustering.cpp:3-270, `clustering`). */
+typedef struct {
+  uint64_t reconstructible; /* expected clusters */
+  uint64_t reconstructed;   /* found clusters */
+  uint64_t correct;         /* found clusters that match an expected one */
+  uint64_t clones;          /* per distinct found cluster, its further matches among the found */
+  uint64_t ghosts;          /* found clusters that match no expected one */
+  float efficiency;         /* correct / reconstructible */
+  float clone_fraction;     /* clones / reconstructed */
+  float ghost_fraction;     /* ghosts / reconstructed */
+} velo_clustering_check_result;
+
+/* The x86 clustering of n_events raw VP events (offsets: n_events + 1 byte
+ * offsets into events): per bank, isolated SPs through the SP pattern
+ * table (one or two clusters), the pixels of the other SPs through an
+ * 8-connected row-scan flood fill seeded in SP, pixel order; each cluster's
+ * LHCb ID is its centroid pixel (integer quotients of the pixel sums).  IDs
+ * per event in bank order, event e's at [id_offsets[e], id_offsets[e+1]).
+ * Pixels outside the sensor's 768 columns are skipped (the reference
+ * writes them past its pixel buffer).  Returns 0; 1 when more than
+ * `capacity` IDs exist (*n_ids says how many, ids holds the first
+ * capacity); -1 on a null pointer or a bank whose sensor is out of range. */
+int velo_x86_clustering(const char* events, const uint32_t* offsets, uint32_t n_events, uint32_t* ids,
+                        uint32_t* id_offsets, size_t capacity, size_t* n_ids);
+
+/* The reference's clustering report: found vs expected cluster LHCb IDs of
+ * n_events events (per event at [offsets[e], offsets[e+1])).  A found
+ * cluster is correct when an expected one has its ID or, unless
+ * just_check_ids, lies on the same sensor and chip within
+ * allowed_distance_error pixels (row and column of the ID's low 16 bits);
+ * otherwise it is a ghost.  Returns 0, or -1 on a null pointer. */
+int velo_clustering_check(uint32_t n_events, const uint32_t* found_offsets, const uint32_t* found_ids,
+                          const uint32_t* expected_offsets, const uint32_t* expected_ids, int just_check_ids,
+                          float allowed_distance_error, velo_clustering_check_result* result);
+
 #ifdef __cplusplus
 }
 #endif
