@@ -80,52 +80,92 @@ __device__ inline void track_features(
 }
 
 // Logit of p(ghost) for normalised inputs x; w holds the parameters as FP32
-// (widened from the model's FP16), accumulation in FP32 in index order.
+// (widened from the model's FP16), accumulation in FP32 in index order, one
+// fused multiply-add per weight (the order and roundings of ghost_logit_model).
 __device__ __forceinline__ float ghost_logit(const float* w, const float* x, float* h1, float* h2)
 {
 #pragma unroll
   for (uint32_t j = 0; j < VELO_GHOST_HIDDEN; ++j) {
     float s = w[kGhostB1 + j];
 #pragma unroll
-    for (uint32_t i = 0; i < VELO_GHOST_FEATURES; ++i) s += w[kGhostW1 + j * VELO_GHOST_FEATURES + i] * x[i];
+    for (uint32_t i = 0; i < VELO_GHOST_FEATURES; ++i) s = fmaf(w[kGhostW1 + j * VELO_GHOST_FEATURES + i], x[i], s);
     h1[j] = s > 0.0f ? s : 0.0f;
   }
 #pragma unroll
   for (uint32_t k = 0; k < VELO_GHOST_HIDDEN; ++k) {
     float s = w[kGhostB2 + k];
 #pragma unroll
-    for (uint32_t j = 0; j < VELO_GHOST_HIDDEN; ++j) s += w[kGhostW2 + k * VELO_GHOST_HIDDEN + j] * h1[j];
+    for (uint32_t j = 0; j < VELO_GHOST_HIDDEN; ++j) s = fmaf(w[kGhostW2 + k * VELO_GHOST_HIDDEN + j], h1[j], s);
     h2[k] = s > 0.0f ? s : 0.0f;
   }
   float z = w[kGhostB3];
 #pragma unroll
-  for (uint32_t k = 0; k < VELO_GHOST_HIDDEN; ++k) z += w[kGhostW3 + k] * h2[k];
+  for (uint32_t k = 0; k < VELO_GHOST_HIDDEN; ++k) z = fmaf(w[kGhostW3 + k], h2[k], z);
   return z;
 }
 
-// The same logit without keeping the second hidden layer: each unit's
-// ReLU output is added to the logit as soon as it is formed (identical
-// operations in identical order, so identical results), which keeps the
-// inference paths free of per-lane arrays indexed at run time.
-__device__ __forceinline__ float ghost_logit_infer(const float* w, const float* x)
+// The model's FP16 parameters as 32-bit pairs in the constant address space:
+// the pointer is the same for every lane, so the compiler fetches the pairs
+// with scalar loads into SGPRs, and each multiply-add becomes one
+// v_fma_mix_f32 (the FP16 weight widened exactly, FP32 activation and
+// accumulator, one rounding) with no per-lane copy of the weights.  Valid
+// for kernels that do not write the model.
+typedef const __attribute__((address_space(4))) uint32_t* ghost_pairs_t;
+typedef const __attribute__((address_space(4))) float* ghost_floats_t;
+
+// The address passes through an empty asm statement, so a loop over tracks
+// cannot hoist the weights out of its body into hundreds of registers: each
+// evaluation streams them through SGPRs again (from the scalar cache).
+__device__ __forceinline__ ghost_pairs_t ghost_pairs(const velo_ghost_model* m)
 {
+  uint64_t addr = (uint64_t) (uintptr_t) m->weights;
+  asm volatile("" : "+s"(addr));
+  return (ghost_pairs_t) (uintptr_t) addr;
+}
+
+// Parameter p of the pair table (FP16 widened to FP32).
+__device__ __forceinline__ float ghost_param(ghost_pairs_t w, uint32_t p)
+{
+  const uint32_t pair = w[p >> 1];
+  const uint16_t bits = (p & 1u) ? (uint16_t) (pair >> 16) : (uint16_t) (pair & 0xFFFFu);
+  return __half2float(__ushort_as_half(bits));
+}
+
+// Normalised inputs (f - mean) * scale of one track's features, in place.
+__device__ __forceinline__ void ghost_normalise(const velo_ghost_model* m, float* f)
+{
+  const ghost_floats_t mean = (ghost_floats_t) (const void*) m->mean;
+  const ghost_floats_t scale = (ghost_floats_t) (const void*) m->scale;
+#pragma unroll
+  for (uint32_t k = 0; k < VELO_GHOST_FEATURES; ++k) f[k] = (f[k] - mean[k]) * scale[k];
+}
+
+// The logit of ghost_logit (same operations in the same order, so the same
+// result) evaluated from the model in device memory.  Each second-layer
+// unit's ReLU output is added to the logit as soon as it is formed, so no
+// per-lane array is indexed at run time.
+__device__ __forceinline__ float ghost_logit_model(const velo_ghost_model* m, const float* x)
+{
+  static_assert(kGhostW1 % 2 == 0 && kGhostB1 % 2 == 0 && kGhostW2 % 2 == 0 && VELO_GHOST_FEATURES % 2 == 0 &&
+                  VELO_GHOST_HIDDEN % 2 == 0,
+                "parameter blocks start on FP16 pairs");
+  const ghost_pairs_t w = ghost_pairs(m);
   float h1[VELO_GHOST_HIDDEN];
 #pragma unroll
   for (uint32_t j = 0; j < VELO_GHOST_HIDDEN; ++j) {
-    float s = w[kGhostB1 + j];
+    float s = ghost_param(w, kGhostB1 + j);
 #pragma unroll
-    for (uint32_t i = 0; i < VELO_GHOST_FEATURES; ++i) s += w[kGhostW1 + j * VELO_GHOST_FEATURES + i] * x[i];
+    for (uint32_t i = 0; i < VELO_GHOST_FEATURES; ++i) s = fmaf(ghost_param(w, kGhostW1 + j * VELO_GHOST_FEATURES + i), x[i], s);
     h1[j] = s > 0.0f ? s : 0.0f;
   }
-  float z = w[kGhostB3];
-  // one second-layer unit at a time: its 32 weights in flight, not all 1024
-#pragma unroll 1
+  float z = ghost_param(w, kGhostB3);
+#pragma unroll 2
   for (uint32_t k = 0; k < VELO_GHOST_HIDDEN; ++k) {
-    float s = w[kGhostB2 + k];
+    float s = ghost_param(w, kGhostB2 + k);
 #pragma unroll
-    for (uint32_t j = 0; j < VELO_GHOST_HIDDEN; ++j) s += w[kGhostW2 + k * VELO_GHOST_HIDDEN + j] * h1[j];
+    for (uint32_t j = 0; j < VELO_GHOST_HIDDEN; ++j) s = fmaf(ghost_param(w, kGhostW2 + k * VELO_GHOST_HIDDEN + j), h1[j], s);
     const float h2 = s > 0.0f ? s : 0.0f;
-    z += w[kGhostW3 + k] * h2;
+    z = fmaf(ghost_param(w, kGhostW3 + k), h2, z);
   }
   return z;
 }

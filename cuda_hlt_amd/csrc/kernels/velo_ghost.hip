@@ -2,8 +2,10 @@
 // training step (gradient of the weighted cross-entropy + Adam).
 //
 // The network is tiny (1441 parameters, 1376 multiply-adds per track), so
-// every kernel keeps the whole model in LDS (FP16 weights widened to FP32)
-// and gives one lane to one track; nothing here is worth a matrix-core tile.
+// every kernel gives one lane to one track; nothing here is worth a
+// matrix-core tile.  Inference reads the FP16 weights through scalar loads
+// (one mixed-precision FMA per weight); training keeps them in LDS widened
+// to FP32, the same values in the same order.
 // Training reductions are deterministic: each 64-sample tile writes its own
 // partial gradient row, and one pass sums the rows in tile order.
 #include <hip/hip_fp16.h>
@@ -63,16 +65,15 @@ __global__ void __launch_bounds__(1024) track_features_kernel(velo_track_feature
 }
 
 // ---- inference -----------------------------------------------------------------
+// One lane per track; the weights come through scalar loads (ghost_logit_model).
 __global__ void __launch_bounds__(256) ghost_infer_kernel(velo_ghost_infer_args a)
 {
-  __shared__ float w[VELO_GHOST_PARAMS];
-  load_weights(w, a.dev_model);
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.n) return;
   float x[VELO_GHOST_FEATURES];
-  for (uint32_t k = 0; k < VELO_GHOST_FEATURES; ++k)
-    x[k] = (a.dev_features[(size_t) s * VELO_GHOST_FEATURES + k] - a.dev_model->mean[k]) * a.dev_model->scale[k];
-  a.dev_score[s] = ghost_sigmoid(ghost_logit_infer(w, x));
+  for (uint32_t k = 0; k < VELO_GHOST_FEATURES; ++k) x[k] = a.dev_features[(size_t) s * VELO_GHOST_FEATURES + k];
+  ghost_normalise(a.dev_model, x);
+  a.dev_score[s] = ghost_sigmoid(ghost_logit_model(a.dev_model, x));
 }
 
 // ---- training: gradient of the weighted cross-entropy ----------------------------

@@ -162,11 +162,11 @@ struct TrackHitsOf {
   }
 };
 
-// Filter: score tracks with the ghost model whose FP32 parameters are at gw
+// Filter: score tracks with the ghost model at a.dev_ghost_model
 template<bool Filter, typename V>
 __device__ __forceinline__ void consolidate_event(
   const V& hv, const velo_consolidate_velo_tracks_args& a, uint32_t e, uint32_t n, uint32_t toff,
-  uint32_t total_tracks, uint32_t total_hits, const float* gw)
+  uint32_t total_tracks, uint32_t total_hits)
 {
   float* cx = reinterpret_cast<float*>(a.dev_velo_track_hits);
   float* cy = cx + total_hits;
@@ -267,13 +267,8 @@ __device__ __forceinline__ void consolidate_event(
       const velo_ghost_model* model = static_cast<const velo_ghost_model*>(a.dev_ghost_model);
       float f[VELO_GHOST_FEATURES];
       track_features(TrackHitsOf<V> {hv, t}, nh, x0, y0, tx, ty, zb, chi2, f);
-      for (uint32_t k = 0; k < VELO_GHOST_FEATURES; ++k) f[k] = (f[k] - model->mean[k]) * model->scale[k];
-      // an offset the compiler cannot see through: the network's LDS reads
-      // stay inside the per-track loop instead of being hoisted out of it
-      // into hundreds of registers
-      uint32_t wbase = 0;
-      asm volatile("" : "+s"(wbase));
-      const float p = ghost_sigmoid(ghost_logit_infer(gw + wbase, f));
+      ghost_normalise(model, f);
+      const float p = ghost_sigmoid(ghost_logit_model(model, f));
       accept = p < model->threshold ? 1 : 0;
       if (a.dev_ghost_score) a.dev_ghost_score[ti] = p;
     }
@@ -281,27 +276,15 @@ __device__ __forceinline__ void consolidate_event(
   }
 }
 
-// The filtering instance keeps the network's activations in registers:
-// at most kFilterBlock threads per workgroup.
-constexpr uint32_t kFilterBlock = 256;
-
-// The plain instance at most 80 VGPRs: six waves per SIMD, as many as its
-// LDS allows for 512-thread workgroups.
+// At most 80 VGPRs: six waves per SIMD, as many as the LDS allows for
+// 512-thread workgroups.  The filtering instance needs no LDS of its own:
+// the network's weights come through scalar loads.
 template<bool Filter>
-__global__ void __launch_bounds__(Filter ? kFilterBlock : 1024) __attribute__((amdgpu_waves_per_eu(Filter ? 1 : 6)))
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
 consolidate_kernel(velo_consolidate_velo_tracks_args a)
 {
   extern __shared__ uint32_t s_hits[];
   const uint32_t N = a.n_events;
-  const float* gw = nullptr;
-  if constexpr (Filter) {
-    __shared__ float s_ghost_w[VELO_GHOST_PARAMS];
-    const velo_ghost_model* model = static_cast<const velo_ghost_model*>(a.dev_ghost_model);
-    for (uint32_t i = threadIdx.x; i < VELO_GHOST_PARAMS; i += blockDim.x)
-      s_ghost_w[i] = __half2float(__ushort_as_half(model->weights[i]));
-    __syncthreads();
-    gw = s_ghost_w;
-  }
   for_each_event(a.tuning, a.n_events, [&](const uint32_t e) {
     const uint32_t total_tracks = (uint32_t) a.dev_atomics_velo[2 * N];
     const uint32_t total_hits = a.dev_velo_track_hit_number[total_tracks];
@@ -329,10 +312,10 @@ consolidate_kernel(velo_consolidate_velo_tracks_args a)
       l.y = reinterpret_cast<const float*>(s_hits + cap);
       l.z = reinterpret_cast<const float*>(s_hits + 2 * cap);
       l.id = s_hits + 3 * cap;
-      consolidate_event<Filter>(l, a, e, n, toff, total_tracks, total_hits, gw);
+      consolidate_event<Filter>(l, a, e, n, toff, total_tracks, total_hits);
     }
     else {
-      consolidate_event<Filter>(g, a, e, n, toff, total_tracks, total_hits, gw);
+      consolidate_event<Filter>(g, a, e, n, toff, total_tracks, total_hits);
     }
   });
 }
@@ -448,13 +431,12 @@ int velo_consolidate_velo_tracks_launch(const velo_consolidate_velo_tracks_args*
   if (block % kWave || block > 1024) return (int) hipErrorInvalidValue;
   velo_consolidate_velo_tracks_args args = *a;
   if (!args.tuning.lds_tile) args.tuning.lds_tile = kDefaultConsolidateHits;
-  // staged hits + the ghost model stay within 64 KB of LDS per workgroup
+  // the staged hits stay within 56 KB of LDS per workgroup
   if (args.tuning.lds_tile > kMaxConsolidateHits) return (int) hipErrorInvalidValue;
   const dim3 grid(event_grid(args.tuning, args.n_events));
   const size_t shmem = 4 * sizeof(uint32_t) * args.tuning.lds_tile;
   if (args.dev_ghost_model)
-    hipLaunchKernelGGL(consolidate_kernel<true>, grid, dim3(block < kFilterBlock ? block : kFilterBlock), shmem,
-                       (hipStream_t) stream, args);
+    hipLaunchKernelGGL(consolidate_kernel<true>, grid, dim3(block), shmem, (hipStream_t) stream, args);
   else
     hipLaunchKernelGGL(consolidate_kernel<false>, grid, dim3(block), shmem, (hipStream_t) stream, args);
   return (int) hipGetLastError();
