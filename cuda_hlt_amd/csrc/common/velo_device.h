@@ -219,14 +219,29 @@ __device__ inline uint32_t wave_sum(uint32_t v)
   return v;
 }
 
-// Inclusive prefix sum across the wave (Hillis-Steele on 64 lanes).
+// Inclusive prefix sum across the wave by DPP (no LDS): shifts by 1, 2, 4
+// and 8 lanes within each row of 16, then lane 15 of rows 0 and 2 added to
+// rows 1 and 3, then lane 31 added to rows 2 and 3.
 __device__ inline uint32_t wave_inclusive_scan(uint32_t v)
 {
-  const uint32_t l = lane_id();
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t u = __shfl_up(v, o, 64);
-    if (l >= (uint32_t) o) v += u;
-  }
+  v += (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x111, 0xf, 0xf, false); // row_shr:1
+  v += (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x112, 0xf, 0xf, false); // row_shr:2
+  v += (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x114, 0xf, 0xf, false); // row_shr:4
+  v += (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x118, 0xf, 0xf, false); // row_shr:8
+  v += (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x142, 0xa, 0xf, false); // row_bcast:15
+  v += (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x143, 0xc, 0xf, false); // row_bcast:31
+  return v;
+}
+
+// Inclusive maximum across the wave (the same DPP pattern).
+__device__ inline uint32_t wave_inclusive_max(uint32_t v)
+{
+  v = max(v, (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x111, 0xf, 0xf, false));
+  v = max(v, (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x112, 0xf, 0xf, false));
+  v = max(v, (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x114, 0xf, 0xf, false));
+  v = max(v, (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x118, 0xf, 0xf, false));
+  v = max(v, (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x142, 0xa, 0xf, false));
+  v = max(v, (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x143, 0xc, 0xf, false));
   return v;
 }
 

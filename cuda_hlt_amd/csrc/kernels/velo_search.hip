@@ -8,12 +8,15 @@
 // weak tracks) take their slots from a ballot prefix in entry order, so the
 // output is deterministic and equal to the CPU oracle's sequential order.
 //
-// State: hit_used as a bitmask in LDS (events up to 16384 estimated hits;
-// larger events use the global byte array directly); x, y, z, phi of the
-// six modules a step searches (f .. f-5 for first module f) are copied into
-// an LDS window at the start of every step when they fit, other hits are
-// read from global memory; the tracks-to-follow ring lives in dev_tracks_to_follow
-// (2000 entries of 8 bytes per event):
+// State: hit_used as a bitmask in LDS (events up to 8192 estimated hits;
+// larger events use the global byte array directly); the event's module
+// table (first hit, count, z) in registers, lane m holding module m; x, y,
+// z, phi of the six modules a step searches (f .. f-5 for first module f) in
+// an LDS window that is a ring over hit numbers when they fit, else every
+// read of the step goes to global memory (the step body is instantiated for
+// both, so window reads need no per-access test and issue together); the
+// tracks-to-follow ring lives in dev_tracks_to_follow (2000 entries of 8
+// bytes per event):
 //   word0 = h[n-2] | h[n-1] << 16
 //   word1 = (tracklet ? first hit : track number) | n << 16 | skipped << 28
 //           | tracklet << 31
@@ -88,40 +91,32 @@ struct EventHits {
     if (r >= n) v = gz[h];
     return v;
   }
-  __device__ float phi(uint32_t h) const
+  // The step's hits through the window (W: it holds the step) or global
+  // memory; one mode for a whole step, so loads need no per-access select
+  // and are free to be issued together.
+  template<bool W>
+  __device__ float vx(uint32_t h) const
   {
-    const uint32_t r = h - lo;
-    float v = launder(s_window[3 * cap + at(h, r)]);
-    if (r >= n) v = gphi[h];
-    return v;
+    if constexpr (W) return s_window[h & (cap - 1u)];
+    else return gx[h];
   }
-  // Hits of the step's six modules (seeding's h0, h1, h2 and forwarding's
-  // candidates): in the window whenever it holds the step (n > 0), so only
-  // the window's presence is tested, once per step on every lane alike.
-  __device__ uint32_t wat(uint32_t h) const { return n ? (h & (cap - 1u)) : 0u; }
-  __device__ float wx(uint32_t h) const
+  template<bool W>
+  __device__ float vy(uint32_t h) const
   {
-    float v = launder(s_window[wat(h)]);
-    if (!n) v = gx[h];
-    return v;
+    if constexpr (W) return s_window[cap + (h & (cap - 1u))];
+    else return gy[h];
   }
-  __device__ float wy(uint32_t h) const
+  template<bool W>
+  __device__ float vz(uint32_t h) const
   {
-    float v = launder(s_window[cap + wat(h)]);
-    if (!n) v = gy[h];
-    return v;
+    if constexpr (W) return s_window[2 * cap + (h & (cap - 1u))];
+    else return gz[h];
   }
-  __device__ float wz(uint32_t h) const
+  template<bool W>
+  __device__ float vphi(uint32_t h) const
   {
-    float v = launder(s_window[2 * cap + wat(h)]);
-    if (!n) v = gz[h];
-    return v;
-  }
-  __device__ float wphi(uint32_t h) const
-  {
-    float v = launder(s_window[3 * cap + wat(h)]);
-    if (!n) v = gphi[h];
-    return v;
+    if constexpr (W) return s_window[3 * cap + (h & (cap - 1u))];
+    else return gphi[h];
   }
   __device__ void copy(uint32_t from, uint32_t to)
   {
@@ -162,25 +157,32 @@ struct GlobalHits {
   __device__ float z(uint32_t h) const { return gz[h]; }
 };
 
-__device__ inline void load_modules(
-  ModuleData md[6], uint32_t first, const uint32_t* starts, const uint32_t* nums, uint32_t off, const float* zs)
+// The event's module table, lane m holding module m (m < 52): its first hit
+// (event-relative), hit count and z; read once per event.
+struct ModuleTable {
+  uint32_t start, num;
+  float z;
+};
+
+__device__ inline void load_modules(ModuleData md[6], uint32_t first, const ModuleTable& t)
 {
 #pragma unroll
   for (uint32_t i = 0; i < 6; ++i) {
     const uint32_t m = first - i;
-    md[i].start = starts[m] - off;
-    md[i].num = nums[m];
-    md[i].z = zs[m];
+    md[i].start = __builtin_amdgcn_readlane(t.start, m);
+    md[i].num = __builtin_amdgcn_readlane(t.num, m);
+    md[i].z = __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(t.z), m));
   }
 }
 
 // Best (h0, h2) for one h1 (TrackSeeding.cu:266-328); h0 then h2 ascending,
 // so equal scatters resolve to the lexicographically smallest pair.
+template<bool W>
 __device__ inline float seed_best(
   uint32_t h1, const EventHits& H, const int16_t* h0c, const int16_t* h2c, const UsedSet& used, uint32_t* b0,
   uint32_t* b2)
 {
-  const float h1x = H.wx(h1), h1y = H.wy(h1), h1z = H.wz(h1);
+  const float h1x = H.template vx<W>(h1), h1y = H.template vy<W>(h1), h1z = H.template vz<W>(h1);
   const int f0 = h0c[2 * h1], l0 = h0c[2 * h1 + 1];
   const int f2 = h2c[2 * h1], l2 = h2c[2 * h1 + 1];
   float best = kFltMax;
@@ -191,13 +193,13 @@ __device__ inline float seed_best(
   // iteration order, same result.
   for (int h0 = f0; h0 < l0; ++h0) {
     if (used.test(h0)) continue;
-    const float h0x = H.wx(h0), h0y = H.wy(h0), h0z = H.wz(h0);
+    const float h0x = H.template vx<W>(h0), h0y = H.template vy<W>(h0), h0z = H.template vz<W>(h0);
     const float dmax = kMaxSlope * (h0z - h1z);
     if (!(fabsf(h1x - h0x) < dmax && fabsf(h1y - h0y) < dmax)) continue;
     const float dz10 = h1z - h0z;
     for (int h2 = f2; h2 < l2; ++h2) {
       if (used.test(h2)) continue;
-      const float h2x = H.wx(h2), h2y = H.wy(h2), h2z = H.wz(h2);
+      const float h2x = H.template vx<W>(h2), h2y = H.template vy<W>(h2), h2z = H.template vz<W>(h2);
       const float z2_tz = (h2z - h0z) / dz10;
       const float x = h0x + (h1x - h0x) * z2_tz;
       const float y = h0y + (h1y - h0y) * z2_tz;
@@ -220,34 +222,94 @@ __device__ inline float seed_best(
   return best;
 }
 
-// phi of hit base + i through the window
-struct PhiAt {
-  const EventHits* H;
-  uint32_t base;
-  __device__ float operator()(int i) const { return H->wphi(base + (uint32_t) i); }
-};
-
-// TrackForwarding.cuh:28-58
-__device__ inline void forward_window(
-  const ModuleData& md, float tx, float ty, const EventHits& H, float h0x, float h0y, float h0z, bool odd, int* f,
-  int* l)
+// TrackForwarding.cuh:28-58 for the odd (md[4]) and the even (md[5])
+// module at once: the two binary searches of each kind run in one loop, so
+// their LDS reads are in flight together (same probes, same results as two
+// separate searches).
+template<bool W>
+__device__ inline void forward_windows(
+  const ModuleData& mo, const ModuleData& me, float tx, float ty, const EventHits& H, float h0x, float h0y,
+  float h0z, int* of, int* ol, int* ef, int* el)
 {
-  const float dz = md.z - h0z;
-  const float predx = tx * dz;
-  const float predy = ty * dz;
-  const float xp = h0x + predx;
-  const float yp = h0y + predy;
-  const float phi = odd ? phi_odd(xp, yp) : phi_even(xp, yp);
-  int first = bs_first_candidate(PhiAt {&H, md.start}, md.num, phi, kForwardPhiTolerance);
-  int last = -1;
-  if (first != -1) {
-    last = bs_second_candidate(PhiAt {&H, md.start + (uint32_t) first}, md.num - (uint32_t) first, phi,
-                               kForwardPhiTolerance);
-    first += (int) md.start;
-    last = last == 0 ? first + 1 : first + last;
+  const float dzo = mo.z - h0z;
+  const float xo = h0x + tx * dzo;
+  const float yo = h0y + ty * dzo;
+  const float dze = me.z - h0z;
+  const float xe = h0x + tx * dze;
+  const float ye = h0y + ty * dze;
+  const float po = phi_odd(xo, yo);
+  const float pe = phi_even(xe, ye);
+  const float m = kForwardPhiTolerance;
+  // bs_first_candidate on both modules
+  bool fo = false, fe = false;
+  int lo = 0, ro = (int) (mo.num - 1u), le = 0, re = (int) (me.num - 1u);
+  const float loo = po - m, loe = pe - m;
+  while (lo < ro || le < re) {
+    if (lo < ro) {
+      const int k = (lo + ro) / 2;
+      const float v = H.template vphi<W>(mo.start + (uint32_t) k);
+      const float d = po - v;
+      fo |= fabsf(d) < m;
+      if (loo > v) lo = k + 1;
+      else ro = k;
+    }
+    if (le < re) {
+      const int k = (le + re) / 2;
+      const float v = H.template vphi<W>(me.start + (uint32_t) k);
+      const float d = pe - v;
+      fe |= fabsf(d) < m;
+      if (loe > v) le = k + 1;
+      else re = k;
+    }
   }
-  *f = first;
-  *l = last;
+  {
+    const float vo = H.template vphi<W>(mo.start + (uint32_t) lo);
+    const float ve = H.template vphi<W>(me.start + (uint32_t) le);
+    const float d_o = po - vo, d_e = pe - ve;
+    fo |= fabsf(d_o) < m;
+    fe |= fabsf(d_e) < m;
+  }
+  const int fo_ = fo ? lo : -1, fe_ = fe ? le : -1;
+  // bs_second_candidate from each first candidate
+  const uint32_t bo = mo.start + (uint32_t) (fo ? lo : 0), be = me.start + (uint32_t) (fe ? le : 0);
+  int so = 0, to = fo ? (int) (mo.num - (uint32_t) lo - 1u) : 0;
+  int se = 0, te = fe ? (int) (me.num - (uint32_t) le - 1u) : 0;
+  const float hio = po + m, hie = pe + m;
+  while (so < to || se < te) {
+    if (so < to) {
+      const int k = (so + to) / 2;
+      if (hio > H.template vphi<W>(bo + (uint32_t) k)) so = k + 1;
+      else to = k;
+    }
+    if (se < te) {
+      const int k = (se + te) / 2;
+      if (hie > H.template vphi<W>(be + (uint32_t) k)) se = k + 1;
+      else te = k;
+    }
+  }
+  int lasto = -1, laste = -1;
+  if (fo) {
+    const float d = po - H.template vphi<W>(bo + (uint32_t) so);
+    const int l = fabsf(d) < m ? so + 1 : so;
+    const int first = fo_ + (int) mo.start;
+    lasto = l == 0 ? first + 1 : first + l;
+    *of = first;
+  }
+  else {
+    *of = -1;
+  }
+  if (fe) {
+    const float d = pe - H.template vphi<W>(be + (uint32_t) se);
+    const int l = fabsf(d) < m ? se + 1 : se;
+    const int first = fe_ + (int) me.start;
+    laste = l == 0 ? first + 1 : first + l;
+    *ef = first;
+  }
+  else {
+    *ef = -1;
+  }
+  *ol = lasto;
+  *el = laste;
 }
 
 // The first kTtfLds entries of a step stay in LDS (double-buffered: the
@@ -338,6 +400,8 @@ __device__ inline void push_ttf(bool want, uint32_t w0, uint32_t w1, const Ring&
 struct SeedScratch {
   uint32_t item_begin[kWave + 1];
   uint64_t best[kWave];
+  uint32_t mark[kWave]; // per item pass: (pass tag << 8) | (h1 slot + 1) at each h1's first item
+  uint32_t tag;         // item passes so far (tags of the marks; mark[] starts zeroed)
 };
 
 // h1 hits staged for a seeding batch with their h0 and h2 windows (first |
@@ -362,12 +426,13 @@ constexpr uint32_t kMaxSeedItems = 1u << 17;
 // when no h2 qualifies.  Scatters are non-negative (sign bit 0), so their
 // bit patterns order like the floats; among equal scatters the lower item
 // (smaller h0), then the first h2, wins, as in seed_best's loop order.
+template<bool W>
 __device__ inline uint64_t seed_item(
   uint32_t h1, int h0, uint32_t item, const EventHits& H, int f2, int l2, const UsedSet& used)
 {
   if (used.test(h0)) return ~0ull;
-  const float h1x = H.wx(h1), h1y = H.wy(h1), h1z = H.wz(h1);
-  const float h0x = H.wx(h0), h0y = H.wy(h0), h0z = H.wz(h0);
+  const float h1x = H.template vx<W>(h1), h1y = H.template vy<W>(h1), h1z = H.template vz<W>(h1);
+  const float h0x = H.template vx<W>(h0), h0y = H.template vy<W>(h0), h0z = H.template vz<W>(h0);
   const float dmax = kMaxSlope * (h0z - h1z);
   if (!(fabsf(h1x - h0x) < dmax && fabsf(h1y - h0y) < dmax)) return ~0ull;
   const float dz10 = h1z - h0z;
@@ -375,7 +440,7 @@ __device__ inline uint64_t seed_item(
   uint32_t bb2 = 0;
   for (int h2 = f2; h2 < l2; ++h2) {
     if (used.test(h2)) continue;
-    const float h2x = H.wx(h2), h2y = H.wy(h2), h2z = H.wz(h2);
+    const float h2x = H.template vx<W>(h2), h2y = H.template vy<W>(h2), h2z = H.template vz<W>(h2);
     const float z2_tz = (h2z - h0z) / dz10;
     const float x = h0x + (h1x - h0x) * z2_tz;
     const float y = h0y + (h1y - h0y) * z2_tz;
@@ -400,6 +465,7 @@ __device__ inline uint64_t seed_item(
 // loop); each pair's best is folded into its h1 with an LDS 64-bit minimum.
 // The result per h1 equals seed_best's, which runs instead (one h1 per
 // lane) for a batch with more pairs than the key can number.
+template<bool W>
 __device__ inline uint32_t seed_batch(
   const SeedStage& stage, uint32_t k, const EventHits& H, const int16_t* h0c, const int16_t* h2c, const UsedSet& used,
   const Ring& ring, SearchState& st, SeedScratch& sc)
@@ -414,11 +480,11 @@ __device__ inline uint32_t seed_batch(
     c = l0 > f0 ? (uint32_t) (l0 - f0) : 0u;
   }
   const uint32_t inc = wave_inclusive_scan(c);
-  const uint32_t items = __shfl(inc, kWave - 1, kWave);
+  const uint32_t items = __builtin_amdgcn_readlane(inc, kWave - 1);
   if (items >= kMaxSeedItems) {
     bool emit = false;
     uint32_t b0 = 0, b2 = 0;
-    if (lane < k) emit = seed_best(h1, H, h0c, h2c, used, &b0, &b2) < kFltMax;
+    if (lane < k) emit = seed_best<W>(h1, H, h0c, h2c, used, &b0, &b2) < kFltMax;
     const uint32_t cnt = (uint32_t) __popcll(ballot(emit));
     st.n_tracklets += cnt;
     push_ttf(emit, h1 | (b2 << 16), b0 | (3u << 16) | kTtfTrackletFlag, ring, st);
@@ -428,21 +494,26 @@ __device__ inline uint32_t seed_batch(
   if (lane == kWave - 1) sc.item_begin[kWave] = inc;
   sc.best[lane] = ~0ull;
   wave_sync();
+  // every staged h1 has at least one item (its h0 window is not empty), so
+  // the h1 of item i is the last one whose first item is at or before i:
+  // each h1 marks its first item, and a running maximum carries the marks
+  uint32_t owner = 0;
   for (uint32_t base = 0; base < items; base += kWave) {
     const uint32_t i = base + lane;
+    const uint32_t my_begin = inc - c;
+    const uint32_t tag = ++sc.tag;
+    if (lane < k && c > 0 && my_begin - base < kWave) sc.mark[my_begin - base] = (tag << 8) | (lane + 1u);
+    wave_sync();
+    const uint32_t mk = sc.mark[lane];
+    const uint32_t here = (mk >> 8) == (tag & 0xFFFFFFu) ? (mk & 0xFFu) : 0u;
+    owner = max(wave_inclusive_max(here), __builtin_amdgcn_readlane(owner, kWave - 1));
+    wave_sync(); // the marks are rewritten by the next pass
     if (i < items) {
-      // the staged h1 whose items contain i: last j < k with item_begin[j] <= i
-      uint32_t lo = 0, hi = k - 1;
-      while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1) >> 1;
-        if (sc.item_begin[mid] <= i) lo = mid;
-        else hi = mid - 1;
-      }
-      const uint32_t j = lo;
+      const uint32_t j = owner - 1u;
       const uint32_t hj = stage.h1[j];
       const int h0 = window_first(stage.c0[j]) + (int) (i - sc.item_begin[j]);
       const uint32_t c2 = stage.c2[j];
-      const uint64_t key = seed_item(hj, h0, i, H, window_first(c2), window_last(c2), used);
+      const uint64_t key = seed_item<W>(hj, h0, i, H, window_first(c2), window_last(c2), used);
       if (key != ~0ull) atomicMin(&sc.best[j], key);
     }
   }
@@ -464,9 +535,31 @@ __device__ inline uint32_t seed_batch(
 }
 
 // TrackSeeding.cu:7-363: h1 candidates of module_data[2] then [3].
+// The h0 / h2 windows of the first 64 h1 hits of each seeding module, read
+// at the start of a step so that their latency overlaps forwarding.
+struct SeedPrefetch {
+  uint32_t c0[2], c2[2];
+};
+
+__device__ inline SeedPrefetch prefetch_seed_windows(const ModuleData* md, const int16_t* h0c, const int16_t* h2c)
+{
+  const uint32_t lane = lane_id();
+  SeedPrefetch p;
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    p.c0[s] = p.c2[s] = 0u;
+    if (lane < md[2 + s].num) {
+      p.c0[s] = window_pair(h0c, md[2 + s].start + lane);
+      p.c2[s] = window_pair(h2c, md[2 + s].start + lane);
+    }
+  }
+  return p;
+}
+
+template<bool W>
 __device__ inline uint32_t seeding(
   const ModuleData* md, SeedStage& stage, const EventHits& H, const int16_t* h0c, const int16_t* h2c,
-  const UsedSet& used, const Ring& ring, SearchState& st, SeedScratch& sc)
+  const UsedSet& used, const Ring& ring, SearchState& st, SeedScratch& sc, const SeedPrefetch& pf)
 {
   const uint32_t lane = lane_id();
   uint32_t staged = 0, tracklets = 0;
@@ -479,8 +572,14 @@ __device__ inline uint32_t seeding(
       bool cand = false;
       uint32_t c0 = 0, c2 = 0;
       if (r < side_num) {
-        c0 = window_pair(h0c, h1);
-        c2 = window_pair(h2c, h1);
+        if (base == 0) {
+          c0 = pf.c0[side - 2];
+          c2 = pf.c2[side - 2];
+        }
+        else {
+          c0 = window_pair(h0c, h1);
+          c2 = window_pair(h2c, h1);
+        }
         cand = !used.test((int) h1) && window_first(c0) != -1 && window_first(c2) != -1;
       }
       const uint64_t m = ballot(cand);
@@ -493,7 +592,7 @@ __device__ inline uint32_t seeding(
       staged += (uint32_t) __popcll(m);
       wave_sync();
       if (staged >= kWave) {
-        tracklets += seed_batch(stage, kWave, H, h0c, h2c, used, ring, st, sc);
+        tracklets += seed_batch<W>(stage, kWave, H, h0c, h2c, used, ring, st, sc);
         wave_sync();
         const uint32_t rest = staged - kWave;
         uint16_t carry = 0;
@@ -514,12 +613,13 @@ __device__ inline uint32_t seeding(
       }
     }
   }
-  if (staged) tracklets += seed_batch(stage, staged, H, h0c, h2c, used, ring, st, sc);
+  if (staged) tracklets += seed_batch<W>(stage, staged, H, h0c, h2c, used, ring, st, sc);
   wave_sync();
   return tracklets;
 }
 
 // TrackForwarding.cu:41-182 over the entries [cur_begin, cur_begin + cur_n).
+template<bool W>
 __device__ inline void forwarding(
   const ModuleData* md, uint32_t cur_begin, uint32_t cur_n, const EventHits& H, const UsedSet& used, const Ring& ring,
   uint8_t* tracks, uint8_t* weak, SearchState& st)
@@ -539,23 +639,36 @@ __device__ inline void forwarding(
 
     bool found = false;
     uint32_t best_h2 = 0;
+    // the entry's last two hits are in the window for every lane (the
+    // common case): plain LDS reads
+    const bool all_in_window = ballot(active && (ha - H.lo >= H.n || hb - H.lo >= H.n)) == 0;
     if (active) {
-      const float h0x = H.x(ha), h0y = H.y(ha), h0z = H.z(ha);
-      const float h1x = H.x(hb), h1y = H.y(hb), h1z = H.z(hb);
+      float h0x, h0y, h0z, h1x, h1y, h1z;
+      if (W && all_in_window) {
+        h0x = H.template vx<true>(ha);
+        h0y = H.template vy<true>(ha);
+        h0z = H.template vz<true>(ha);
+        h1x = H.template vx<true>(hb);
+        h1y = H.template vy<true>(hb);
+        h1z = H.template vz<true>(hb);
+      }
+      else {
+        h0x = H.x(ha), h0y = H.y(ha), h0z = H.z(ha);
+        h1x = H.x(hb), h1y = H.y(hb), h1z = H.z(hb);
+      }
       const float td = 1.0f / (h1z - h0z);
       const float txn = h1x - h0x;
       const float tyn = h1y - h0y;
       const float tx = txn * td;
       const float ty = tyn * td;
       int of, ol, ef, el;
-      forward_window(md[4], tx, ty, H, h0x, h0y, h0z, true, &of, &ol);
-      forward_window(md[5], tx, ty, H, h0x, h0y, h0z, false, &ef, &el);
+      forward_windows<W>(md[4], md[5], tx, ty, H, h0x, h0y, h0z, &of, &ol, &ef, &el);
       const int n_odd = ol - of;
       const int n_all = n_odd + (el - ef);
       float best = kFltMax;
       for (int j = 0; j < n_all; ++j) {
         const int h2 = j < n_odd ? of + j : ef + j - n_odd;
-        const float h2x = H.wx(h2), h2y = H.wy(h2), h2z = H.wz(h2);
+        const float h2x = H.template vx<W>(h2), h2y = H.template vy<W>(h2), h2z = H.template vz<W>(h2);
         const float dz = h2z - h0z;
         const float predx = tx * dz;
         const float predy = ty * dz;
@@ -654,6 +767,17 @@ static uint32_t window_hits(uint32_t lds_tile)
   return p;
 }
 
+// One step whose modules do not fit the window: every hit from global memory.
+__device__ inline uint32_t step_global(
+  const ModuleData* md, uint32_t cur_begin, uint32_t cur_n, const EventHits& H, const UsedSet& used,
+  const Ring& ring, uint8_t* tracks, uint8_t* weak, SearchState& st, SeedStage& stage, const int16_t* h0c,
+  const int16_t* h2c, SeedScratch& sc, const SeedPrefetch& pf)
+{
+  forwarding<false>(md, cur_begin, cur_n, H, used, ring, tracks, weak, st);
+  wave_sync();
+  return seeding<false>(md, stage, H, h0c, h2c, used, ring, st, sc, pf);
+}
+
 __global__ void __launch_bounds__(64) search_by_triplet_kernel(velo_search_by_triplet_args a)
 {
   __shared__ uint32_t used_bits[kUsedWords];
@@ -662,6 +786,9 @@ __global__ void __launch_bounds__(64) search_by_triplet_kernel(velo_search_by_tr
   __shared__ SeedScratch seed_scratch;
 
   const uint32_t N = a.n_events;
+  seed_scratch.mark[lane_id()] = 0u;
+  if (lane_id() == 0) seed_scratch.tag = 0u;
+  wave_sync();
   for_each_event(a.tuning, a.n_events, [&](const uint32_t e) {
     const uint32_t lane = lane_id();
     const uint32_t total = a.dev_module_cluster_start[VELO_N_MODULES * N];
@@ -681,8 +808,14 @@ __global__ void __launch_bounds__(64) search_by_triplet_kernel(velo_search_by_tr
     // window of the six modules a step searches, first .. first-5
     // (contiguous in the container): seeding's h0, h1, h2 and forwarding's
     // phi windows; the tracks' previous hits are read from global memory
+    ModuleTable mt {0u, 0u, 0.0f};
+    if (lane <= VELO_N_MODULES) mt.start = starts[lane] - off; // lane 52: the event's end
+    if (lane < VELO_N_MODULES) {
+      mt.num = nums[lane];
+      mt.z = a.dev_velo_module_zs[lane];
+    }
     const auto stage_window = [&](uint32_t first) {
-      H.stage(starts[first - 5] - off, starts[first + 1] - off);
+      H.stage(__builtin_amdgcn_readlane(mt.start, first - 5), __builtin_amdgcn_readlane(mt.start, first + 1));
     };
     const int16_t* h0c = a.dev_h0_candidates + 2 * (size_t) off;
     const int16_t* h2c = a.dev_h2_candidates + 2 * (size_t) off;
@@ -701,29 +834,36 @@ __global__ void __launch_bounds__(64) search_by_triplet_kernel(velo_search_by_tr
 
     SearchState st {0, 0, 0, 0, 0, 0, VELO_TTF_MODULO};
     ModuleData md[6];
-    uint32_t first = VELO_N_MODULES - 1;
-    load_modules(md, first, starts, nums, off, a.dev_velo_module_zs);
-    stage_window(first);
-    uint32_t prev_tracklets = seeding(md, stage, H, h0c, h2c, used, ring, st, seed_scratch);
-    ring.swap();
-    uint32_t cur_begin = 0, cur_n = st.next_n;
-    first -= 2;
-    while (first >= 4) {
-      load_modules(md, first, starts, nums, off, a.dev_velo_module_zs);
+    // steps with first module 51, 49, ..., 5; the first has nothing to
+    // forward (ProcessModules.cu:22-59 seeds it, then the pairs loop)
+    uint32_t prev_tracklets = 0, cur_begin = 0, cur_n = 0;
+    for (uint32_t first = VELO_N_MODULES - 1;; first -= 2) {
+      load_modules(md, first, mt);
+      const SeedPrefetch pf = prefetch_seed_windows(md, h0c, h2c);
       stage_window(first);
       st.next_begin = (cur_begin + cur_n) % VELO_TTF_MODULO;
       st.next_n = 0;
       st.cap = VELO_TTF_MODULO - cur_n;
-      forwarding(md, cur_begin, cur_n, H, used, ring, tracks, weak, st);
-      wave_sync(); // hit_used marks of forwarding are read by seeding
-      const uint32_t tl = seeding(md, stage, H, h0c, h2c, used, ring, st, seed_scratch);
+      uint32_t tl;
+      if (H.n) {
+        // an empty forwarding module still has its first slot probed
+        // (VeloTools.cuh:6-26 inspects array[0] for size 0), which may lie
+        // past the window: that step forwards from global memory
+        if (md[4].num && md[5].num) forwarding<true>(md, cur_begin, cur_n, H, used, ring, tracks, weak, st);
+        else forwarding<false>(md, cur_begin, cur_n, H, used, ring, tracks, weak, st);
+        wave_sync(); // hit_used marks of forwarding are read by seeding
+        tl = seeding<true>(md, stage, H, h0c, h2c, used, ring, st, seed_scratch, pf);
+      }
+      else {
+        tl = step_global(md, cur_begin, cur_n, H, used, ring, tracks, weak, st, stage, h0c, h2c, seed_scratch, pf);
+      }
       if (tl + prev_tracklets > VELO_TTF_MODULO) st.status |= VELO_STATUS_TRACKLETS_OVERFLOW;
       prev_tracklets = tl;
       cur_begin = st.next_begin;
       cur_n = st.next_n;
       ring.swap();
-      first -= 2;
       wave_sync();
+      if (first < 7) break;
     }
 
     // ProcessModules.cu:141-162: tracklets still followed become weak tracks
