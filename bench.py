@@ -244,6 +244,22 @@ def batched_run(args, geometry, zs):
     return out
 
 
+def serving_run(args):
+    """The continuous-batching server (batched graphs with the ghost filter,
+    results kept in device memory): peak events/s with every request queued
+    at once, then Poisson arrivals at 20/50/90 % of the peak, with request
+    and batch latencies."""
+    from cuda_hlt_amd import serving
+    from cuda_hlt_amd.ghost import GhostModel
+    model = GhostModel.load(args.ghost_model or str(GHOST_MODEL))
+    t0 = time.perf_counter()
+    out = serving.run_loads((0.2, 0.5, 0.9), n_requests=args.serving_requests, seed=5, batched=True,
+                            ghost_model=model, log=log)
+    out["requests_per_run"] = args.serving_requests
+    out["wall_s"] = round(time.perf_counter() - t0, 1)
+    return out
+
+
 def extended_run(args, ev, seq, raw, batch_offsets, B, dev):
     """The sequence's opt-in steps on the first batch of the same workload:
     the global event cut (UT and SciFi banks from gen.gec_banks),
@@ -418,6 +434,9 @@ def main():
                     help="events of timed batch 0 compared bit for bit with the oracle")
     ap.add_argument("--pcie-steps", type=int, default=5,
                     help="launches of the host-input run (input copied in, outputs copied back); 0 skips it")
+    ap.add_argument("--serving-requests", type=int, default=60,
+                    help="requests per serving run (peak, then 20/50/90 %% of it; batched + filtered, "
+                         "results in device memory); 0 skips serving")
     ap.add_argument("--no-extended", action="store_true",
                     help="skip the run with the global event cut, velo_kalman_fit and beamline PV finding")
     ap.add_argument("--traffic", default=str(ROOT / "profiles" / "traffic_minbias_10k.json"),
@@ -555,6 +574,11 @@ def main():
     result["pcie_inclusive"] = pcie_run(args, ev, geometry, zs, B, tuning)
     if args.batched_events:
         result["batched"] = batched_run(args, geometry, zs)
+    if args.serving_requests:
+        try:
+            result["serving"] = serving_run(args)
+        except Exception as e:  # the headline line is printed regardless
+            result["serving"] = {"error": repr(e)}
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(ev.subset(0, B), geometry, zs)
     print(json.dumps(result), flush=True)

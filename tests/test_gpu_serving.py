@@ -10,12 +10,18 @@ pytestmark = pytest.mark.gpu
 
 
 def same(a, b):
+    a, b = host(a), host(b)
     np.testing.assert_array_equal(a["tracks_per_event"], b["tracks_per_event"])
     np.testing.assert_array_equal(a["hit_number"], b["hit_number"])
     np.testing.assert_array_equal(a["hits"], b["hits"])
     np.testing.assert_array_equal(a["states"], b["states"])
     if a["accepted"] is not None or b["accepted"] is not None:
         np.testing.assert_array_equal(a["accepted"], b["accepted"])
+
+
+def host(result):
+    """A request's result on the host (device results are copied back)."""
+    return result if isinstance(result, dict) else result.host()
 
 
 @pytest.fixture(scope="module")
@@ -25,18 +31,18 @@ def requests(gpu):
     return reqs
 
 
-@pytest.mark.parametrize("batched", [True, False])
-def test_results_same_alone_and_in_the_mix(gpu, geometry, module_zs, requests, batched):
+@pytest.mark.parametrize("batched,host_results", [(True, False), (False, False), (True, True)])
+def test_results_same_alone_and_in_the_mix(gpu, geometry, module_zs, requests, batched, host_results):
     from cuda_hlt_amd import serving
     srv = serving.Server(geometry, module_zs, hbm_budget_mb=16384, max_slots=2, slot_events=1024,
-                         slot_bytes=1024 * 40000, batched=batched)
+                         slot_bytes=1024 * 40000, batched=batched, host_results=host_results)
     try:
-        reqs = [serving.Request(r.id, r.events, r.offsets, 0.0) for r in requests]
+        reqs = [serving.Request(r.id, r.events, r.offsets, 0.0, pinned=r.pinned) for r in requests]
         srv.serve(reqs)  # all queued at once: requests share batches
         assert any(len(b.requests) > 1 for b in srv.batches)
         assert sorted(i for b in srv.batches for i in b.requests) == [r.id for r in reqs]
         for r in reqs:
-            alone = serving.Request(r.id, r.events, r.offsets, 0.0)
+            alone = serving.Request(r.id, r.events, r.offsets, 0.0)  # not pinned: staged by the slot
             srv.serve([alone])
             same(alone.result, r.result)
     finally:
@@ -48,17 +54,20 @@ def test_results_match_the_oracle(gpu, geometry, module_zs, requests):
     srv = serving.Server(geometry, module_zs, hbm_budget_mb=16384, max_slots=2, slot_events=1024,
                          slot_bytes=1024 * 40000)
     try:
-        reqs = [serving.Request(r.id, r.events, r.offsets, 0.0) for r in requests[:4]]
+        reqs = [serving.Request(r.id, r.events, r.offsets, 0.0, pinned=r.pinned) for r in requests[:4]]
         srv.serve(reqs)
         for r in reqs:
             ref = po.pipeline(r.events, r.offsets, geometry, module_zs)
             n = r.n
-            np.testing.assert_array_equal(r.result["tracks_per_event"], ref.atomics[:n])
+            res = host(r.result)
+            np.testing.assert_array_equal(res["tracks_per_event"], ref.atomics[:n])
             T = int(ref.atomics[2 * n])
             Ht = int(ref.hit_number[T])
-            np.testing.assert_array_equal(r.result["hit_number"], ref.hit_number[: T + 1])
-            np.testing.assert_array_equal(r.result["hits"], ref.track_hits[: 16 * Ht])
-            np.testing.assert_array_equal(r.result["states"], ref.states[: 49 * T])
+            assert r.result.n_tracks == T
+            np.testing.assert_array_equal(res["hit_number"], ref.hit_number[: T + 1])
+            np.testing.assert_array_equal(res["hits"], ref.track_hits[: 16 * Ht])
+            np.testing.assert_array_equal(res["states"], ref.states[: 49 * T])
+            assert res["accepted"].all()
     finally:
         srv.close()
 
