@@ -213,11 +213,11 @@ __device__ inline uint32_t wave_prefix_small(uint32_t v, uint32_t* total)
   return count_below(b0) + 2u * count_below(b1);
 }
 
-__device__ inline uint32_t wave_sum(uint32_t v)
-{
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
+__device__ inline uint32_t wave_inclusive_scan(uint32_t v);
+
+// Sum over the wave (the same value on every lane): the last lane of the
+// DPP scan.
+__device__ inline uint32_t wave_sum(uint32_t v) { return __builtin_amdgcn_readlane(wave_inclusive_scan(v), 63); }
 
 // Inclusive prefix sum across the wave by DPP (no LDS): shifts by 1, 2, 4
 // and 8 lanes within each row of 16, then lane 15 of rows 0 and 2 added to
