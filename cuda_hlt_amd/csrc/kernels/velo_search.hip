@@ -12,9 +12,11 @@
 // larger events use the global byte array directly); the event's module
 // table (first hit, count, z) in registers, lane m holding module m; x, y,
 // z, phi of the six modules a step searches (f .. f-5 for first module f) in
-// an LDS window that is a ring over hit numbers when they fit, else every
-// read of the step goes to global memory (the step body is instantiated for
-// both, so window reads need no per-access test and issue together); the
+// an LDS window that is a ring over hit numbers when they fit, else the
+// four modules that hold h1, h2 and the forwarding candidates in the window
+// and the two h0 modules in global memory, else every read of the step in
+// global memory (the step body is instantiated for each mode, so window
+// reads need no per-access test and issue together); the
 // tracks-to-follow ring lives in dev_tracks_to_follow (2000 entries of 8
 // bytes per event):
 //   word0 = h[n-2] | h[n-1] << 16
@@ -48,6 +50,12 @@ struct ModuleData {
   uint32_t start, num;
   float z;
 };
+
+// Window modes of a step: its six modules in global memory, all six in the
+// LDS window, or the four that hold h1, h2 and the forwarding candidates
+// (md[2] .. md[5]) in the window and the h0 modules (md[0], md[1]) in
+// global memory, for steps whose six modules do not fit.
+constexpr int kWindowNone = 0, kWindowAll = 1, kWindowNoH0 = 2;
 
 // Dynamic LDS of search_by_triplet: the hit window, four arrays of `cap`.
 extern __shared__ float s_window[];
@@ -91,31 +99,31 @@ struct EventHits {
     if (r >= n) v = gz[h];
     return v;
   }
-  // The step's hits through the window (W: it holds the step) or global
-  // memory; one mode for a whole step, so loads need no per-access select
-  // and are free to be issued together.
-  template<bool W>
+  // The step's hits through the window (W > 0: it holds them) or global
+  // memory (W = 0); one mode for a whole step, so loads need no per-access
+  // select and are free to be issued together.
+  template<int W>
   __device__ float vx(uint32_t h) const
   {
-    if constexpr (W) return s_window[h & (cap - 1u)];
+    if constexpr (W != 0) return s_window[h & (cap - 1u)];
     else return gx[h];
   }
-  template<bool W>
+  template<int W>
   __device__ float vy(uint32_t h) const
   {
-    if constexpr (W) return s_window[cap + (h & (cap - 1u))];
+    if constexpr (W != 0) return s_window[cap + (h & (cap - 1u))];
     else return gy[h];
   }
-  template<bool W>
+  template<int W>
   __device__ float vz(uint32_t h) const
   {
-    if constexpr (W) return s_window[2 * cap + (h & (cap - 1u))];
+    if constexpr (W != 0) return s_window[2 * cap + (h & (cap - 1u))];
     else return gz[h];
   }
-  template<bool W>
+  template<int W>
   __device__ float vphi(uint32_t h) const
   {
-    if constexpr (W) return s_window[3 * cap + (h & (cap - 1u))];
+    if constexpr (W != 0) return s_window[3 * cap + (h & (cap - 1u))];
     else return gphi[h];
   }
   __device__ void copy(uint32_t from, uint32_t to)
@@ -177,11 +185,12 @@ __device__ inline void load_modules(ModuleData md[6], uint32_t first, const Modu
 
 // Best (h0, h2) for one h1 (TrackSeeding.cu:266-328); h0 then h2 ascending,
 // so equal scatters resolve to the lexicographically smallest pair.
-template<bool W>
+template<int W>
 __device__ inline float seed_best(
   uint32_t h1, const EventHits& H, const int16_t* h0c, const int16_t* h2c, const UsedSet& used, uint32_t* b0,
   uint32_t* b2)
 {
+  constexpr int W0 = W == kWindowAll ? kWindowAll : kWindowNone; // h0 modules: in the window only in mode all
   const float h1x = H.template vx<W>(h1), h1y = H.template vy<W>(h1), h1z = H.template vz<W>(h1);
   const int f0 = h0c[2 * h1], l0 = h0c[2 * h1 + 1];
   const int f2 = h2c[2 * h1], l2 = h2c[2 * h1 + 1];
@@ -193,7 +202,7 @@ __device__ inline float seed_best(
   // iteration order, same result.
   for (int h0 = f0; h0 < l0; ++h0) {
     if (used.test(h0)) continue;
-    const float h0x = H.template vx<W>(h0), h0y = H.template vy<W>(h0), h0z = H.template vz<W>(h0);
+    const float h0x = H.template vx<W0>(h0), h0y = H.template vy<W0>(h0), h0z = H.template vz<W0>(h0);
     const float dmax = kMaxSlope * (h0z - h1z);
     if (!(fabsf(h1x - h0x) < dmax && fabsf(h1y - h0y) < dmax)) continue;
     const float dz10 = h1z - h0z;
@@ -226,7 +235,7 @@ __device__ inline float seed_best(
 // module at once: the two binary searches of each kind run in one loop, so
 // their LDS reads are in flight together (same probes, same results as two
 // separate searches).
-template<bool W>
+template<int W>
 __device__ inline void forward_windows(
   const ModuleData& mo, const ModuleData& me, float tx, float ty, const EventHits& H, float h0x, float h0y,
   float h0z, int* of, int* ol, int* ef, int* el)
@@ -426,13 +435,14 @@ constexpr uint32_t kMaxSeedItems = 1u << 17;
 // when no h2 qualifies.  Scatters are non-negative (sign bit 0), so their
 // bit patterns order like the floats; among equal scatters the lower item
 // (smaller h0), then the first h2, wins, as in seed_best's loop order.
-template<bool W>
+template<int W>
 __device__ inline uint64_t seed_item(
   uint32_t h1, int h0, uint32_t item, const EventHits& H, int f2, int l2, const UsedSet& used)
 {
+  constexpr int W0 = W == kWindowAll ? kWindowAll : kWindowNone; // h0 modules: in the window only in mode all
   if (used.test(h0)) return ~0ull;
   const float h1x = H.template vx<W>(h1), h1y = H.template vy<W>(h1), h1z = H.template vz<W>(h1);
-  const float h0x = H.template vx<W>(h0), h0y = H.template vy<W>(h0), h0z = H.template vz<W>(h0);
+  const float h0x = H.template vx<W0>(h0), h0y = H.template vy<W0>(h0), h0z = H.template vz<W0>(h0);
   const float dmax = kMaxSlope * (h0z - h1z);
   if (!(fabsf(h1x - h0x) < dmax && fabsf(h1y - h0y) < dmax)) return ~0ull;
   const float dz10 = h1z - h0z;
@@ -465,7 +475,7 @@ __device__ inline uint64_t seed_item(
 // loop); each pair's best is folded into its h1 with an LDS 64-bit minimum.
 // The result per h1 equals seed_best's, which runs instead (one h1 per
 // lane) for a batch with more pairs than the key can number.
-template<bool W>
+template<int W>
 __device__ inline uint32_t seed_batch(
   const SeedStage& stage, uint32_t k, const EventHits& H, const int16_t* h0c, const int16_t* h2c, const UsedSet& used,
   const Ring& ring, SearchState& st, SeedScratch& sc)
@@ -556,7 +566,7 @@ __device__ inline SeedPrefetch prefetch_seed_windows(const ModuleData* md, const
   return p;
 }
 
-template<bool W>
+template<int W>
 __device__ inline uint32_t seeding(
   const ModuleData* md, SeedStage& stage, const EventHits& H, const int16_t* h0c, const int16_t* h2c,
   const UsedSet& used, const Ring& ring, SearchState& st, SeedScratch& sc, const SeedPrefetch& pf)
@@ -619,7 +629,7 @@ __device__ inline uint32_t seeding(
 }
 
 // TrackForwarding.cu:41-182 over the entries [cur_begin, cur_begin + cur_n).
-template<bool W>
+template<int W>
 __device__ inline void forwarding(
   const ModuleData* md, uint32_t cur_begin, uint32_t cur_n, const EventHits& H, const UsedSet& used, const Ring& ring,
   uint8_t* tracks, uint8_t* weak, SearchState& st)
@@ -644,13 +654,13 @@ __device__ inline void forwarding(
     const bool all_in_window = ballot(active && (ha - H.lo >= H.n || hb - H.lo >= H.n)) == 0;
     if (active) {
       float h0x, h0y, h0z, h1x, h1y, h1z;
-      if (W && all_in_window) {
-        h0x = H.template vx<true>(ha);
-        h0y = H.template vy<true>(ha);
-        h0z = H.template vz<true>(ha);
-        h1x = H.template vx<true>(hb);
-        h1y = H.template vy<true>(hb);
-        h1z = H.template vz<true>(hb);
+      if (W != kWindowNone && all_in_window) {
+        h0x = H.template vx<kWindowAll>(ha);
+        h0y = H.template vy<kWindowAll>(ha);
+        h0z = H.template vz<kWindowAll>(ha);
+        h1x = H.template vx<kWindowAll>(hb);
+        h1y = H.template vy<kWindowAll>(hb);
+        h1z = H.template vz<kWindowAll>(hb);
       }
       else {
         h0x = H.x(ha), h0y = H.y(ha), h0z = H.z(ha);
@@ -773,9 +783,9 @@ __device__ inline uint32_t step_global(
   const Ring& ring, uint8_t* tracks, uint8_t* weak, SearchState& st, SeedStage& stage, const int16_t* h0c,
   const int16_t* h2c, SeedScratch& sc, const SeedPrefetch& pf)
 {
-  forwarding<false>(md, cur_begin, cur_n, H, used, ring, tracks, weak, st);
+  forwarding<kWindowNone>(md, cur_begin, cur_n, H, used, ring, tracks, weak, st);
   wave_sync();
-  return seeding<false>(md, stage, H, h0c, h2c, used, ring, st, sc, pf);
+  return seeding<kWindowNone>(md, stage, H, h0c, h2c, used, ring, st, sc, pf);
 }
 
 __global__ void __launch_bounds__(64) search_by_triplet_kernel(velo_search_by_triplet_args a)
@@ -814,8 +824,15 @@ __global__ void __launch_bounds__(64) search_by_triplet_kernel(velo_search_by_tr
       mt.num = nums[lane];
       mt.z = a.dev_velo_module_zs[lane];
     }
-    const auto stage_window = [&](uint32_t first) {
-      H.stage(__builtin_amdgcn_readlane(mt.start, first - 5), __builtin_amdgcn_readlane(mt.start, first + 1));
+    // the step's window: all six modules when they fit, else the four from
+    // md[5] to md[2] (contiguous: first - 5 .. first - 2), else none
+    const auto stage_window = [&](uint32_t first) -> int {
+      const uint32_t lo = __builtin_amdgcn_readlane(mt.start, first - 5);
+      const uint32_t hi_all = __builtin_amdgcn_readlane(mt.start, first + 1);
+      const uint32_t hi_part = __builtin_amdgcn_readlane(mt.start, first - 1);
+      const bool all = hi_all - lo <= H.cap;
+      H.stage(lo, all ? hi_all : hi_part);
+      return all ? kWindowAll : (H.n ? kWindowNoH0 : kWindowNone);
     };
     const int16_t* h0c = a.dev_h0_candidates + 2 * (size_t) off;
     const int16_t* h2c = a.dev_h2_candidates + 2 * (size_t) off;
@@ -840,19 +857,25 @@ __global__ void __launch_bounds__(64) search_by_triplet_kernel(velo_search_by_tr
     for (uint32_t first = VELO_N_MODULES - 1;; first -= 2) {
       load_modules(md, first, mt);
       const SeedPrefetch pf = prefetch_seed_windows(md, h0c, h2c);
-      stage_window(first);
+      const int mode = stage_window(first);
       st.next_begin = (cur_begin + cur_n) % VELO_TTF_MODULO;
       st.next_n = 0;
       st.cap = VELO_TTF_MODULO - cur_n;
       uint32_t tl;
-      if (H.n) {
+      if (mode == kWindowAll) {
         // an empty forwarding module still has its first slot probed
         // (VeloTools.cuh:6-26 inspects array[0] for size 0), which may lie
         // past the window: that step forwards from global memory
-        if (md[4].num && md[5].num) forwarding<true>(md, cur_begin, cur_n, H, used, ring, tracks, weak, st);
-        else forwarding<false>(md, cur_begin, cur_n, H, used, ring, tracks, weak, st);
+        if (md[4].num && md[5].num) forwarding<kWindowAll>(md, cur_begin, cur_n, H, used, ring, tracks, weak, st);
+        else forwarding<kWindowNone>(md, cur_begin, cur_n, H, used, ring, tracks, weak, st);
         wave_sync(); // hit_used marks of forwarding are read by seeding
-        tl = seeding<true>(md, stage, H, h0c, h2c, used, ring, st, seed_scratch, pf);
+        tl = seeding<kWindowAll>(md, stage, H, h0c, h2c, used, ring, st, seed_scratch, pf);
+      }
+      else if (mode == kWindowNoH0) {
+        if (md[4].num && md[5].num) forwarding<kWindowNoH0>(md, cur_begin, cur_n, H, used, ring, tracks, weak, st);
+        else forwarding<kWindowNone>(md, cur_begin, cur_n, H, used, ring, tracks, weak, st);
+        wave_sync();
+        tl = seeding<kWindowNoH0>(md, stage, H, h0c, h2c, used, ring, st, seed_scratch, pf);
       }
       else {
         tl = step_global(md, cur_begin, cur_n, H, used, ring, tracks, weak, st, stage, h0c, h2c, seed_scratch, pf);

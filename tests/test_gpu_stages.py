@@ -322,10 +322,29 @@ def test_search_by_triplet_uneven(stages, consts, chain_uneven):
     _check_search(stages, consts, chain_uneven)
 
 
+def window_modes(starts, n, cap):
+    """Per step of every event: 1 when its six modules fit a window of cap
+    hits, 2 when only the four h1 / h2 modules do, 0 when neither (the
+    kernel's three step modes)."""
+    st = np.asarray(starts[: n * N_MODULES + 1], np.int64)
+    modes = []
+    for e in range(n):
+        s = np.append(st[e * N_MODULES: (e + 1) * N_MODULES], st[(e + 1) * N_MODULES])
+        for first in range(N_MODULES - 1, 4, -2):
+            lo = s[first - 5]
+            modes.append(1 if s[first + 1] - lo <= cap else (2 if s[first - 1] - lo <= cap else 0))
+    return np.array(modes)
+
+
 @pytest.mark.parametrize("tile", [64, 200, 256])
 def test_search_by_triplet_small_windows(stages, consts, chain_uneven, tile):
     """Windows too small for most steps (hits read from global memory), a
-    tile that rounds down to 128, and one that fits part of the steps."""
+    tile that rounds down to 128, and one that fits part of the steps: each
+    of the three step modes occurs."""
+    cap = 1 << (tile.bit_length() - 1)
+    modes = window_modes(chain_uneven.starts, chain_uneven.n, cap)
+    if tile == 200:
+        assert {0, 1, 2} <= set(modes.tolist()), "every step mode occurs"
     _check_search(stages, consts, chain_uneven, (64, tile))
 
 
