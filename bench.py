@@ -434,7 +434,7 @@ def main():
                     help="events of timed batch 0 compared bit for bit with the oracle")
     ap.add_argument("--pcie-steps", type=int, default=5,
                     help="launches of the host-input run (input copied in, outputs copied back); 0 skips it")
-    ap.add_argument("--serving-requests", type=int, default=60,
+    ap.add_argument("--serving-requests", type=int, default=150,
                     help="requests per serving run (peak, then 20/50/90 %% of it; batched + filtered, "
                          "results in device memory); 0 skips serving")
     ap.add_argument("--no-extended", action="store_true",

@@ -232,9 +232,14 @@ __device__ inline float seed_best(
 }
 
 // TrackForwarding.cuh:28-58 for the odd (md[4]) and the even (md[5])
-// module at once: the two binary searches of each kind run in one loop, so
-// their LDS reads are in flight together (same probes, same results as two
-// separate searches).
+// module at once.  bs_first_candidate keeps its probe sequence (its "found"
+// flag depends on the elements it probes).  bs_second_candidate searches
+// the module from the first candidate on for the first phi not below
+// phi + tolerance (else the last slot): in a module sorted by phi that
+// position is the same whether the search starts at the first candidate or
+// at the module's first hit, clamped to the first candidate, so it runs on
+// the whole module alongside the first search.  The four searches share one
+// loop, so their LDS reads are in flight together.
 template<int W>
 __device__ inline void forward_windows(
   const ModuleData& mo, const ModuleData& me, float tx, float ty, const EventHits& H, float h0x, float h0y,
@@ -249,11 +254,13 @@ __device__ inline void forward_windows(
   const float po = phi_odd(xo, yo);
   const float pe = phi_even(xe, ye);
   const float m = kForwardPhiTolerance;
-  // bs_first_candidate on both modules
   bool fo = false, fe = false;
+  // first candidate (l, r) and the second search (u, v) on each module
   int lo = 0, ro = (int) (mo.num - 1u), le = 0, re = (int) (me.num - 1u);
+  int uo = 0, vo = ro, ue = 0, ve = re;
   const float loo = po - m, loe = pe - m;
-  while (lo < ro || le < re) {
+  const float hio = po + m, hie = pe + m;
+  while (lo < ro || le < re || uo < vo || ue < ve) {
     if (lo < ro) {
       const int k = (lo + ro) / 2;
       const float v = H.template vphi<W>(mo.start + (uint32_t) k);
@@ -270,37 +277,33 @@ __device__ inline void forward_windows(
       if (loe > v) le = k + 1;
       else re = k;
     }
+    if (uo < vo) {
+      const int k = (uo + vo) / 2;
+      if (hio > H.template vphi<W>(mo.start + (uint32_t) k)) uo = k + 1;
+      else vo = k;
+    }
+    if (ue < ve) {
+      const int k = (ue + ve) / 2;
+      if (hie > H.template vphi<W>(me.start + (uint32_t) k)) ue = k + 1;
+      else ve = k;
+    }
   }
+  // the second search from the first candidate on
+  const int so = uo > lo ? uo : lo, se = ue > le ? ue : le;
+  const float vo_first = H.template vphi<W>(mo.start + (uint32_t) lo);
+  const float ve_first = H.template vphi<W>(me.start + (uint32_t) le);
+  const float vo_second = H.template vphi<W>(mo.start + (uint32_t) so);
+  const float ve_second = H.template vphi<W>(me.start + (uint32_t) se);
   {
-    const float vo = H.template vphi<W>(mo.start + (uint32_t) lo);
-    const float ve = H.template vphi<W>(me.start + (uint32_t) le);
-    const float d_o = po - vo, d_e = pe - ve;
+    const float d_o = po - vo_first, d_e = pe - ve_first;
     fo |= fabsf(d_o) < m;
     fe |= fabsf(d_e) < m;
   }
-  const int fo_ = fo ? lo : -1, fe_ = fe ? le : -1;
-  // bs_second_candidate from each first candidate
-  const uint32_t bo = mo.start + (uint32_t) (fo ? lo : 0), be = me.start + (uint32_t) (fe ? le : 0);
-  int so = 0, to = fo ? (int) (mo.num - (uint32_t) lo - 1u) : 0;
-  int se = 0, te = fe ? (int) (me.num - (uint32_t) le - 1u) : 0;
-  const float hio = po + m, hie = pe + m;
-  while (so < to || se < te) {
-    if (so < to) {
-      const int k = (so + to) / 2;
-      if (hio > H.template vphi<W>(bo + (uint32_t) k)) so = k + 1;
-      else to = k;
-    }
-    if (se < te) {
-      const int k = (se + te) / 2;
-      if (hie > H.template vphi<W>(be + (uint32_t) k)) se = k + 1;
-      else te = k;
-    }
-  }
   int lasto = -1, laste = -1;
   if (fo) {
-    const float d = po - H.template vphi<W>(bo + (uint32_t) so);
-    const int l = fabsf(d) < m ? so + 1 : so;
-    const int first = fo_ + (int) mo.start;
+    const float d = po - vo_second;
+    const int l = fabsf(d) < m ? so - lo + 1 : so - lo;
+    const int first = lo + (int) mo.start;
     lasto = l == 0 ? first + 1 : first + l;
     *of = first;
   }
@@ -308,9 +311,9 @@ __device__ inline void forward_windows(
     *of = -1;
   }
   if (fe) {
-    const float d = pe - H.template vphi<W>(be + (uint32_t) se);
-    const int l = fabsf(d) < m ? se + 1 : se;
-    const int first = fe_ + (int) me.start;
+    const float d = pe - ve_second;
+    const int l = fabsf(d) < m ? se - le + 1 : se - le;
+    const int first = le + (int) me.start;
     laste = l == 0 ? first + 1 : first + l;
     *ef = first;
   }

@@ -493,6 +493,7 @@ def main():
     ap.add_argument("--loads", type=float, nargs="*", default=[0.2, 0.5, 0.9])
     ap.add_argument("--plain", action="store_true", help="plain path (host reads between steps, no filter)")
     ap.add_argument("--ghost-model", default=None, help="trained filter (.npz) for the batched path")
+    ap.add_argument("--max-slots", type=int, default=6)
     ap.add_argument("--host-results", action="store_true",
                     help="copy every batch's output to the host and cut it there")
     ap.add_argument("--out", default=None)
@@ -503,7 +504,7 @@ def main():
         model = GhostModel.load(a.ghost_model)
     say = lambda *x: print(*x, file=sys.stderr, flush=True)  # noqa: E731
     res = run_loads(a.loads, a.requests, batched=not a.plain, ghost_model=model, log=say,
-                    host_results=a.host_results)
+                    host_results=a.host_results, max_slots=a.max_slots)
     text = json.dumps(res)
     if a.out:
         with open(a.out, "w") as f:
