@@ -227,8 +227,9 @@ void SequenceVisitor::visit<velo_estimate_input_size_t>(
   hipStream_t& stream,
   hipEvent_t&)
 {
-  // 512 threads: eight waves share the event's staged words and bank table
-  state.set_opts(dim3(host_buffers.host_number_of_selected_events[0]), dim3(512), stream);
+  // 256 threads: four waves share the event's staged words and flat bank
+  // table; small workgroups keep more events in flight per CU
+  state.set_opts(dim3(host_buffers.host_number_of_selected_events[0]), dim3(256), stream);
   state.set_arguments(
     arguments.offset<dev_velo_raw_input>(),
     arguments.offset<dev_velo_raw_input_offsets>(),

@@ -2,14 +2,16 @@
 // clustering for gfx950, plus the device constants.
 //
 // Work layout: one workgroup per event.  The workgroup first copies the raw
-// event into LDS (coalesced); wave w then owns modules w, w+W, w+2W, ...
-// (W = waves per workgroup) and takes the SPs of a module's four banks as
-// one list, one SP per lane.  estimate_input_size resolves each SP's
-// neighbourhood by scanning its bank in LDS and lists the candidates;
-// masked clustering reads that list (or scans the banks itself when it has
-// none) and grows one candidate per lane.  Output slots come from a
-// ballot-based wave prefix sum, so clusters are written in bank, SP, cluster
-// order with no atomics and no cross-wave synchronisation.
+// event into LDS (coalesced).  estimate_input_size then takes the event's
+// SPs as one flat list in bank order, 64 per wave step, each wave a
+// contiguous run of them; it resolves each SP's neighbourhood in its bank
+// in LDS, sums the estimates per module and lists the candidates in bank,
+// SP, nibble order.  Masked clustering gives wave w modules w, w+W, w+2W,
+// ... (W = waves per workgroup), takes the SPs of a module's four banks as
+// one list, one SP per lane, reads the candidate list (or scans the banks
+// itself when it has none) and grows one candidate per lane.  Output slots
+// come from ballot-based wave prefix sums, so clusters are written in bank,
+// SP, cluster order with no atomics and no cross-wave synchronisation.
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -349,8 +351,7 @@ __global__ void __launch_bounds__(kOrderBlock)
 // ---------------------------------------------------------------------------
 // The raw event is copied word by word into LDS by the whole workgroup when
 // it fits (tuning.lds_tile words); otherwise the same code reads it from
-// global memory.  Each wave owns whole modules; the SPs of a module's four
-// banks are taken as one list in bank order, one SP per lane.
+// global memory.
 constexpr uint32_t kDefaultEventWords = 6144;   // estimate: 24 KB of LDS per workgroup
 constexpr uint32_t kDefaultClusterWords = 8192; // clustering: 32 KB
 constexpr uint32_t kClusterEntries = kWave; // candidate entries per wave
@@ -371,7 +372,11 @@ struct ModuleBanks {
 
 // Bank b of an event of nwords words.  A bank whose offset table, header
 // or SP words would reach past the event is flagged and skipped, like a
-// mislabelled one (the reference reads on past the event).
+// mislabelled one (the reference reads on past the event).  So is a bank
+// of kMaxBankSps SPs or more (a 16 MB bank; VELO banks hold a few hundred
+// SPs), so that the SP counts of an event's 208 banks sum below 2^30.
+constexpr uint32_t kMaxBankSps = 1u << 22;
+
 template<typename P>
 __device__ inline void bank_of(
   P ev, uint32_t n_banks, uint32_t nwords, uint32_t b, uint32_t* base, uint32_t* n, uint32_t* status)
@@ -391,7 +396,7 @@ __device__ inline void bank_of(
   }
   const uint32_t sensor = ev[h];
   const uint32_t cnt = ev[h + 1];
-  if (sensor != b || cnt > nwords - 2u - h) {
+  if (sensor != b || cnt > nwords - 2u - h || cnt >= kMaxBankSps) {
     *status |= VELO_STATUS_BAD_BANK;
     return;
   }
@@ -519,135 +524,271 @@ __device__ inline uint32_t block_or(uint32_t v, uint32_t* shared_word)
 }
 
 // ---------------------------------------------------------------------------
-// K1: estimate_input_size
+// K1 over one flat SP list per event
 // ---------------------------------------------------------------------------
-template<typename P>
-__device__ __forceinline__ void estimate_event(
-  P ev, const BankTable& banks, const velo_estimate_input_size_args& a, uint32_t e, uint32_t* module_cands,
-  uint32_t* module_cand_offset, uint32_t* block_status, uint32_t* stash)
+// The SPs of all banks are numbered in bank order (flat index f).  The
+// banks that hold SPs are numbered k = 0..nnz-1; bank k holds flat SPs
+// [start[k], start[k+1]) and SP f of it is event word f + delta[k].  Wave w
+// takes a contiguous run of 64-SP chunks, so every wave does the same work
+// whatever the module sizes; within a chunk each lane finds its bank from
+// the bank starts that fall inside the chunk (one LDS read and a wave OR,
+// no search).  Flat order is bank order, which is module order, so the
+// candidate slots are one running count per wave.
+struct FlatBanks {
+  uint32_t start[VELO_N_SENSORS + 1]; // start[nnz] = number of SPs
+  uint32_t delta[VELO_N_SENSORS];
+  uint32_t unsorted[(VELO_N_SENSORS + 31) / 32]; // bit k: bank k's addresses do not strictly increase
+  uint32_t est[VELO_N_MODULES];
+  uint32_t cand[VELO_N_MODULES];
+  uint32_t wave_cands[1024 / kWave];
+  uint32_t nnz, total, status;
+  uint8_t bank[VELO_N_SENSORS];
+};
+constexpr uint32_t kFlatZeroWords = (VELO_N_SENSORS + 31) / 32 + 2 * VELO_N_MODULES;
+
+// OR over the wave (the same value on every lane)
+__device__ inline uint32_t wave_or(uint32_t v)
 {
-  // stash: the event's LDS copy (null when the event is read from global
-  // memory).  Pass 1 keeps each non-isolated SP's candidate byte in bits
-  // 23-30 of its word there, bits no reader of the copy looks at, so pass 2
-  // need not compute it again.
+  v |= (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x111, 0xf, 0xf, false);
+  v |= (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x112, 0xf, 0xf, false);
+  v |= (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x114, 0xf, 0xf, false);
+  v |= (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x118, 0xf, 0xf, false);
+  v |= (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x142, 0xa, 0xf, false);
+  v |= (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x143, 0xc, 0xf, false);
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
+// The lane's SP f0 + lane: its bank k and the bank's flat range [s, e).
+struct FlatSp {
+  uint32_t k, s, e;
+};
+
+// kcur (the same on every lane): the bank holding SP f0; on return, the
+// bank holding SP f0 + 64.
+__device__ inline FlatSp flat_sp(const FlatBanks& fb, uint32_t nnz, uint32_t f0, uint32_t& kcur)
+{
+  const uint32_t lane = lane_id();
+  const uint32_t kk = kcur + lane;
+  // starts increase strictly, so lanes 1.. hold starts above f0 and the
+  // ones inside the chunk come first
+  const uint32_t sl = kk <= nnz ? fb.start[kk] : 0xFFFFFFFFu;
+  const uint32_t p = sl - f0;
+  const bool inside = lane > 0 && p < kWave;
+  const uint32_t lo = inside && p < 32u ? 1u << p : 0u;
+  const uint32_t hi = inside && p >= 32u ? 1u << (p - 32u) : 0u;
+  const uint64_t bits = ((uint64_t) wave_or(hi) << 32) | (uint64_t) wave_or(lo);
+  const uint64_t upto = lane == kWave - 1 ? ~0ull : (2ull << lane) - 1ull;
+  const uint64_t below = bits & upto, above = bits & ~upto;
+  const uint32_t inner = (uint32_t) __popcll(bits);
+  uint32_t next; // first bank start past the chunk
+  if (inner + 1 < kWave) next = __builtin_amdgcn_readlane(sl, inner + 1);
+  else next = kcur + kWave <= nnz ? fb.start[kcur + kWave] : 0xFFFFFFFFu;
+  FlatSp r;
+  r.k = kcur + (uint32_t) __popcll(below);
+  r.s = below ? f0 + 63u - (uint32_t) __clzll(below) : __builtin_amdgcn_readlane(sl, 0);
+  r.e = above ? f0 + (uint32_t) __ffsll((unsigned long long) above) - 1u : next;
+  kcur += inner + (next == f0 + kWave ? 1u : 0u);
+  return r;
+}
+
+// The bank holding SP f (f below the number of SPs).
+__device__ inline uint32_t flat_bank_of(const FlatBanks& fb, uint32_t nnz, uint32_t f)
+{
+  uint32_t c = 0;
+  for (uint32_t k0 = 0; k0 < nnz; k0 += kWave) {
+    const uint32_t k = k0 + lane_id();
+    c += (uint32_t) __popcll(ballot(k < nnz && fb.start[k] <= f));
+  }
+  return c - 1u;
+}
+
+// Bank table of the event in flat form, by wave 0.
+template<typename P>
+__device__ inline void build_flat_banks(P ev, uint32_t nwords, FlatBanks& fb)
+{
+  const uint32_t lane = lane_id();
+  const uint32_t n_banks = nwords ? ev[0] : 0u;
+  uint32_t nnz = 0, total = 0, st = 0;
+#pragma unroll
+  for (uint32_t b0 = 0; b0 < VELO_N_SENSORS; b0 += kWave) {
+    const uint32_t b = b0 + lane;
+    uint32_t base = 0, n = 0;
+    if (b < VELO_N_SENSORS) bank_of(ev, n_banks, nwords, b, &base, &n, &st);
+    // a bank holds fewer than 2^22 SPs, so no sum here wraps
+    const uint64_t nz = ballot(n != 0);
+    const uint32_t inc = wave_inclusive_scan(n);
+    const uint32_t s = total + inc - n;
+    const uint32_t k = nnz + count_below(nz);
+    if (n) {
+      fb.start[k] = s;
+      fb.delta[k] = base - s;
+      fb.bank[k] = (uint8_t) b;
+    }
+    nnz += (uint32_t) __popcll(nz);
+    total += __builtin_amdgcn_readlane(inc, 63);
+  }
+  st = wave_or(st);
+  if (lane == 0) {
+    fb.start[nnz] = total;
+    fb.nnz = nnz;
+    fb.total = total;
+    fb.status = st;
+  }
+}
+
+template<typename P>
+__device__ __forceinline__ void estimate_event_flat(
+  P ev, FlatBanks& fb, const velo_estimate_input_size_args& a, uint32_t e, uint32_t* stash)
+{
+  // stash: as in estimate_event
   const uint32_t waves = blockDim.x / kWave;
   const uint32_t wave = threadIdx.x / kWave;
   const uint32_t lane = lane_id();
-  uint32_t status = 0;
+  const uint32_t nnz = fb.nnz, total = fb.total;
+  const uint32_t chunks = (total + kWave - 1) / kWave;
+  const uint32_t per_wave = (chunks + waves - 1) / waves;
+  const uint32_t c0 = wave * per_wave < chunks ? wave * per_wave : chunks;
+  const uint32_t c1 = c0 + per_wave < chunks ? c0 + per_wave : chunks;
+  const uint32_t kfirst = c0 < c1 ? flat_bank_of(fb, nnz, c0 * kWave) : 0u;
 
-  // pass 1: estimates and candidate counts per module
-  for (uint32_t m = wave; m < VELO_N_MODULES; m += waves) {
-    ModuleBanks mb = module_banks(banks, m);
-    check_sorted(ev, mb);
-    status |= mb.status;
-    if (mb.sorted != 0xFu) status |= VELO_STATUS_UNSORTED_BANK;
-    uint32_t est = 0, ncand = 0;
-    for (uint32_t j = lane; j < mb.total; j += kWave) {
-      uint32_t k;
-      const uint32_t i = bank_slot(mb, j, &k);
-      const uint32_t base = sel4(k, mb.b0, mb.b1, mb.b2, mb.b3);
-      const uint32_t w = ev[base + i];
-      if (w & 0x80000000u) {
-        est += isolated_cluster_count(w & 0xFFu);
+  // pass 1: estimates and candidate counts, summed per module.  Banks are
+  // taken as sorted and checked on the way; an event with an unsorted bank
+  // (the flags are then known) runs the pass again.
+  uint32_t wave_cands = 0;
+  for (uint32_t attempt = 0;; ++attempt) {
+    wave_cands = 0;
+    uint32_t kcur = kfirst;
+    for (uint32_t c = c0; c < c1; ++c) {
+      const uint32_t f = c * kWave + lane;
+      const FlatSp sp = flat_sp(fb, nnz, c * kWave, kcur);
+      uint32_t est = 0, nc = 0, mod = 0xFFFFFFFFu;
+      if (f < total) {
+        const uint32_t d = fb.delta[sp.k];
+        mod = fb.bank[sp.k] / 4u;
+        const uint32_t w = ev[f + d];
+        if (attempt == 0 && f + 1 < sp.e && sp_address(w) >= sp_address(ev[f + d + 1]))
+          atomicOr(&fb.unsorted[sp.k / 32], 1u << (sp.k % 32));
+        if (w & 0x80000000u) {
+          est = isolated_cluster_count(w & 0xFFu);
+        }
+        else {
+          const uint32_t base = sp.s + d, n = sp.e - sp.s;
+          const bool sorted = attempt == 0 || !((fb.unsorted[sp.k / 32] >> (sp.k % 32)) & 1u);
+          const uint32_t cb = candidates_from_pixels(sorted ? neighbour_pixels_sorted(w, f - sp.s, ev + base, n)
+                                                            : neighbour_pixels(w, ev + base, n));
+          if (stash) stash[f + d] = (w & 0x807FFFFFu) | (cb << 23);
+          nc = ((cb & 0x0Fu) ? 1u : 0u) + ((cb & 0xF0u) ? 1u : 0u);
+          est = nc;
+        }
       }
-      else {
-        const uint32_t n = sel4(k, mb.n0, mb.n1, mb.n2, mb.n3);
-        const uint32_t c = candidates_from_pixels(((mb.sorted >> k) & 1u) ? neighbour_pixels_sorted(w, i, ev + base, n)
-                                                                          : neighbour_pixels(w, ev + base, n));
-        if (stash) stash[base + i] = (w & 0x807FFFFFu) | (c << 23);
-        const uint32_t f = ((c & 0x0Fu) ? 1u : 0u) + ((c & 0xF0u) ? 1u : 0u);
-        est += f;
-        ncand += f;
+      // modules are runs of lanes: the last lane of each run adds the scan
+      // up to it to its module and takes it off the next run's module
+      const uint32_t ie = wave_inclusive_scan(est);
+      const uint32_t ic = wave_inclusive_scan(nc);
+      const uint32_t nmod = __shfl_down(mod, 1, kWave);
+      const bool last = lane == kWave - 1 || nmod != mod;
+      if (last && mod != 0xFFFFFFFFu) {
+        atomicAdd(&fb.est[mod], ie);
+        atomicAdd(&fb.cand[mod], ic);
       }
+      if (last && lane != kWave - 1 && mod != 0xFFFFFFFFu && nmod != 0xFFFFFFFFu) {
+        atomicSub(&fb.est[nmod], ie);
+        atomicSub(&fb.cand[nmod], ic);
+      }
+      wave_cands += __builtin_amdgcn_readlane(ic, 63);
     }
-    est = wave_sum(est);
-    ncand = wave_sum(ncand);
-    if (lane == 0) {
-      a.dev_estimated_input_size[e * VELO_N_MODULES + m] = est;
-      a.dev_module_cluster_num[e * VELO_N_MODULES + m] = 0;
-      module_cands[m] = ncand;
+    if (lane == 0) fb.wave_cands[wave] = wave_cands;
+    __syncthreads();
+    bool redo = false;
+    for (uint32_t i = 0; i < (VELO_N_SENSORS + 31) / 32; ++i) redo |= fb.unsorted[i] != 0;
+    if (attempt > 0 || !redo) break;
+    __syncthreads();
+    for (uint32_t m = threadIdx.x; m < VELO_N_MODULES; m += blockDim.x) {
+      fb.est[m] = 0;
+      fb.cand[m] = 0;
     }
+    __syncthreads();
   }
-  __syncthreads();
-  if (threadIdx.x < kWave) {
-    const uint32_t v = lane < VELO_N_MODULES ? module_cands[lane] : 0u;
-    const uint32_t inc = wave_inclusive_scan(v);
-    if (lane < VELO_N_MODULES) module_cand_offset[lane] = inc - v;
-    if (lane == VELO_N_MODULES - 1) module_cand_offset[VELO_N_MODULES] = inc;
+
+  for (uint32_t m = threadIdx.x; m < VELO_N_MODULES; m += blockDim.x) {
+    a.dev_estimated_input_size[e * VELO_N_MODULES + m] = fb.est[m];
+    a.dev_module_cluster_num[e * VELO_N_MODULES + m] = 0;
   }
-  __syncthreads();
+  const uint32_t before = wave_sum(lane < wave ? fb.wave_cands[lane] : 0u);
+  if (threadIdx.x == 0) {
+    uint32_t total_cands = 0;
+    for (uint32_t w = 0; w < waves; ++w) total_cands += fb.wave_cands[w];
+    uint32_t s = fb.status | VELO_STATUS_BANK_ORDER_CHECKED;
+    for (uint32_t i = 0; i < (VELO_N_SENSORS + 31) / 32; ++i)
+      if (fb.unsorted[i]) s |= VELO_STATUS_UNSORTED_BANK;
+    if (total_cands > VELO_MAX_CANDIDATES_EVENT) s |= VELO_STATUS_CANDIDATES_OVERFLOW;
+    a.dev_event_candidate_num[e] = total_cands < VELO_MAX_CANDIDATES_EVENT ? total_cands : VELO_MAX_CANDIDATES_EVENT;
+    // first writer of the event's status word: store (no memset needed)
+    if (a.dev_velo_event_status) a.dev_velo_event_status[e] = s;
+  }
+  if (!wave_cands) return;
 
   // pass 2: candidates in bank, SP, low-then-high-nibble order
   uint32_t* cands = a.dev_cluster_candidates + (size_t) e * VELO_MAX_CANDIDATES_EVENT;
-  for (uint32_t m = wave; m < VELO_N_MODULES; m += waves) {
-    if (module_cands[m] == 0) continue;
-    ModuleBanks mb = module_banks(banks, m);
-    if (!stash) check_sorted(ev, mb);
-    uint32_t pos = module_cand_offset[m];
-    for (uint32_t jb = 0; jb < mb.total; jb += kWave) {
-      const uint32_t j = jb + lane;
-      uint32_t c = 0, i = 0, k = 0;
-      if (j < mb.total) {
-        i = bank_slot(mb, j, &k);
-        const uint32_t base = sel4(k, mb.b0, mb.b1, mb.b2, mb.b3);
-        const uint32_t w = ev[base + i];
-        const uint32_t n = sel4(k, mb.n0, mb.n1, mb.n2, mb.n3);
-        if (!(w & 0x80000000u))
-          c = stash ? (w >> 23) & 0xFFu
-                    : candidates_from_pixels(((mb.sorted >> k) & 1u) ? neighbour_pixels_sorted(w, i, ev + base, n)
-                                                                     : neighbour_pixels(w, ev + base, n));
+  uint32_t pos = before;
+  uint32_t kcur = kfirst;
+  for (uint32_t c = c0; c < c1; ++c) {
+    const uint32_t f = c * kWave + lane;
+    const FlatSp sp = flat_sp(fb, nnz, c * kWave, kcur);
+    uint32_t cb = 0, b = 0;
+    if (f < total) {
+      const uint32_t d = fb.delta[sp.k];
+      b = fb.bank[sp.k];
+      const uint32_t w = ev[f + d];
+      if (!(w & 0x80000000u)) {
+        if (stash) {
+          cb = (w >> 23) & 0xFFu;
+        }
+        else {
+          const uint32_t base = sp.s + d, n = sp.e - sp.s;
+          const bool sorted = !((fb.unsorted[sp.k / 32] >> (sp.k % 32)) & 1u);
+          cb = candidates_from_pixels(sorted ? neighbour_pixels_sorted(w, f - sp.s, ev + base, n)
+                                             : neighbour_pixels(w, ev + base, n));
+        }
       }
-      const uint32_t cnt = ((c & 0x0Fu) ? 1u : 0u) + ((c & 0xF0u) ? 1u : 0u);
-      uint32_t total;
-      const uint32_t pre = wave_prefix_small(cnt, &total);
-      uint32_t slot = pos + pre;
-      const uint32_t b = 4 * m + k;
-      if (c & 0x0Fu) {
-        if (slot < VELO_MAX_CANDIDATES_EVENT) cands[slot] = (i << 11) | (b << 3) | c_candidate_ks[c & 0x0Fu];
-        ++slot;
-      }
-      if ((c & 0xF0u) && slot < VELO_MAX_CANDIDATES_EVENT)
-        cands[slot] = (i << 11) | (b << 3) | (c_candidate_ks[c >> 4] + 2u);
-      pos += total;
     }
-  }
-  const uint32_t total_cands = module_cand_offset[VELO_N_MODULES];
-  if (threadIdx.x == 0) {
-    a.dev_event_candidate_num[e] = total_cands < VELO_MAX_CANDIDATES_EVENT ? total_cands : VELO_MAX_CANDIDATES_EVENT;
-    if (total_cands > VELO_MAX_CANDIDATES_EVENT) status |= VELO_STATUS_CANDIDATES_OVERFLOW;
-  }
-  if (a.dev_velo_event_status) {
-    // first writer of the event's status word: store (no memset needed)
-    const uint32_t s = block_or(status, block_status);
-    if (threadIdx.x == 0) a.dev_velo_event_status[e] = s | VELO_STATUS_BANK_ORDER_CHECKED;
+    const uint32_t cnt = ((cb & 0x0Fu) ? 1u : 0u) + ((cb & 0xF0u) ? 1u : 0u);
+    uint32_t tot;
+    const uint32_t pre = wave_prefix_small(cnt, &tot);
+    uint32_t slot = pos + pre;
+    const uint32_t i = f - sp.s;
+    if (cb & 0x0Fu) {
+      if (slot < VELO_MAX_CANDIDATES_EVENT) cands[slot] = (i << 11) | (b << 3) | c_candidate_ks[cb & 0x0Fu];
+      ++slot;
+    }
+    if ((cb & 0xF0u) && slot < VELO_MAX_CANDIDATES_EVENT)
+      cands[slot] = (i << 11) | (b << 3) | (c_candidate_ks[cb >> 4] + 2u);
+    pos += tot;
   }
 }
 
 __global__ void __launch_bounds__(1024) estimate_input_size_kernel(velo_estimate_input_size_args a)
 {
   extern __shared__ uint32_t lds[];
-  __shared__ uint32_t module_cands[VELO_N_MODULES];
-  __shared__ uint32_t module_cand_offset[VELO_N_MODULES + 1];
-  __shared__ uint32_t block_status;
-  __shared__ BankTable banks;
+  __shared__ FlatBanks fb;
   for_each_event(a.tuning, a.n_events, [&](const uint32_t e) {
     const uint32_t ei = event_index(a.dev_event_list, e);
     const uint32_t off = a.dev_raw_input_offsets[ei];
     const uint32_t nwords = (a.dev_raw_input_offsets[ei + 1] - off) / 4u;
     const uint32_t* gev = reinterpret_cast<const uint32_t*>(a.dev_raw_input + off);
-    if (threadIdx.x == 0) block_status = 0;
+    uint32_t* zero = fb.unsorted;
+    for (uint32_t i = threadIdx.x; i < kFlatZeroWords; i += blockDim.x) zero[i] = 0;
     if (nwords <= a.tuning.lds_tile) {
       stage_words(lds, gev, nwords);
       __syncthreads();
-      build_bank_table(static_cast<const uint32_t*>(lds), nwords, banks);
+      if (threadIdx.x < kWave) build_flat_banks(static_cast<const uint32_t*>(lds), nwords, fb);
       __syncthreads();
-      estimate_event(static_cast<const uint32_t*>(lds), banks, a, e, module_cands, module_cand_offset, &block_status, lds);
+      estimate_event_flat(static_cast<const uint32_t*>(lds), fb, a, e, lds);
     }
     else {
+      if (threadIdx.x < kWave) build_flat_banks(gev, nwords, fb);
       __syncthreads();
-      build_bank_table(gev, nwords, banks);
-      __syncthreads();
-      estimate_event(gev, banks, a, e, module_cands, module_cand_offset, &block_status, nullptr);
+      estimate_event_flat(gev, fb, a, e, nullptr);
     }
   });
 }

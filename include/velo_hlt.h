@@ -57,7 +57,7 @@ extern "C" {
 #define VELO_STATUS_TRACKLETS_OVERFLOW 0x020u  /* >2000 tracklets within two steps */
 #define VELO_STATUS_WEAK_OVERFLOW 0x040u       /* >500 weak tracks; extra dropped */
 #define VELO_STATUS_TRACKS_OVERFLOW 0x080u     /* >1200 tracks; extra dropped */
-#define VELO_STATUS_BAD_BANK 0x100u            /* bank index != sensor index, or the bank reaches past its event; bank skipped */
+#define VELO_STATUS_BAD_BANK 0x100u            /* bank index != sensor index, the bank reaches past its event, or it holds 2^22 SPs or more; bank skipped */
 #define VELO_STATUS_UNSORTED_BANK 0x200u       /* a bank's SP addresses do not strictly increase (set by estimate_input_size) */
 /* Not an error: estimate_input_size sets this bit on every event it
  * processes, after checking the order of the event's banks.  Masked

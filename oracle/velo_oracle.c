@@ -149,7 +149,9 @@ static raw_event make_event(const char* ev, uint32_t n_words)
 /* Bank b starts at word n_banks + 2 + offset / 4 (the decoder's word
    addressing).  A bank whose offset table, header or SP words would reach
    past the event does not fit: it is flagged and skipped like a mislabelled
-   one (the reference reads on past the event). */
+   one (the reference reads on past the event).  Neither does a bank of 2^22
+   SPs or more (16 MB; VELO banks hold a few hundred), a limit of the device
+   decoder, which sums the SP counts of an event's banks in 32 bits. */
 static raw_bank make_bank(const raw_event* e, uint32_t b)
 {
   raw_bank k;
@@ -164,7 +166,7 @@ static raw_bank make_bank(const raw_event* e, uint32_t b)
   k.sensor_index = rd32(p);
   k.sp_count = rd32(p + 4);
   k.sp_words = p + 8;
-  k.fits = h + 2u + k.sp_count <= e->n_words;
+  k.fits = h + 2u + k.sp_count <= e->n_words && k.sp_count < (1u << 22);
   if (!k.fits) k.sp_count = 0;
   return k;
 }
