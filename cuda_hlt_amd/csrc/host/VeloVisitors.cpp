@@ -298,7 +298,8 @@ void SequenceVisitor::visit<velo_masked_clustering_t>(
   hipStream_t& stream,
   hipEvent_t&)
 {
-  state.set_opts(dim3(host_buffers.host_number_of_selected_events[0]), dim3(512), stream);
+  // 256 threads: four waves per event keep more events in flight per CU
+  state.set_opts(dim3(host_buffers.host_number_of_selected_events[0]), dim3(256), stream);
   state.set_arguments(
     arguments.offset<dev_velo_raw_input>(),
     arguments.offset<dev_velo_raw_input_offsets>(),

@@ -2,16 +2,18 @@
 // clustering for gfx950, plus the device constants.
 //
 // Work layout: one workgroup per event.  The workgroup first copies the raw
-// event into LDS (coalesced).  estimate_input_size then takes the event's
-// SPs as one flat list in bank order, 64 per wave step, each wave a
-// contiguous run of them; it resolves each SP's neighbourhood in its bank
-// in LDS, sums the estimates per module and lists the candidates in bank,
-// SP, nibble order.  Masked clustering gives wave w modules w, w+W, w+2W,
-// ... (W = waves per workgroup), takes the SPs of a module's four banks as
-// one list, one SP per lane, reads the candidate list (or scans the banks
+// event into LDS (coalesced) and wave 0 builds a flat table of the event's
+// banks: the SPs of all banks numbered in bank order, which is module
+// order.  estimate_input_size gives each wave a contiguous run of 64-SP
+// chunks, resolves each SP's neighbourhood in its bank in LDS, sums the
+// estimates per module and lists the candidates in bank, SP, nibble order.
+// Masked clustering gives each wave the whole modules nearest its share of
+// the SPs, takes up to 64 consecutive SPs per pass whatever module they
+// belong to, reads their candidates from the list (or scans the banks
 // itself when it has none) and grows one candidate per lane.  Output slots
-// come from ballot-based wave prefix sums, so clusters are written in bank,
-// SP, cluster order with no atomics and no cross-wave synchronisation.
+// come from ballot-based wave prefix sums restarted at each module, so
+// clusters are written in bank, SP, cluster order with no atomics and no
+// cross-wave synchronisation.
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -353,22 +355,8 @@ __global__ void __launch_bounds__(kOrderBlock)
 // it fits (tuning.lds_tile words); otherwise the same code reads it from
 // global memory.
 constexpr uint32_t kDefaultEventWords = 6144;   // estimate: 24 KB of LDS per workgroup
-constexpr uint32_t kDefaultClusterWords = 8192; // clustering: 32 KB
+constexpr uint32_t kDefaultClusterWords = 6144; // clustering: 24 KB (event and candidate list)
 constexpr uint32_t kClusterEntries = kWave; // candidate entries per wave
-constexpr uint32_t kClusterSeeds = 6 * kWave; // staged clusters per wave: key, fx, fy for 128
-
-__device__ inline uint32_t sel4(uint32_t k, uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3)
-{
-  return k == 0 ? a0 : (k == 1 ? a1 : (k == 2 ? a2 : a3));
-}
-
-struct ModuleBanks {
-  uint32_t b0, b1, b2, b3; // word index of sp_word[0] of banks 4m..4m+3
-  uint32_t n0, n1, n2, n3; // SP counts (0 for a missing or mislabelled bank)
-  uint32_t p1, p2, p3, total;
-  uint32_t status;
-  uint32_t sorted; // bit k: bank 4m+k has strictly increasing SP addresses
-};
 
 // Bank b of an event of nwords words.  A bank whose offset table, header
 // or SP words would reach past the event is flagged and skipped, like a
@@ -405,97 +393,6 @@ __device__ inline void bank_of(
   *n = cnt;
 }
 
-// The event's banks, one entry per sensor (first SP word, SP count, status
-// bits), found once per event by the whole workgroup.
-struct BankTable {
-  uint32_t base[VELO_N_SENSORS];
-  uint32_t n[VELO_N_SENSORS];
-  uint32_t status[VELO_N_SENSORS];
-};
-
-template<typename P>
-__device__ inline void build_bank_table(P ev, uint32_t nwords, BankTable& t)
-{
-  const uint32_t n_banks = nwords ? ev[0] : 0u;
-  for (uint32_t b = threadIdx.x; b < VELO_N_SENSORS; b += blockDim.x) {
-    uint32_t base, n, st = 0;
-    bank_of(ev, n_banks, nwords, b, &base, &n, &st);
-    t.base[b] = base;
-    t.n[b] = n;
-    t.status[b] = st;
-  }
-}
-
-__device__ inline ModuleBanks module_banks(const BankTable& t, uint32_t m)
-{
-  ModuleBanks mb;
-  mb.b0 = t.base[4 * m + 0];
-  mb.b1 = t.base[4 * m + 1];
-  mb.b2 = t.base[4 * m + 2];
-  mb.b3 = t.base[4 * m + 3];
-  mb.n0 = t.n[4 * m + 0];
-  mb.n1 = t.n[4 * m + 1];
-  mb.n2 = t.n[4 * m + 2];
-  mb.n3 = t.n[4 * m + 3];
-  mb.status = t.status[4 * m + 0] | t.status[4 * m + 1] | t.status[4 * m + 2] | t.status[4 * m + 3];
-  // the same on every lane: kept in scalar registers
-  mb.b0 = __builtin_amdgcn_readfirstlane(mb.b0);
-  mb.b1 = __builtin_amdgcn_readfirstlane(mb.b1);
-  mb.b2 = __builtin_amdgcn_readfirstlane(mb.b2);
-  mb.b3 = __builtin_amdgcn_readfirstlane(mb.b3);
-  mb.n0 = __builtin_amdgcn_readfirstlane(mb.n0);
-  mb.n1 = __builtin_amdgcn_readfirstlane(mb.n1);
-  mb.n2 = __builtin_amdgcn_readfirstlane(mb.n2);
-  mb.n3 = __builtin_amdgcn_readfirstlane(mb.n3);
-  mb.p1 = mb.n0;
-  mb.p2 = mb.p1 + mb.n1;
-  mb.p3 = mb.p2 + mb.n2;
-  mb.total = mb.p3 + mb.n3;
-  mb.sorted = 0xFu;
-  return mb;
-}
-
-__device__ inline uint32_t bank_slot(const ModuleBanks& mb, uint32_t j, uint32_t* k);
-
-// Clears mb.sorted bit k for every bank whose SP addresses do not strictly
-// increase (wave-wide; uniform result).
-template<typename P>
-__device__ inline void check_sorted(P ev, ModuleBanks& mb)
-{
-  uint32_t bad = 0;
-  for (uint32_t jb = 0; jb < mb.total; jb += kWave) {
-    const uint32_t j = jb + lane_id();
-    bool b = false;
-    if (j < mb.total) {
-      uint32_t k;
-      const uint32_t i = bank_slot(mb, j, &k);
-      const uint32_t base = sel4(k, mb.b0, mb.b1, mb.b2, mb.b3);
-      const uint32_t n = sel4(k, mb.n0, mb.n1, mb.n2, mb.n3);
-      b = i + 1 < n && sp_address(ev[base + i]) >= sp_address(ev[base + i + 1]);
-    }
-    const uint64_t m = ballot(b);
-    if (m) {
-      // the lanes of bank k are the module positions [p_k, p_k+1) - jb
-      const uint32_t bounds[5] = {0u, mb.p1, mb.p2, mb.p3, mb.total};
-      for (uint32_t k = 0; k < 4; ++k) {
-        const uint32_t lo = bounds[k] > jb ? bounds[k] - jb : 0u;
-        const uint32_t hi = bounds[k + 1] > jb ? bounds[k + 1] - jb : 0u;
-        if (lo >= kWave || hi <= lo) continue;
-        const uint64_t upto = hi >= kWave ? ~0ull : ((1ull << hi) - 1ull);
-        if (m & upto & ~((1ull << lo) - 1ull)) bad |= 1u << k;
-      }
-    }
-  }
-  mb.sorted &= ~bad;
-}
-
-// Position j of the module's SP list -> (bank k, index in bank).
-__device__ inline uint32_t bank_slot(const ModuleBanks& mb, uint32_t j, uint32_t* k)
-{
-  *k = (j >= mb.p1 ? 1u : 0u) + (j >= mb.p2 ? 1u : 0u) + (j >= mb.p3 ? 1u : 0u);
-  return j - sel4(*k, 0u, mb.p1, mb.p2, mb.p3);
-}
-
 // Copies n words to LDS with the whole workgroup, 4 loads in flight per lane.
 __device__ inline void stage_words(uint32_t* dst, const uint32_t* src, uint32_t n)
 {
@@ -515,14 +412,6 @@ __device__ inline void stage_words(uint32_t* dst, const uint32_t* src, uint32_t 
   }
 }
 
-__device__ inline uint32_t block_or(uint32_t v, uint32_t* shared_word)
-{
-  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
-  if (lane_id() == 0 && v) atomicOr(shared_word, v);
-  __syncthreads();
-  return *shared_word;
-}
-
 // ---------------------------------------------------------------------------
 // K1 over one flat SP list per event
 // ---------------------------------------------------------------------------
@@ -535,14 +424,18 @@ __device__ inline uint32_t block_or(uint32_t v, uint32_t* shared_word)
 // no search).  Flat order is bank order, which is module order, so the
 // candidate slots are one running count per wave.
 struct FlatBanks {
-  uint32_t start[VELO_N_SENSORS + 1]; // start[nnz] = number of SPs
-  uint32_t delta[VELO_N_SENSORS];
-  uint32_t unsorted[(VELO_N_SENSORS + 31) / 32]; // bit k: bank k's addresses do not strictly increase
-  uint32_t est[VELO_N_MODULES];
+  uint32_t start[VELO_N_SENSORS + 1]; // by k; start[nnz] = number of SPs
+  uint32_t delta[VELO_N_SENSORS];     // by k
+  uint32_t bstart[VELO_N_SENSORS + 1]; // flat index of the first SP, by bank; [208] = number of SPs
+  uint32_t bbase[VELO_N_SENSORS];     // event word of the first SP, by bank
+  uint32_t bn[VELO_N_SENSORS];        // SP count, by bank
+  uint32_t mst[VELO_N_MODULES + 1];   // clustering: the event's module cluster starts and end
+  uint32_t unsorted[(VELO_N_SENSORS + 31) / 32]; // bit b: bank b's addresses do not strictly increase
+  uint32_t est[VELO_N_MODULES];       // estimate: estimates; clustering: cluster counts
   uint32_t cand[VELO_N_MODULES];
   uint32_t wave_cands[1024 / kWave];
   uint32_t nnz, total, status;
-  uint8_t bank[VELO_N_SENSORS];
+  uint8_t bank[VELO_N_SENSORS];       // by k
 };
 constexpr uint32_t kFlatZeroWords = (VELO_N_SENSORS + 31) / 32 + 2 * VELO_N_MODULES;
 
@@ -563,9 +456,9 @@ struct FlatSp {
   uint32_t k, s, e;
 };
 
-// kcur (the same on every lane): the bank holding SP f0; on return, the
-// bank holding SP f0 + 64.
-__device__ inline FlatSp flat_sp(const FlatBanks& fb, uint32_t nnz, uint32_t f0, uint32_t& kcur)
+// kcur (the same on every lane): the bank holding SP f0; *knext: the bank
+// holding SP f0 + 64.
+__device__ inline FlatSp flat_sp(const FlatBanks& fb, uint32_t nnz, uint32_t f0, uint32_t kcur, uint32_t* knext)
 {
   const uint32_t lane = lane_id();
   const uint32_t kk = kcur + lane;
@@ -587,7 +480,7 @@ __device__ inline FlatSp flat_sp(const FlatBanks& fb, uint32_t nnz, uint32_t f0,
   r.k = kcur + (uint32_t) __popcll(below);
   r.s = below ? f0 + 63u - (uint32_t) __clzll(below) : __builtin_amdgcn_readlane(sl, 0);
   r.e = above ? f0 + (uint32_t) __ffsll((unsigned long long) above) - 1u : next;
-  kcur += inner + (next == f0 + kWave ? 1u : 0u);
+  *knext = kcur + inner + (next == f0 + kWave ? 1u : 0u);
   return r;
 }
 
@@ -619,6 +512,11 @@ __device__ inline void build_flat_banks(P ev, uint32_t nwords, FlatBanks& fb)
     const uint32_t inc = wave_inclusive_scan(n);
     const uint32_t s = total + inc - n;
     const uint32_t k = nnz + count_below(nz);
+    if (b < VELO_N_SENSORS) {
+      fb.bstart[b] = s;
+      fb.bbase[b] = base;
+      fb.bn[b] = n;
+    }
     if (n) {
       fb.start[k] = s;
       fb.delta[k] = base - s;
@@ -630,6 +528,7 @@ __device__ inline void build_flat_banks(P ev, uint32_t nwords, FlatBanks& fb)
   st = wave_or(st);
   if (lane == 0) {
     fb.start[nnz] = total;
+    fb.bstart[VELO_N_SENSORS] = total;
     fb.nnz = nnz;
     fb.total = total;
     fb.status = st;
@@ -660,20 +559,21 @@ __device__ __forceinline__ void estimate_event_flat(
     uint32_t kcur = kfirst;
     for (uint32_t c = c0; c < c1; ++c) {
       const uint32_t f = c * kWave + lane;
-      const FlatSp sp = flat_sp(fb, nnz, c * kWave, kcur);
+      const FlatSp sp = flat_sp(fb, nnz, c * kWave, kcur, &kcur);
       uint32_t est = 0, nc = 0, mod = 0xFFFFFFFFu;
       if (f < total) {
         const uint32_t d = fb.delta[sp.k];
-        mod = fb.bank[sp.k] / 4u;
+        const uint32_t b = fb.bank[sp.k];
+        mod = b / 4u;
         const uint32_t w = ev[f + d];
         if (attempt == 0 && f + 1 < sp.e && sp_address(w) >= sp_address(ev[f + d + 1]))
-          atomicOr(&fb.unsorted[sp.k / 32], 1u << (sp.k % 32));
+          atomicOr(&fb.unsorted[b / 32], 1u << (b % 32));
         if (w & 0x80000000u) {
           est = isolated_cluster_count(w & 0xFFu);
         }
         else {
           const uint32_t base = sp.s + d, n = sp.e - sp.s;
-          const bool sorted = attempt == 0 || !((fb.unsorted[sp.k / 32] >> (sp.k % 32)) & 1u);
+          const bool sorted = attempt == 0 || !((fb.unsorted[b / 32] >> (b % 32)) & 1u);
           const uint32_t cb = candidates_from_pixels(sorted ? neighbour_pixels_sorted(w, f - sp.s, ev + base, n)
                                                             : neighbour_pixels(w, ev + base, n));
           if (stash) stash[f + d] = (w & 0x807FFFFFu) | (cb << 23);
@@ -734,7 +634,7 @@ __device__ __forceinline__ void estimate_event_flat(
   uint32_t kcur = kfirst;
   for (uint32_t c = c0; c < c1; ++c) {
     const uint32_t f = c * kWave + lane;
-    const FlatSp sp = flat_sp(fb, nnz, c * kWave, kcur);
+    const FlatSp sp = flat_sp(fb, nnz, c * kWave, kcur, &kcur);
     uint32_t cb = 0, b = 0;
     if (f < total) {
       const uint32_t d = fb.delta[sp.k];
@@ -746,7 +646,7 @@ __device__ __forceinline__ void estimate_event_flat(
         }
         else {
           const uint32_t base = sp.s + d, n = sp.e - sp.s;
-          const bool sorted = !((fb.unsorted[sp.k / 32] >> (sp.k % 32)) & 1u);
+          const bool sorted = !((fb.unsorted[b / 32] >> (b % 32)) & 1u);
           cb = candidates_from_pixels(sorted ? neighbour_pixels_sorted(w, f - sp.s, ev + base, n)
                                              : neighbour_pixels(w, ev + base, n));
         }
@@ -796,21 +696,54 @@ __global__ void __launch_bounds__(1024) estimate_input_size_kernel(velo_estimate
 // ---------------------------------------------------------------------------
 // K5: masked clustering
 // ---------------------------------------------------------------------------
-// Per wave iteration over up to 64 SPs of a module:
+// The event's SPs are taken in flat bank order (as in estimate_input_size).
+// Wave w takes the modules whose SPs fall nearest to its share of the list
+// (module boundaries closest to w/W of the SPs), so every wave has about the
+// same number of SPs and each module is one wave's.  Per wave pass over up
+// to 64 consecutive SPs:
 //  1. each lane classifies its SP: isolated (pattern table) or not;
 //  2. the candidates of the non-isolated SPs (low nibble first) form the
 //     wave's entries, one per lane, so one growth pass serves up to 64
-//     candidates whatever SP they came from.  They are read from the event's
-//     candidate list (written by estimate_input_size in bank, SP, nibble
-//     order) when it is given and staged, else derived from the raw banks;
-//  3. a wave prefix over the per-SP cluster counts gives each SP its slots,
+//     candidates whatever SP and module they came from.  They are read from
+//     the event's candidate list (written by estimate_input_size in bank,
+//     SP, nibble order) when it is given and staged, else derived from the
+//     raw banks;
+//  3. a wave prefix over the per-SP cluster counts, restarted at each module
+//     and carried across passes, gives each cluster its slot in its module,
 //     in bank, SP, cluster order.
-// cl / cl_mod: the event's candidate list in LDS and the first entry of each
-// module (53 values), or null.
+// The pass takes the SPs whose candidates fit in one wave of entries (at
+// least 32 SPs).
+
+// Value of the lane below (lane 0: its own) and of the lane above (lane 63:
+// its own), by DPP wave shifts.
+__device__ inline uint32_t lane_below(uint32_t v)
+{
+  return (uint32_t) __builtin_amdgcn_update_dpp((int) v, (int) v, 0x138, 0xf, 0xf, false); // wave_shr:1
+}
+__device__ inline uint32_t lane_above(uint32_t v)
+{
+  return (uint32_t) __builtin_amdgcn_update_dpp((int) v, (int) v, 0x130, 0xf, 0xf, false); // wave_shl:1
+}
+
+// First module of wave w's share: the module boundary nearest to w/W of the
+// event's SPs (boundaries are flat starts of banks 4m).
+__device__ inline uint32_t wave_first_module(const FlatBanks& fb, uint32_t waves, uint32_t w)
+{
+  if (w == 0) return 0;
+  if (w >= waves) return VELO_N_MODULES;
+  const uint32_t lane = lane_id();
+  const uint32_t t = (uint32_t) (((uint64_t) fb.total * w) / waves);
+  const uint32_t ms = lane <= VELO_N_MODULES ? fb.bstart[4 * lane] : 0xFFFFFFFFu;
+  const uint32_t lo = (uint32_t) __popcll(ballot(ms <= t)) - 1u; // bstart[0] = 0
+  if (lo >= VELO_N_MODULES) return VELO_N_MODULES;
+  const uint32_t slo = __builtin_amdgcn_readlane(ms, lo), shi = __builtin_amdgcn_readlane(ms, lo + 1);
+  return t - slo <= shi - t ? lo : lo + 1;
+}
+
 template<typename P>
 __device__ __forceinline__ void cluster_event(
-  P ev, const BankTable& banks, const velo_masked_clustering_args& a, uint32_t e, uint32_t* block_words,
-  uint32_t* entries, uint32_t* seeds, const uint32_t* cl, const uint32_t* cl_mod)
+  P ev, FlatBanks& fb, const velo_masked_clustering_args& a, uint32_t e, uint32_t* entries,
+  const uint32_t* cl, const uint32_t* cl_mod, bool check_order)
 {
   const uint32_t waves = blockDim.x / kWave;
   const uint32_t wave = threadIdx.x / kWave;
@@ -820,194 +753,209 @@ __device__ __forceinline__ void cluster_event(
   float* ys = xs + total;
   float* zs = ys + total;
   uint32_t* ids = a.dev_velo_cluster_container + 3 * (size_t) total;
-  const uint32_t* starts = a.dev_module_cluster_start + e * VELO_N_MODULES;
-  uint32_t status = 0;
-  uint32_t event_hits = 0;
-  // The bank-order check is skipped only where estimate_input_size checked
-  // the event's banks (BANK_ORDER_CHECKED) and found them sorted (no
-  // UNSORTED_BANK); any other status word keeps the check.
-  const bool check_order =
-    !a.dev_velo_event_status ||
-    (a.dev_velo_event_status[e] & (VELO_STATUS_BANK_ORDER_CHECKED | VELO_STATUS_UNSORTED_BANK)) !=
-      VELO_STATUS_BANK_ORDER_CHECKED;
+  const uint32_t nnz = fb.nnz;
+  const uint32_t m0 = wave_first_module(fb, waves, wave), m1 = wave_first_module(fb, waves, wave + 1);
+  const uint32_t fbeg = fb.bstart[4 * m0], fend = fb.bstart[4 * m1];
+  const uint32_t kfirst = fbeg < fend ? flat_bank_of(fb, nnz, fbeg) : 0u;
 
-  for (uint32_t m = wave; m < VELO_N_MODULES; m += waves) {
-    ModuleBanks mb = module_banks(banks, m);
-    if (check_order) check_sorted(ev, mb);
-    const uint32_t mstart = starts[m];
-    const uint32_t mend = starts[m + 1]; // the module's estimated slots end here
-    uint32_t pos = 0;
-    uint32_t cptr = cl ? cl_mod[m] : 0u;            // next list entry of the module
-    const uint32_t cend = cl ? cl_mod[m + 1] : 0u;
-    for (uint32_t jb = 0; jb < mb.total;) {
-      const uint32_t j = jb + lane;
-      uint32_t i = 0, k = 0, w = 0, c = 0, idx = 0;
-      bool live = j < mb.total;
-      if (live) {
-        i = bank_slot(mb, j, &k);
-        const uint32_t base = sel4(k, mb.b0, mb.b1, mb.b2, mb.b3);
-        const uint32_t n = sel4(k, mb.n0, mb.n1, mb.n2, mb.n3);
-        w = ev[base + i];
-        if (w & 0x80000000u) idx = a.dev_velo_sp_patterns[w & 0xFFu];
-        else if (!cl)
-          c = candidates_from_pixels(((mb.sorted >> k) & 1u) ? neighbour_pixels_sorted(w, i, ev + base, n)
-                                                             : neighbour_pixels(w, ev + base, n));
+  if (check_order) {
+    uint32_t kcur = kfirst;
+    for (uint32_t jb = fbeg; jb < fend; jb += kWave) {
+      const uint32_t f = jb + lane;
+      const FlatSp sp = flat_sp(fb, nnz, jb, kcur, &kcur);
+      if (f < fend && f + 1 < sp.e) {
+        const uint32_t d = fb.delta[sp.k];
+        const uint32_t b = fb.bank[sp.k];
+        if (sp_address(ev[f + d]) >= sp_address(ev[f + d + 1])) atomicOr(&fb.unsorted[b / 32], 1u << (b % 32));
       }
-      // per SP: first entry and number of entries; per entry: its SP
-      uint32_t first = 0, n_own = 0, taken, n_ent;
-      uint32_t src = 0, si = 0, sk = 0, kp = 0; // per entry: its SP's lane, SP index, bank, pixel
-      if (!cl) {
-        uint32_t has_lo = (c & 0x0Fu) ? 1u : 0u;
-        uint32_t has_hi = (c & 0xF0u) ? 1u : 0u;
-        first = wave_prefix_small(has_lo + has_hi, &n_ent);
-        // this pass takes the SPs whose candidates fit in one wave of
-        // entries (at least 32 SPs); the rest start the next pass
-        const uint32_t end = first + has_lo + has_hi;
-        if (end > kWave) {
-          live = false;
-          has_lo = has_hi = 0;
-        }
-        n_own = has_lo + has_hi;
-        taken = (uint32_t) __popcll(ballot(live));
-        n_ent = n_ent > kWave ? __shfl(end, taken - 1u, 64) : n_ent;
-        if (has_lo) entries[first] = lane;
-        if (has_hi) entries[first + has_lo] = lane | 64u;
-        wave_sync();
-        const uint32_t d = lane < n_ent ? entries[lane] : 0u;
-        src = d & 63u;
-        const uint32_t sc = __shfl(c, src, 64);
-        si = __shfl(i, src, 64);
-        sk = __shfl(k, src, 64);
-        kp = (d & 64u) ? c_candidate_ks[sc >> 4] + 2u : c_candidate_ks[sc & 0x0Fu];
-      }
-      else {
-        // entries of SPs [jb, jb + 64) from the list; when 64 entries do not
-        // reach past the pass, its SPs end before the SP of entry 64
-        const bool ok = cptr + lane < cend;
-        const uint32_t ent = ok ? cl[cptr + lane] : 0u;
-        si = ent >> 11;
-        sk = (ent >> 3) & 3u;
-        kp = ent & 7u;
-        const uint32_t epos = ok ? sel4(sk, 0u, mb.p1, mb.p2, mb.p3) + si : 0xFFFFFFFFu;
-        uint32_t limit = jb + kWave;
-        if (cptr + kWave < cend) {
-          const uint32_t e64 = cl[cptr + kWave];
-          const uint32_t p64 = sel4((e64 >> 3) & 3u, 0u, mb.p1, mb.p2, mb.p3) + (e64 >> 11);
-          limit = p64 < limit ? p64 : limit;
-        }
-        limit = limit < mb.total ? limit : mb.total;
-        limit = limit > jb ? limit : jb + 1; // a list that does not match the banks still ends
-        taken = limit - jb;
-        live = lane < taken;
-        const bool mine = epos < limit;
-        n_ent = (uint32_t) __popcll(ballot(mine));
-        src = mine ? epos - jb : 0u;
-        // the SP of each entry records its first entry and count
-        entries[lane] = 0u;
-        wave_sync();
-        const uint32_t prev = __shfl_up(src, 1, 64);
-        const bool lead = mine && (lane == 0 || prev != src);
-        const uint32_t next = __shfl_down(src, 1, 64);
-        const bool pair = lead && lane + 1 < n_ent && next == src;
-        if (lead) entries[src] = (lane + 1) | (pair ? 0x100u : 0u);
-        wave_sync();
-        if (live) {
-          const uint32_t info = entries[lane];
-          first = (info & 0xFFu) - 1u;
-          n_own = info ? 1u + (info >> 8) : 0u;
-        }
-        cptr += n_ent;
-      }
-      const bool iso = live && (w & 0x80000000u);
-
-      // growth: lane t takes entry t
-      HitSeed r {};
-      uint64_t acc = 0;
-      if (n_ent) {
-        bool good = false;
-        if (lane < n_ent) {
-          const uint32_t base = sel4(sk, mb.b0, mb.b1, mb.b2, mb.b3);
-          const uint32_t n = sel4(sk, mb.n0, mb.n1, mb.n2, mb.n3);
-          // the reference decodes the candidate's SP index as uint8_t (:179)
-          const uint32_t cw = ev[base + (si & 0xFFu)];
-          const uint32_t caddr = (cw & 0x007FFF00u) >> 8;
-          const int32_t sp_row = (int32_t) (caddr & 0x3Fu), sp_col = (int32_t) (caddr >> 6);
-          uint32_t pa[3];
-          if ((mb.sorted >> sk) & 1u) load_window_sorted(sp_row, sp_col, ev + base, n, si & 0xFFu, pa);
-          else load_window(sp_row, sp_col, ev + base, n, pa);
-          good = cluster_candidate(pa[0], pa[1], pa[2], sp_row, sp_col, kp, 4 * m + sk, &r);
-        }
-        acc = ballot(good);
-      }
-
-      // clusters per SP and their slots
-      const uint32_t own_ok = n_own ? (uint32_t) __popcll((acc >> first) & (n_own > 1 ? 3u : 1u)) : 0u;
-      const uint32_t cnt = iso ? ((idx & 8u) ? 2u : 1u) : own_ok;
-      // every cluster of the pass is staged at its slot relative to the
-      // pass (below 128), then transformed and stored one per lane
-      uint32_t tot;
-      const uint32_t rel = wave_prefix_small(cnt, &tot);
-      auto stage = [&](uint32_t t, uint32_t key, float fx, float fy) {
-        seeds[t] = key;
-        seeds[2 * kWave + t] = __float_as_uint(fx);
-        seeds[4 * kWave + t] = __float_as_uint(fy);
-      };
-      if (iso) {
-        // isolated SP: pre-computed pattern (MaskedVeloClustering.cu:105-167)
-        const uint32_t sp_addr = (w & 0x007FFF00u) >> 8, sp = w & 0xFFu;
-        const uint32_t sp_row = sp_addr & 0x3Fu, sp_col = sp_addr >> 6, sensor = (4 * m + k) << 18;
-        stage(rel, (sp_col * 2u + ((idx >> 2) & 1u)) | ((sp_row * 4u + (idx & 3u)) << 10) | sensor,
-              a.dev_velo_sp_fx[2 * sp], a.dev_velo_sp_fy[2 * sp]);
-        if (idx & 8u)
-          stage(rel + 1, (sp_col * 2u + ((idx >> 6) & 1u)) | ((sp_row * 4u + ((idx >> 4) & 3u)) << 10) | sensor,
-                a.dev_velo_sp_fx[2 * sp + 1], a.dev_velo_sp_fy[2 * sp + 1]);
-      }
-      if (n_ent) {
-        // an SP's second cluster follows its first when that one was kept
-        const uint32_t s = __shfl(rel, src, 64);
-        const uint32_t f = __shfl(first, src, 64);
-        const uint32_t after = lane > f ? (uint32_t) (acc >> f) & 1u : 0u;
-        if ((acc >> lane) & 1u) stage(s + after, r.key, r.fx, r.fy);
-      }
-      wave_sync();
-      for (uint32_t t = lane; t < tot; t += kWave) {
-        const uint32_t key = seeds[t];
-        const Hit h = make_hit(a.dev_velo_geometry, key >> 18, key & 0x3FFu, (key >> 10) & 0xFFu,
-                               __uint_as_float(seeds[2 * kWave + t]), __uint_as_float(seeds[4 * kWave + t]));
-        const uint32_t slot = mstart + pos + t;
-        if (slot < mend) {
-          xs[slot] = h.x;
-          ys[slot] = h.y;
-          zs[slot] = h.z;
-          ids[slot] = h.id;
-        }
-      }
-      pos += tot;
-      jb += taken;
-      // the entry list is rewritten by the next pass
-      wave_sync();
     }
-    if (lane == 0) a.dev_module_cluster_num[e * VELO_N_MODULES + m] = pos;
-    if (pos > VELO_MAX_NUMHITS_IN_MODULE) status |= VELO_STATUS_MODULE_HITS_OVERFLOW;
-    event_hits += pos;
-  }
-  if (a.dev_velo_event_status) {
-    if (lane == 0) atomicAdd(&block_words[1], event_hits);
     __syncthreads();
-    if (threadIdx.x == 0 && block_words[1] >= VELO_MAX_HITS_PER_EVENT) status |= VELO_STATUS_EVENT_HITS_OVERFLOW;
-    const uint32_t s = block_or(status, &block_words[0]);
-    if (threadIdx.x == 0 && s) atomicOr(a.dev_velo_event_status + e, s);
+  }
+
+  uint32_t cptr = cl ? cl_mod[m0] : 0u; // next list entry of the wave
+  const uint32_t cend = cl ? cl_mod[m1] : 0u;
+  uint32_t cmod = 0xFFFFFFFFu, carry = 0; // the module continuing from the last pass, its clusters so far
+  uint32_t kcur = kfirst;
+  for (uint32_t jb = fbeg; jb < fend;) {
+    const uint32_t f = jb + lane;
+    uint32_t knext;
+    const FlatSp sp = flat_sp(fb, nnz, jb, kcur, &knext);
+    uint32_t i = 0, b = 0, w = 0, c = 0, idx = 0, mod = 0xFFFFFFFFu;
+    bool live = f < fend;
+    if (live) {
+      const uint32_t d = fb.delta[sp.k];
+      b = fb.bank[sp.k];
+      mod = b / 4u;
+      i = f - sp.s;
+      w = ev[f + d];
+      if (w & 0x80000000u) idx = a.dev_velo_sp_patterns[w & 0xFFu];
+      else if (!cl) {
+        const uint32_t base = sp.s + d, n = sp.e - sp.s;
+        const bool sorted = !((fb.unsorted[b / 32] >> (b % 32)) & 1u);
+        c = candidates_from_pixels(sorted ? neighbour_pixels_sorted(w, i, ev + base, n)
+                                          : neighbour_pixels(w, ev + base, n));
+      }
+    }
+    // per SP: first entry and number of entries; per entry: its SP's lane,
+    // SP index, bank, pixel
+    uint32_t first = 0, n_own = 0, taken, n_ent;
+    uint32_t src = 0, si = 0, sb = 0, kp = 0;
+    if (!cl) {
+      uint32_t has_lo = (c & 0x0Fu) ? 1u : 0u;
+      uint32_t has_hi = (c & 0xF0u) ? 1u : 0u;
+      first = wave_prefix_small(has_lo + has_hi, &n_ent);
+      const uint32_t end = first + has_lo + has_hi;
+      if (end > kWave) {
+        live = false;
+        has_lo = has_hi = 0;
+      }
+      n_own = has_lo + has_hi;
+      taken = (uint32_t) __popcll(ballot(live));
+      n_ent = n_ent > kWave ? __shfl(end, taken - 1u, 64) : n_ent;
+      if (has_lo) entries[first] = lane;
+      if (has_hi) entries[first + has_lo] = lane | 64u;
+      wave_sync();
+      const uint32_t dd = lane < n_ent ? entries[lane] : 0u;
+      src = dd & 63u;
+      const uint32_t sc = __shfl(c, src, 64);
+      si = __shfl(i, src, 64);
+      sb = __shfl(b, src, 64);
+      kp = (dd & 64u) ? c_candidate_ks[sc >> 4] + 2u : c_candidate_ks[sc & 0x0Fu];
+    }
+    else {
+      // entries of SPs [jb, jb + 64) from the list; when 64 entries do not
+      // reach past the pass, its SPs end before the SP of entry 64
+      const bool ok = cptr + lane < cend;
+      const uint32_t ent = ok ? cl[cptr + lane] : 0u;
+      si = ent >> 11;
+      sb = (ent >> 3) & 0xFFu;
+      kp = ent & 7u;
+      const uint32_t epos = ok && sb < VELO_N_SENSORS ? fb.bstart[sb] + si : 0xFFFFFFFFu;
+      uint32_t limit = jb + kWave;
+      if (cptr + kWave < cend) {
+        const uint32_t e64 = cl[cptr + kWave];
+        const uint32_t b64 = (e64 >> 3) & 0xFFu;
+        const uint32_t p64 = b64 < VELO_N_SENSORS ? fb.bstart[b64] + (e64 >> 11) : 0xFFFFFFFFu;
+        limit = p64 < limit ? p64 : limit;
+      }
+      limit = limit < fend ? limit : fend;
+      limit = limit > jb ? limit : jb + 1; // a list that does not match the banks still ends
+      taken = limit - jb;
+      live = lane < taken;
+      const bool mine = epos >= jb && epos < limit;
+      n_ent = (uint32_t) __popcll(ballot(mine));
+      src = mine ? epos - jb : 0u;
+      // the SP of each entry records its first entry and count
+      entries[lane] = 0u;
+      wave_sync();
+      const uint32_t prev = lane_below(src);
+      const bool lead = mine && (lane == 0 || prev != src);
+      const uint32_t next = lane_above(src);
+      const bool pair = lead && lane + 1 < n_ent && next == src;
+      if (lead) entries[src] = (lane + 1) | (pair ? 0x100u : 0u);
+      wave_sync();
+      if (live) {
+        const uint32_t info = entries[lane];
+        first = (info & 0xFFu) - 1u;
+        n_own = info ? 1u + (info >> 8) : 0u;
+      }
+      cptr += n_ent;
+    }
+    if (!live) mod = 0xFFFFFFFFu;
+    const bool iso = live && (w & 0x80000000u);
+
+    // growth: lane t takes entry t
+    HitSeed r {};
+    uint64_t acc = 0;
+    if (n_ent) {
+      bool good = false;
+      if (lane < n_ent) {
+        const uint32_t base = fb.bbase[sb], n = fb.bn[sb];
+        // the reference decodes the candidate's SP index as uint8_t (:179)
+        const uint32_t cw = ev[base + (si & 0xFFu)];
+        const uint32_t caddr = (cw & 0x007FFF00u) >> 8;
+        const int32_t sp_row = (int32_t) (caddr & 0x3Fu), sp_col = (int32_t) (caddr >> 6);
+        uint32_t pa[3];
+        if (!((fb.unsorted[sb / 32] >> (sb % 32)) & 1u)) load_window_sorted(sp_row, sp_col, ev + base, n, si & 0xFFu, pa);
+        else load_window(sp_row, sp_col, ev + base, n, pa);
+        good = cluster_candidate(pa[0], pa[1], pa[2], sp_row, sp_col, kp, sb, &r);
+      }
+      acc = ballot(good);
+    }
+
+    // clusters per SP; their rank in the pass and slot in the module
+    const uint32_t own_ok = n_own ? (uint32_t) __popcll((acc >> first) & (n_own > 1 ? 3u : 1u)) : 0u;
+    const uint32_t cnt = iso ? ((idx & 8u) ? 2u : 1u) : own_ok;
+    uint32_t pass_clusters;
+    const uint32_t rel = wave_prefix_small(cnt, &pass_clusters);
+    const uint32_t prev_mod = lane_below(mod);
+    const bool seg_start = lane == 0 || prev_mod != mod;
+    const uint32_t seg_base = wave_inclusive_max(seg_start ? rel : 0u);
+    const uint32_t in_mod = rel - seg_base + (mod == cmod ? carry : 0u);
+    uint32_t slot0 = 0xFFFFFFFFu, slot_end = 0;
+    if (live) {
+      slot0 = fb.mst[mod] + in_mod; // the module's estimated slots end at mst[mod + 1]
+      slot_end = fb.mst[mod + 1];
+    }
+    auto store = [&](uint32_t key, float fx, float fy, uint32_t slot) {
+      const Hit h = make_hit(a.dev_velo_geometry, key >> 18, key & 0x3FFu, (key >> 10) & 0xFFu, fx, fy);
+      xs[slot] = h.x;
+      ys[slot] = h.y;
+      zs[slot] = h.z;
+      ids[slot] = h.id;
+    };
+    if (iso) {
+      // isolated SP: pre-computed pattern (MaskedVeloClustering.cu:105-167)
+      const uint32_t sp_addr = (w & 0x007FFF00u) >> 8, sp = w & 0xFFu;
+      const uint32_t sp_row = sp_addr & 0x3Fu, sp_col = sp_addr >> 6, sensor = b << 18;
+      if (slot0 < slot_end)
+        store((sp_col * 2u + ((idx >> 2) & 1u)) | ((sp_row * 4u + (idx & 3u)) << 10) | sensor,
+              a.dev_velo_sp_fx[2 * sp], a.dev_velo_sp_fy[2 * sp], slot0);
+      if ((idx & 8u) && slot0 + 1 < slot_end)
+        store((sp_col * 2u + ((idx >> 6) & 1u)) | ((sp_row * 4u + ((idx >> 4) & 3u)) << 10) | sensor,
+              a.dev_velo_sp_fx[2 * sp + 1], a.dev_velo_sp_fy[2 * sp + 1], slot0 + 1);
+    }
+    if (n_ent) {
+      // an SP's second cluster follows its first when that one was kept
+      const uint32_t room = live && slot0 < slot_end ? min(slot_end - slot0, 3u) : 0u;
+      const uint32_t packed = __shfl(first | (room << 8), src, 64);
+      const uint32_t s0 = __shfl(slot0, src, 64);
+      const uint32_t f1 = packed & 0xFFu;
+      const uint32_t after = lane > f1 ? (uint32_t) (acc >> f1) & 1u : 0u;
+      if (((acc >> lane) & 1u) && after < (packed >> 8)) store(r.key, r.fx, r.fy, s0 + after);
+    }
+    // cluster counts per module: the last lane of each module's run adds
+    // the run's clusters
+    const uint32_t next_mod = lane_above(mod);
+    if (mod != 0xFFFFFFFFu && (lane == kWave - 1 || next_mod != mod)) atomicAdd(&fb.est[mod], rel + cnt - seg_base);
+    cmod = __builtin_amdgcn_readlane(mod, taken - 1u);
+    carry = __builtin_amdgcn_readlane(in_mod + cnt, taken - 1u);
+    kcur = taken < kWave ? __builtin_amdgcn_readlane(sp.k, taken) : knext;
+    jb += taken;
+    // the entry list is rewritten by the next pass
+    wave_sync();
+  }
+  __syncthreads();
+  if (threadIdx.x < kWave) {
+    const uint32_t n = lane < VELO_N_MODULES ? fb.est[lane] : 0u;
+    if (lane < VELO_N_MODULES) a.dev_module_cluster_num[e * VELO_N_MODULES + lane] = n;
+    const bool module_overflow = ballot(n > VELO_MAX_NUMHITS_IN_MODULE) != 0;
+    const uint32_t event_hits = wave_sum(n);
+    uint32_t status = module_overflow ? VELO_STATUS_MODULE_HITS_OVERFLOW : 0u;
+    if (event_hits >= VELO_MAX_HITS_PER_EVENT) status |= VELO_STATUS_EVENT_HITS_OVERFLOW;
+    if (lane == 0 && status && a.dev_velo_event_status) atomicOr(a.dev_velo_event_status + e, status);
   }
 }
 
-// At most 80 VGPRs: six waves per SIMD (the kernel is VALU-bound and needs
-// the waves to cover LDS latency).
+// At most 80 VGPRs: six waves per SIMD.  The kernel is bound by how many
+// events a CU holds at once (LDS: the event tile and each wave's entry list)
+// and by the latency of each pass, so it keeps no per-wave cluster staging
+// and every lane writes its own clusters.
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
 masked_clustering_kernel(velo_masked_clustering_args a)
 {
   extern __shared__ uint32_t lds[];
-  __shared__ uint32_t block_words[2];
-  __shared__ BankTable banks;
+  __shared__ FlatBanks fb;
   for_each_event(a.tuning, a.n_events, [&](const uint32_t e) {
     const uint32_t ei = event_index(a.dev_event_list, e);
     const uint32_t off = a.dev_raw_input_offsets[ei];
@@ -1015,9 +963,17 @@ masked_clustering_kernel(velo_masked_clustering_args a)
     const uint32_t* gev = reinterpret_cast<const uint32_t*>(a.dev_raw_input + off);
     // each wave's candidate entry list follows the event tile
     uint32_t* entries = lds + a.tuning.lds_tile + (threadIdx.x / kWave) * kClusterEntries;
-    // then each wave's staged clusters of one pass (3 x 128 words)
-    uint32_t* seeds = lds + a.tuning.lds_tile + (blockDim.x / kWave) * kClusterEntries + (threadIdx.x / kWave) * kClusterSeeds;
-    if (threadIdx.x < 2) block_words[threadIdx.x] = 0;
+    uint32_t* zero = fb.unsorted;
+    for (uint32_t i = threadIdx.x; i < kFlatZeroWords; i += blockDim.x) zero[i] = 0;
+    for (uint32_t m = threadIdx.x; m <= VELO_N_MODULES; m += blockDim.x)
+      fb.mst[m] = a.dev_module_cluster_start[e * VELO_N_MODULES + m];
+    // The bank-order check is skipped only where estimate_input_size checked
+    // the event's banks (BANK_ORDER_CHECKED) and found them sorted (no
+    // UNSORTED_BANK); any other status word keeps the check.
+    const bool check_order =
+      !a.dev_velo_event_status ||
+      (a.dev_velo_event_status[e] & (VELO_STATUS_BANK_ORDER_CHECKED | VELO_STATUS_UNSORTED_BANK)) !=
+        VELO_STATUS_BANK_ORDER_CHECKED;
     // the event's candidate list follows the event in LDS when it is given,
     // complete (below the list's capacity) and fits
     const bool list = a.dev_cluster_candidates && a.dev_event_candidate_num;
@@ -1033,25 +989,28 @@ masked_clustering_kernel(velo_masked_clustering_args a)
         stage_words(cl, a.dev_cluster_candidates + (size_t) e * VELO_MAX_CANDIDATES_EVENT, ncand);
       }
       __syncthreads();
-      build_bank_table(static_cast<const uint32_t*>(lds), nwords, banks);
-      if (cl && threadIdx.x <= VELO_N_MODULES) {
-        // first entry of module threadIdx.x (bank field bits 3-10, module = bank / 4)
-        uint32_t lo = 0, hi = ncand;
-        while (lo < hi) {
-          const uint32_t mid = (lo + hi) >> 1;
-          if (((cl[mid] >> 5) & 0x3Fu) < threadIdx.x) lo = mid + 1;
-          else hi = mid;
+      if (threadIdx.x < kWave) build_flat_banks(static_cast<const uint32_t*>(lds), nwords, fb);
+      // the other waves (or wave 0 alone) find each module's first entry
+      const uint32_t helper = blockDim.x > kWave ? kWave : 0u;
+      if (cl && threadIdx.x >= helper) {
+        for (uint32_t m = threadIdx.x - helper; m <= VELO_N_MODULES; m += blockDim.x - helper) {
+          // bank field bits 3-10, module = bank / 4
+          uint32_t lo = 0, hi = ncand;
+          while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (((cl[mid] >> 5) & 0x3Fu) < m) lo = mid + 1;
+            else hi = mid;
+          }
+          cl_mod[m] = lo;
         }
-        cl_mod[threadIdx.x] = lo;
       }
       __syncthreads();
-      cluster_event(static_cast<const uint32_t*>(lds), banks, a, e, block_words, entries, seeds, cl, cl_mod);
+      cluster_event(static_cast<const uint32_t*>(lds), fb, a, e, entries, cl, cl_mod, check_order);
     }
     else {
+      if (threadIdx.x < kWave) build_flat_banks(gev, nwords, fb);
       __syncthreads();
-      build_bank_table(gev, nwords, banks);
-      __syncthreads();
-      cluster_event(gev, banks, a, e, block_words, entries, seeds, nullptr, nullptr);
+      cluster_event(gev, fb, a, e, entries, nullptr, nullptr, check_order);
     }
   });
 }
@@ -1224,7 +1183,7 @@ int velo_masked_clustering_launch(const velo_masked_clustering_args* in, void* s
   if (!block) return (int) hipErrorInvalidValue;
   if (!a.tuning.lds_tile) a.tuning.lds_tile = kDefaultClusterWords;
   const size_t shmem =
-    ((size_t) a.tuning.lds_tile + (block / kWave) * (kClusterEntries + kClusterSeeds)) * sizeof(uint32_t);
+    ((size_t) a.tuning.lds_tile + (block / kWave) * kClusterEntries) * sizeof(uint32_t);
   hipLaunchKernelGGL(masked_clustering_kernel, dim3(event_grid(a.tuning, a.n_events)), dim3(block), shmem, (hipStream_t) stream, a);
   return (int) hipGetLastError();
 }

@@ -475,6 +475,31 @@ def test_null_argument_is_rejected(stages):
         nat.launch("velo_search_by_triplet_launch", nat.SearchByTripletArgs())
 
 
+def test_huge_bank_is_skipped(stages, consts, geometry):
+    """A bank of 2^22 SPs (inside its 16 MB event) is flagged BAD_BANK and
+    skipped as the oracle does; the other banks decode as usual."""
+    from tests import rawevent
+    events, offsets = rawevent.encode([(5, 10, 10), (9, 20, 20), (9, 21, 21)])
+    w = events.view(np.uint32)
+    h = int(w[0]) + 2 + int(w[1 + 5]) // 4
+    big = np.zeros(len(w) + (1 << 22), np.uint32)
+    big[: len(w)] = w
+    big[h + 1] = 1 << 22
+    ev, off = big.view(np.uint8), np.array([0, 4 * len(big)], np.uint32)
+    oest, _, ocnum, ocand, ost = po.estimate_input_size(ev, off)
+    est, _, cnum, cand, st = stages.estimate(ev, off, consts)
+    assert ost[0] & 0x100 and st[0] == ost[0]
+    np.testing.assert_array_equal(est[:N_MODULES], oest[:N_MODULES])
+    assert cnum[0] == ocnum[0]
+    np.testing.assert_array_equal(cand[: ocnum[0]], ocand[: ocnum[0]])
+    starts = po.prefix_sum(oest)
+    onums, ocont, _ = po.masked_clustering(ev, off, starts, geometry)
+    nums, cont, _ = stages.clustering(ev, off, starts, consts, (256, 0), None)
+    np.testing.assert_array_equal(nums[:N_MODULES], onums)
+    total = int(starts[N_MODULES])
+    np.testing.assert_array_equal(cont[: 4 * total], ocont[: 4 * total])
+
+
 @pytest.mark.parametrize("tile", [0, 32])
 def test_damaged_banks(stages, consts, minbias_200, geometry, tile):
     """Banks whose offset, header or SP words reach past their event are

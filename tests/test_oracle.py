@@ -260,6 +260,26 @@ def test_bad_bank_flag():
     assert st[0] & po.STATUS_BITS["BAD_BANK"]
 
 
+def test_huge_bank_is_flagged_and_skipped():
+    """A bank of 2^22 SPs or more (16 MB) is flagged BAD_BANK and skipped,
+    even when its words lie inside the event: the device decoder sums the SP
+    counts of an event's banks in 32 bits."""
+    events, offsets = rawevent.encode([(5, 10, 10), (9, 20, 20)])
+    w = events.view(np.uint32)
+    nb = int(w[0])
+    h = nb + 2 + int(w[1 + 5]) // 4
+    assert w[h] == 5 and w[h + 1] == 1
+    big = np.zeros(len(w) + (1 << 22), np.uint32)
+    big[: len(w)] = w
+    big[h + 1] = 1 << 22
+    est, _, _, _, st = po.estimate_input_size(big.view(np.uint8), np.array([0, 4 * len(big)], np.uint32))
+    assert st[0] & po.STATUS_BITS["BAD_BANK"]
+    assert est[1] == 0 and est[2] == 1  # sensor 5 (module 1) skipped, sensor 9 (module 2) kept
+    big[h + 1] = 1
+    est, _, _, _, st = po.estimate_input_size(big.view(np.uint8), np.array([0, 4 * len(big)], np.uint32))
+    assert not st[0] & po.STATUS_BITS["BAD_BANK"] and est[1] == 1
+
+
 # --------------------------------------------------------------------------
 # a7/a8: sort and windows
 # --------------------------------------------------------------------------
