@@ -45,8 +45,8 @@ BATCHED_GRIDS = (0, 1024, 2048, 4096, 8192)
 # autotuner's result on MI355X for 20 000 uneven events
 # (cuda_hlt_amd/tuning/mi355x_uneven_batched.json)
 BATCHED_DEFAULT = {
-    "estimate_input_size": (512, 12288, 0),
-    "masked_clustering": (512, 8192, 0),
+    "estimate_input_size": (256, 6144, 0),
+    "masked_clustering": (256, 4096, 0),
     "calculate_phi_and_sort": (256, 1024, 0),
     "fill_candidates": (256, 4096, 0),
     "search_by_triplet": (64, 256, 4096),
