@@ -22,7 +22,7 @@ import numpy as np
 SPACE = {
     "estimate_input_size": [(b, t) for b in (128, 256, 512, 1024) for t in (2048, 3072, 4096, 6144, 8192)],
     "masked_clustering": [(b, t) for b in (128, 256, 512, 1024) for t in (2048, 3072, 4096, 6144, 8192)],
-    "calculate_phi_and_sort": [(b, t) for b in (64, 128, 256, 512) for t in (1, 1024, 2048, 3072)],
+    "calculate_phi_and_sort": [(b, 0) for b in (128, 256, 512, 1024)],
     "fill_candidates": [(b, t) for b in (64, 128, 256, 512) for t in (1024, 2048, 4096)],
     "search_by_triplet": [(64, t) for t in (1, 256, 512, 768, 1024, 2048)],
     "consolidate": [(b, t) for b in (64, 128, 256, 512) for t in (1, 1024, 2048, 3072)],
@@ -34,7 +34,7 @@ BATCHED_BLOCKS = (64, 128, 256, 512)
 BATCHED_TILES = {
     "estimate_input_size": (4096, 6144, 8192, 12288),
     "masked_clustering": (4096, 6144, 8192, 12288),
-    "calculate_phi_and_sort": (1, 1024, 2048, 3072),
+    "calculate_phi_and_sort": (0,),
     "fill_candidates": (1024, 2048, 4096, 8192),
     "search_by_triplet": (256, 512, 1024, 2048, 4096),
     "consolidate": (256, 1024, 2048, 3072),
@@ -47,7 +47,7 @@ BATCHED_GRIDS = (0, 1024, 2048, 4096, 8192)
 BATCHED_DEFAULT = {
     "estimate_input_size": (256, 6144, 0),
     "masked_clustering": (256, 4096, 0),
-    "calculate_phi_and_sort": (256, 1024, 0),
+    "calculate_phi_and_sort": (256, 0, 0),
     "fill_candidates": (256, 4096, 0),
     "search_by_triplet": (64, 256, 4096),
     "consolidate": (512, 3072, 0),

@@ -1,26 +1,19 @@
 // calculate_phi_and_sort (CalculatePhiAndSort.cu:6-52) and fill_candidates
 // (FillCandidates.cu:8-97) for gfx950.
 //
-// One workgroup per event, wave w owns modules w, w+W, ...  Phi sort: when
-// the event fits (tuning.lds_tile hits, default 1024) the workgroup first
-// copies its x, y, z, id slots to LDS in one coalesced pass; the wave then
-// computes the module's phis into its LDS slice, ranks every hit against
-// all others (stable: ties broken by slot, CalculatePhi.cu:47-53) and
-// writes x, y, z, id from LDS to their sorted positions.  Larger events
-// gather by the permutation from global memory into registers instead, and
-// modules larger than the register budget (512 hits; the reference caps a
-// module at 500) fall back to the reference's slot-by-slot permutation
-// through global memory.
+// One workgroup per event.  Phi sort takes the event's cluster slots as one
+// flat list (a few per thread, whatever module they belong to): each
+// thread keeps its hits' x, y, z, id in registers and their phi keys in
+// LDS, then, after one workgroup barrier, ranks each hit among its module's
+// keys (stable: ties broken by slot, CalculatePhi.cu:47-53) and writes it
+// to its sorted slot.  fill_candidates takes the hits of modules 2-49 as
+// one range, 64 per pass.
 #include "velo_device.h"
 
 namespace velo {
 
-constexpr uint32_t kSortCap = 512;
-constexpr uint32_t kSortChunks = kSortCap / kWave;
-constexpr uint32_t kSortBlock = 256;
+constexpr uint32_t kSortBlock = 512;
 constexpr uint32_t kHoleBits = 0x7FC00000u; // quiet NaN
-constexpr uint32_t kDefaultSortStage = 1024;  // hits of an event staged in LDS
-constexpr size_t kMaxSortLds = 65536;
 
 __device__ inline float module_phi(uint32_t m, float x, float y) { return (m & 1u) ? phi_odd(x, y) : phi_even(x, y); }
 
@@ -33,61 +26,64 @@ __device__ inline uint32_t phi_key(float p)
 }
 __device__ inline uint32_t key_phi_bits(uint32_t k) { return (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k; }
 
-// Sorted position of each of a module's n hits, by phi with ties in slot
-// order (CalculatePhi.cu:47-53): perm[pos] = i.  keys[i] is phi_key of hit
-// i.  Counting the strictly smaller keys gives every hit its position when
-// no two phis are equal; equal phis share a count, which leaves a position
-// unwritten, and the module is then ranked again with the slot tie-break.
-__device__ inline uint32_t keys_below(uint4 q, uint32_t k)
+// Flat sort: the slots of a group of whole modules are taken as one list,
+// kSortRounds per thread.  Pass 1 reads each filled slot's x, y, z, id into
+// registers and its phi key into LDS (and fills the holes); after a
+// workgroup barrier, pass 2 ranks each hit among its module's keys (ties in
+// slot order) and writes it to its sorted slot.  An event whose slots
+// exceed kSortRounds x workgroup size is taken in several module groups; a
+// single module larger than that is sorted through global memory.
+constexpr uint32_t kSortRounds = 2;
+
+// Largest m in [0, 51] with mst[m] <= i (the module holding slot i).
+__device__ inline uint32_t module_of_slot(const uint32_t* mst, uint32_t i)
 {
-  return (uint32_t) (q.x < k) + (uint32_t) (q.y < k) + (uint32_t) (q.z < k) + (uint32_t) (q.w < k);
+  uint32_t lo = 0, hi = VELO_N_MODULES - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) / 2;
+    if (mst[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
 }
 
-__device__ inline void rank_module(const uint32_t* keys, uint16_t* perm, uint32_t n_any, uint32_t lane)
+// Module m of more than kSortRounds x workgroup size slots, by wave 0,
+// through global memory (phis staged in slot 5, which receives X last).
+__device__ inline void sort_module_global(
+  uint32_t m, uint32_t s, uint32_t n, uint32_t* X, uint32_t* Y, uint32_t* Z, uint32_t* ID, uint32_t* PHI, uint32_t* TMP,
+  uint32_t* perm)
 {
-  const uint32_t n = __builtin_amdgcn_readfirstlane(n_any); // the same on every lane
-  const uint4* k4 = reinterpret_cast<const uint4*>(keys);
-  for (uint32_t p = lane; p < n; p += kWave) perm[p] = 0xFFFFu;
+  const uint32_t lane = lane_id();
+  for (uint32_t i = lane; i < n; i += kWave)
+    TMP[s + i] = __float_as_uint(module_phi(m, __uint_as_float(X[s + i]), __uint_as_float(Y[s + i])));
   wave_sync();
-#pragma unroll 1
-  for (uint32_t i0 = 0; i0 < n; i0 += kWave) {
-    const uint32_t i = i0 + lane;
-    const uint32_t ki = i < n ? keys[i] : 0u;
-    uint32_t r = 0, j = 0;
-    // four 16-byte reads in flight per step
-    for (; j + 16 <= n; j += 16) {
-      const uint4 q0 = k4[j / 4], q1 = k4[j / 4 + 1], q2 = k4[j / 4 + 2], q3 = k4[j / 4 + 3];
-      r += keys_below(q0, ki) + keys_below(q1, ki) + keys_below(q2, ki) + keys_below(q3, ki);
-    }
-    for (; j + 4 <= n; j += 4) r += keys_below(k4[j / 4], ki);
-    for (; j < n; ++j) r += (uint32_t) (keys[j] < ki);
-    if (i < n) perm[r] = (uint16_t) i;
-  }
-  wave_sync();
-  bool clash = false;
-  for (uint32_t p = lane; p < n; p += kWave) clash |= perm[p] == 0xFFFFu;
-  if (ballot(clash) == 0) return;
   for (uint32_t i = lane; i < n; i += kWave) {
-    const uint32_t ki = keys[i];
-    uint32_t r = 0;
+    const float p = __uint_as_float(TMP[s + i]);
+    uint32_t pos = 0;
     for (uint32_t j = 0; j < n; ++j) {
-      const uint32_t kj = keys[j];
-      r += (kj < ki) || (kj == ki && j < i);
+      const float q = __uint_as_float(TMP[s + j]);
+      pos += (p > q) || (p == q && i > j);
     }
-    perm[r] = (uint16_t) i;
+    perm[s + pos] = s + i;
+    PHI[s + pos] = __float_as_uint(p);
   }
   wave_sync();
+  // SortByPhi.cu:21-33 order: X->5, Y->0, Z->1, ID->2
+  for (uint32_t p = lane; p < n; p += kWave) TMP[s + p] = X[perm[s + p]];
+  wave_sync();
+  for (uint32_t p = lane; p < n; p += kWave) X[s + p] = Y[perm[s + p]];
+  wave_sync();
+  for (uint32_t p = lane; p < n; p += kWave) Y[s + p] = Z[perm[s + p]];
+  wave_sync();
+  for (uint32_t p = lane; p < n; p += kWave) Z[s + p] = ID[perm[s + p]];
 }
 
-// At most 96 VGPRs: five waves per SIMD, as many as the LDS of a
-// 256-thread workgroup allows.
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5))) calculate_phi_and_sort_kernel(velo_calculate_phi_and_sort_args a)
+__global__ void __launch_bounds__(1024) calculate_phi_and_sort_kernel(velo_calculate_phi_and_sort_args a)
 {
-  extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
+  extern __shared__ __attribute__((aligned(16))) uint32_t keys[]; // kSortRounds x workgroup size
+  __shared__ uint32_t s_mst[VELO_N_MODULES + 1];
+  __shared__ uint32_t s_mnum[VELO_N_MODULES];
   for_each_event(a.tuning, a.n_events, [&](const uint32_t e) {
-    const uint32_t waves = blockDim.x / kWave;
-    const uint32_t wave = threadIdx.x / kWave;
-    const uint32_t lane = lane_id();
     const uint32_t total = a.dev_module_cluster_start[VELO_N_MODULES * a.n_events];
     uint32_t* X = a.dev_velo_cluster_container;
     uint32_t* Y = X + total;
@@ -95,123 +91,94 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5))) 
     uint32_t* ID = Z + total;
     uint32_t* PHI = ID + total;
     uint32_t* TMP = PHI + total;
-    uint32_t* keys = lds + wave * kSortCap;
-    uint16_t* perm = reinterpret_cast<uint16_t*>(lds + waves * kSortCap) + wave * kSortCap;
-    // the event's x, y, z, id (slots 0-3), when they fit
-    uint32_t* ev = lds + waves * kSortCap + waves * kSortCap / 2;
-    const uint32_t cap = a.tuning.lds_tile;
     const uint32_t off = a.dev_module_cluster_start[e * VELO_N_MODULES];
-    const uint32_t ev_hits = a.dev_module_cluster_start[(e + 1) * VELO_N_MODULES] - off;
-    const bool staged = ev_hits <= cap;
-    if (staged) {
-      for (uint32_t i = threadIdx.x; i < ev_hits; i += blockDim.x) {
-        ev[i] = X[off + i];
-        ev[cap + i] = Y[off + i];
-        ev[2 * cap + i] = Z[off + i];
-        ev[3 * cap + i] = ID[off + i];
+    if (threadIdx.x <= VELO_N_MODULES) s_mst[threadIdx.x] = a.dev_module_cluster_start[e * VELO_N_MODULES + threadIdx.x] - off;
+    if (threadIdx.x < VELO_N_MODULES) s_mnum[threadIdx.x] = a.dev_module_cluster_num[e * VELO_N_MODULES + threadIdx.x];
+    __syncthreads();
+    const uint32_t cap = kSortRounds * blockDim.x;
+    const uint32_t lane = lane_id();
+    for (uint32_t m0 = 0; m0 < VELO_N_MODULES;) {
+      // the group: modules [m0, m1) whose slots fit cap
+      const uint32_t lim = s_mst[m0] + cap;
+      const uint32_t m1 =
+        (uint32_t) __popcll(ballot(lane <= VELO_N_MODULES && s_mst[lane <= VELO_N_MODULES ? lane : 0] <= lim)) - 1u;
+      if (m1 == m0) {
+        // one module of more than cap slots (its holes are filled too)
+        if (threadIdx.x < kWave) {
+          const uint32_t s = off + s_mst[m0], n = s_mnum[m0];
+          for (uint32_t p = s + n + lane; p < off + s_mst[m0 + 1]; p += kWave) {
+            X[p] = kHoleBits;
+            Y[p] = kHoleBits;
+            Z[p] = kHoleBits;
+            PHI[p] = kHoleBits;
+            TMP[p] = kHoleBits;
+          }
+          sort_module_global(m0, s, n, X, Y, Z, ID, PHI, TMP, a.dev_hit_permutation);
+        }
+        __syncthreads();
+        ++m0;
+        continue;
+      }
+      const uint32_t base = s_mst[m0], end = s_mst[m1];
+      uint32_t vx[kSortRounds], vy[kSortRounds], vz[kSortRounds], vid[kSortRounds], vm[kSortRounds];
+#pragma unroll
+      for (uint32_t q = 0; q < kSortRounds; ++q) {
+        const uint32_t i = base + q * blockDim.x + threadIdx.x;
+        vm[q] = 0xFFFFFFFFu;
+        if (i < end) {
+          const uint32_t m = module_of_slot(s_mst, i);
+          if (i - s_mst[m] < s_mnum[m]) {
+            vx[q] = X[off + i];
+            vy[q] = Y[off + i];
+            vz[q] = Z[off + i];
+            vid[q] = ID[off + i];
+            keys[i - base] = phi_key(module_phi(m, __uint_as_float(vx[q]), __uint_as_float(vy[q])));
+            vm[q] = m;
+          }
+          else {
+            // Slots estimated but not filled ("holes") read as NaN in every
+            // sorted slot, so no search can ever select them (the reference
+            // leaves them uninitialised).
+            X[off + i] = kHoleBits;
+            Y[off + i] = kHoleBits;
+            Z[off + i] = kHoleBits;
+            PHI[off + i] = kHoleBits;
+            TMP[off + i] = kHoleBits;
+          }
+        }
       }
       __syncthreads();
-    }
-
-    for (uint32_t m = wave; m < VELO_N_MODULES; m += waves) {
-      const uint32_t s = a.dev_module_cluster_start[e * VELO_N_MODULES + m];
-      const uint32_t n = a.dev_module_cluster_num[e * VELO_N_MODULES + m];
-      // Slots estimated but not filled ("holes") read as NaN in every sorted
-      // slot, so no search can ever select them (the reference leaves them
-      // uninitialised).
-      const uint32_t hole_end = a.dev_module_cluster_start[e * VELO_N_MODULES + m + 1];
-      for (uint32_t p = s + n + lane; p < hole_end; p += kWave) {
-        X[p] = kHoleBits;
-        Y[p] = kHoleBits;
-        Z[p] = kHoleBits;
-        PHI[p] = kHoleBits;
-        TMP[p] = kHoleBits;
-      }
-      if (n == 0) continue;
-      if (staged && n <= kSortCap) {
-        const uint32_t r = s - off;
-        for (uint32_t i = lane; i < n; i += kWave)
-          keys[i] = phi_key(module_phi(m, __uint_as_float(ev[r + i]), __uint_as_float(ev[cap + r + i])));
-        wave_sync();
-        rank_module(keys, perm, n, lane);
-        for (uint32_t p = lane; p < n; p += kWave) {
-          const uint32_t i = perm[p];
-          uint32_t bits = key_phi_bits(keys[i]);
-          if (bits == 0u) bits = __float_as_uint(module_phi(m, __uint_as_float(ev[r + i]), __uint_as_float(ev[cap + r + i])));
-          a.dev_hit_permutation[s + p] = s + i;
-          PHI[s + p] = bits;
-        }
-        // SortByPhi.cu:21-33 order: X->5, Y->0, Z->1, ID->2; every read is from LDS
-        for (uint32_t p = lane; p < n; p += kWave) {
-          const uint32_t q = r + perm[p];
-          TMP[s + p] = ev[q];
-          X[s + p] = ev[cap + q];
-          Y[s + p] = ev[2 * cap + q];
-          Z[s + p] = ev[3 * cap + q];
-        }
-      }
-      else if (n <= kSortCap) {
-        // each lane keeps its hits' x, y, z, id and phi in registers (one
-        // coalesced read) and scatters them to their sorted slots
-        uint32_t vx[kSortChunks], vy[kSortChunks], vz[kSortChunks], vid[kSortChunks];
-        float vp[kSortChunks];
-  #pragma unroll
-        for (uint32_t c = 0; c < kSortChunks; ++c) {
-          const uint32_t i = c * kWave + lane;
-          if (i < n) {
-            vx[c] = X[s + i];
-            vy[c] = Y[s + i];
-            vz[c] = Z[s + i];
-            vid[c] = ID[s + i];
-            vp[c] = module_phi(m, __uint_as_float(vx[c]), __uint_as_float(vy[c]));
-            keys[i] = phi_key(vp[c]);
+#pragma unroll
+      for (uint32_t q = 0; q < kSortRounds; ++q) {
+        if (vm[q] != 0xFFFFFFFFu) {
+          const uint32_t i = base + q * blockDim.x + threadIdx.x;
+          const uint32_t m = vm[q];
+          const uint32_t s = s_mst[m] - base, n = s_mnum[m];
+          const uint32_t k = keys[i - base];
+          // rank in the module, ties in slot order (CalculatePhi.cu:47-53)
+          uint32_t below = 0, equal = 0;
+          for (uint32_t j = s; j < s + n; ++j) {
+            const uint32_t kj = keys[j];
+            below += kj < k ? 1u : 0u;
+            equal += kj == k ? 1u : 0u;
           }
-        }
-        wave_sync();
-        rank_module(keys, perm, n, lane);
-        // the keys' slots take the inverse permutation: keys[i] = sorted position of hit i
-        for (uint32_t p = lane; p < n; p += kWave) keys[perm[p]] = p;
-        wave_sync();
-  #pragma unroll
-        for (uint32_t c = 0; c < kSortChunks; ++c) {
-          const uint32_t i = c * kWave + lane;
-          if (i < n) {
-            const uint32_t r = s + keys[i];
-            a.dev_hit_permutation[r] = s + i;
-            PHI[r] = __float_as_uint(vp[c]);
-            TMP[r] = vx[c];
-            X[r] = vy[c];
-            Y[r] = vz[c];
-            Z[r] = vid[c];
+          if (equal > 1) {
+            for (uint32_t j = s; j < i - base; ++j) below += keys[j] == k ? 1u : 0u;
           }
+          const uint32_t r = off + base + s + below;
+          uint32_t bits = key_phi_bits(k);
+          if (bits == 0u) bits = __float_as_uint(module_phi(m, __uint_as_float(vx[q]), __uint_as_float(vy[q])));
+          a.dev_hit_permutation[r] = off + i;
+          PHI[r] = bits;
+          // SortByPhi.cu:21-33 order: X->5, Y->0, Z->1, ID->2
+          TMP[r] = vx[q];
+          X[r] = vy[q];
+          Y[r] = vz[q];
+          Z[r] = vid[q];
         }
       }
-      else {
-        // phis staged in slot 5 (it receives X only at the end)
-        for (uint32_t i = lane; i < n; i += kWave)
-          TMP[s + i] = __float_as_uint(module_phi(m, __uint_as_float(X[s + i]), __uint_as_float(Y[s + i])));
-        wave_sync();
-        for (uint32_t i = lane; i < n; i += kWave) {
-          const float p = __uint_as_float(TMP[s + i]);
-          uint32_t pos = 0;
-          for (uint32_t j = 0; j < n; ++j) {
-            const float q = __uint_as_float(TMP[s + j]);
-            pos += (p > q) || (p == q && i > j);
-          }
-          a.dev_hit_permutation[s + pos] = s + i;
-          PHI[s + pos] = __float_as_uint(p);
-        }
-        wave_sync();
-        // SortByPhi.cu:21-33 order: X->5, Y->0, Z->1, ID->2
-        for (uint32_t p = lane; p < n; p += kWave) TMP[s + p] = X[a.dev_hit_permutation[s + p]];
-        wave_sync();
-        for (uint32_t p = lane; p < n; p += kWave) X[s + p] = Y[a.dev_hit_permutation[s + p]];
-        wave_sync();
-        for (uint32_t p = lane; p < n; p += kWave) Y[s + p] = Z[a.dev_hit_permutation[s + p]];
-        wave_sync();
-        for (uint32_t p = lane; p < n; p += kWave) Z[s + p] = ID[a.dev_hit_permutation[s + p]];
-      }
-      wave_sync();
+      __syncthreads();
+      m0 = m1;
     }
   });
 }
@@ -316,13 +283,8 @@ int velo_calculate_phi_and_sort_launch(const velo_calculate_phi_and_sort_args* a
   if (!schedule_ok(a->tuning)) return (int) hipErrorInvalidValue;
   const uint32_t block = a->tuning.block_dim ? a->tuning.block_dim : kSortBlock;
   if (block % kWave || block > 1024) return (int) hipErrorInvalidValue;
-  velo_calculate_phi_and_sort_args args = *a;
-  const size_t sort_lds = (size_t) (block / kWave) * kSortCap * (sizeof(float) + sizeof(uint16_t));
-  // event staging: as requested, within 64 KB of LDS per workgroup
-  const size_t room = sort_lds < kMaxSortLds ? (kMaxSortLds - sort_lds) / (4 * sizeof(uint32_t)) : 0;
-  const size_t want = args.tuning.lds_tile ? args.tuning.lds_tile : kDefaultSortStage;
-  args.tuning.lds_tile = (uint32_t) (want < room ? want : room);
-  const size_t shmem = sort_lds + (size_t) args.tuning.lds_tile * 4 * sizeof(uint32_t);
+  const velo_calculate_phi_and_sort_args args = *a;
+  const size_t shmem = (size_t) kSortRounds * block * sizeof(uint32_t);
   hipLaunchKernelGGL(calculate_phi_and_sort_kernel, dim3(event_grid(args.tuning, args.n_events)), dim3(block), shmem,
                      (hipStream_t) stream, args);
   return (int) hipGetLastError();

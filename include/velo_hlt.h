@@ -244,7 +244,9 @@ int velo_masked_clustering_launch(const velo_masked_clustering_args* a, void* st
 
 /* ---- K6: calculate_phi_and_sort (CalculatePhiAndSort.cu:6-52) ----
  * Slots after the sort: Y=0, Z=1, ID=2, unsorted ID=3, phi=4, X=5.  Only
- * [start, start+num) of each module is defined; holes are not written. */
+ * [start, start+num) of each module holds hits; its holes (estimated but
+ * unfilled slots) read as NaN in slots 0, 1, 2, 4 and 5 and are left out of
+ * dev_hit_permutation.  tuning.lds_tile is not used. */
 typedef struct {
   const uint32_t* dev_module_cluster_start;
   const uint32_t* dev_module_cluster_num;

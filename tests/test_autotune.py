@@ -52,7 +52,7 @@ def cost_table(best):
 
 def test_tune_picks_the_fastest_shape_then_grid():
     want = {"estimate_input_size": (512, 4096, 2048), "masked_clustering": (256, 6144, 0),
-            "calculate_phi_and_sort": (128, 1024, 8192), "fill_candidates": (512, 2048, 1024),
+            "calculate_phi_and_sort": (128, 0, 8192), "fill_candidates": (512, 2048, 1024),
             "search_by_triplet": (64, 512, 4096), "consolidate": (256, 2048, 0)}
     seq = FakeSequence(cost_table(want))
     best, table = autotune.tune(seq, 100, 0, 0, repeats=1, log=lambda *a: None, outputs=fake_outputs)
@@ -98,7 +98,7 @@ def test_tune_batched_starts_from_defaults_and_checks_the_graph_run():
     want = dict(autotune.BATCHED_DEFAULT)
     want["masked_clustering"] = (512, 8192, 4096)
     want["search_by_triplet"] = (64, 1024, 2048)
-    want["calculate_phi_and_sort"] = (256, 1, 2048)  # the default's tile 0 is not in the sweep
+    want["calculate_phi_and_sort"] = (256, 0, 2048)  # the sort has no tile
     seq = FakeSequence(cost_table(want))
     best, table = autotune.tune_batched(seq, 50, 1, 2, 3, 4, repeats=1, log=lambda *a: None, outputs=fake_outputs)
     assert best == want
