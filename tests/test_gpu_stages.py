@@ -222,8 +222,10 @@ def test_calculate_phi_and_sort(stages, chain):
 
 @pytest.mark.parametrize("block,tile", [(64, 1), (256, 1024), (512, 4096), (1024, 0)])
 def test_calculate_phi_and_sort_launch_shapes(stages, chain_uneven, block, tile):
-    """Events staged in LDS (up to `tile` hits) and events sorted from
-    global memory give the same result."""
+    """Every workgroup size gives the same result: events taken in one or
+    several module groups (two slots per thread each), and at 64 threads
+    modules over 128 slots sorted through global memory.  The tile is not
+    used by the sort."""
     _check_sort(stages, chain_uneven, (block, tile))
 
 
@@ -236,8 +238,7 @@ def test_calculate_phi_and_sort_equal_phis(stages, chain, tile):
     """Hits sharing a phi keep their slot order, and a phi of -0 sorts level
     with +0 and keeps its sign: in every module of the first events, hits 1-3
     copy hit 0's x and y, and two hits sit at y = -0 and y = +0; 65 hits of
-    one module share one phi, so the tie spans two waves of ranks.
-    tile 1 sorts from global memory, tile 0 from LDS."""
+    one module share one phi, so the tie spans two waves of ranks."""
     n = 20
     starts = chain.starts[: n * N_MODULES + 1].copy()
     nums = chain.nums[: n * N_MODULES].copy()
