@@ -22,9 +22,12 @@ Admission is by HBM footprint: a slot's arena is sized once, at start-up,
 for the largest batch it admits (the batched path's bounds for
 slot_events events and slot_bytes raw bytes); the number of slots is the
 HBM budget divided by that footprint (at most max_slots).  Batches are
-padded with empty events to a size bucket so a slot replays one of a few
-captured graphs; empty events produce no hits and no tracks, and every
-event's output is independent of the other events in its batch.
+padded with empty events to a size bucket (64 to 16384 events) so a slot
+replays one of a few graphs, captured for every bucket before the first
+request (Server.warm); empty events produce no hits and no tracks, and
+every event's output is independent of the other events in its batch.
+Large slots keep the host's per-batch work (placing requests, cutting
+results) small against the device time of the batch.
 """
 from __future__ import annotations
 
@@ -41,7 +44,7 @@ import numpy as np
 from . import autotune, gen
 from .stream import VeloSequence
 
-BUCKETS = (64, 128, 256, 512, 1024, 2048, 4096)
+BUCKETS = (64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384)
 
 
 def empty_event() -> np.ndarray:
@@ -234,13 +237,13 @@ class Slot:
         self.host_offsets = torch.zeros(max_events + 1, dtype=torch.int32).pin_memory()
         self.host_order = torch.zeros(max_events, dtype=torch.int32).pin_memory()
 
-    def run(self, requests):
-        """Places the requests (plus empty events up to the bucket) in the
-        slot's device buffers, runs the sequence and returns (per-request
-        results, bucket)."""
+    def run(self, requests, bucket=None):
+        """Places the requests (plus empty events up to the bucket, at least
+        `bucket` when given) in the slot's device buffers, runs the sequence
+        and returns (per-request results, bucket)."""
         import torch
         n_real = sum(r.n for r in requests)
-        bucket = bucket_of(n_real) if self.batched else n_real
+        bucket = max(bucket or 0, bucket_of(n_real)) if self.batched else n_real
         first, offs, pos, e = [], [np.zeros(1, np.int64)], 0, 0
         he = None
         with torch.cuda.stream(self.copy_stream):
@@ -307,7 +310,7 @@ class Slot:
 class Server:
     """Continuous-batching server over K slots (see the module docstring)."""
 
-    def __init__(self, geometry, zs, hbm_budget_mb=131072, max_slots=6, slot_events=4096, slot_bytes=None,
+    def __init__(self, geometry, zs, hbm_budget_mb=131072, max_slots=6, slot_events=16384, slot_bytes=None,
                  batched=True, tuning=None, ghost_model=None, host_results=False):
         slot_bytes = slot_bytes or slot_events * 40000
         if tuning is None and batched:
@@ -328,6 +331,16 @@ class Server:
         self.cv = threading.Condition()
         self.batches: list[BatchRecord] = []
         self.errors: list[BaseException] = []
+
+    def warm(self):
+        """Captures every slot's graph for every bucket it can take (a batch
+        of empty events each), so no request waits for a capture."""
+        if not self.slots[0].batched:
+            return
+        for slot in self.slots:
+            for b in BUCKETS:
+                if b <= self.slot_events and b * EMPTY.nbytes <= self.slot_bytes:
+                    slot.run([], bucket=b)
 
     def fits(self, reqs, r) -> bool:
         n = sum(x.n for x in reqs) + r.n
@@ -451,17 +464,17 @@ def latency_summary(requests, batches, wall):
 
 
 def run_loads(loads=(0.2, 0.5, 0.9), n_requests=300, seed=0, batched=True, ghost_model=None, tuning=None,
-              hbm_budget_mb=131072, max_slots=6, log=None, host_results=False):
+              hbm_budget_mb=131072, max_slots=6, log=None, host_results=False, slot_events=16384):
     """Peak throughput (every request queued at t = 0), then Poisson
     arrivals at the given fractions of it."""
     geometry, zs = gen.geometry(), gen.module_zs()
     srv = Server(geometry, zs, hbm_budget_mb=hbm_budget_mb, max_slots=max_slots, batched=batched, tuning=tuning,
-                 host_results=host_results,
-                 ghost_model=ghost_model)
+                 host_results=host_results, ghost_model=ghost_model, slot_events=slot_events)
     try:
         # one pool of events for every run: requests of U(1, 2000) events
         # need 1000 per request on average (the margin covers > 4 sigma)
         pool = make_pool(int(n_requests * 1000 * 1.2) + 2000, seed=seed + 1)
+        srv.warm()
         warm, _ = make_requests(max(8, srv.n_slots * 2), 0, seed=seed + 7, pool=pool)
         srv.serve(warm)
         srv.batches.clear()
@@ -494,6 +507,7 @@ def main():
     ap.add_argument("--plain", action="store_true", help="plain path (host reads between steps, no filter)")
     ap.add_argument("--ghost-model", default=None, help="trained filter (.npz) for the batched path")
     ap.add_argument("--max-slots", type=int, default=6)
+    ap.add_argument("--slot-events", type=int, default=16384, help="largest batch a slot takes")
     ap.add_argument("--host-results", action="store_true",
                     help="copy every batch's output to the host and cut it there")
     ap.add_argument("--out", default=None)
@@ -504,7 +518,7 @@ def main():
         model = GhostModel.load(a.ghost_model)
     say = lambda *x: print(*x, file=sys.stderr, flush=True)  # noqa: E731
     res = run_loads(a.loads, a.requests, batched=not a.plain, ghost_model=model, log=say,
-                    host_results=a.host_results, max_slots=a.max_slots)
+                    host_results=a.host_results, max_slots=a.max_slots, slot_events=a.slot_events)
     text = json.dumps(res)
     if a.out:
         with open(a.out, "w") as f:

@@ -18,9 +18,10 @@ def test_empty_event_is_a_valid_event_without_hits(geometry, module_zs):
 def test_buckets():
     from cuda_hlt_amd import serving
     assert serving.bucket_of(1) == 64 and serving.bucket_of(64) == 64 and serving.bucket_of(65) == 128
-    assert serving.bucket_of(4096) == 4096
+    assert serving.bucket_of(4096) == 4096 and serving.bucket_of(4097) == 8192
+    assert serving.bucket_of(16384) == 16384
     with pytest.raises(ValueError):
-        serving.bucket_of(4097)
+        serving.bucket_of(16385)
 
 
 def test_cut_results_equal_each_request_alone(minbias_200, geometry, module_zs):
