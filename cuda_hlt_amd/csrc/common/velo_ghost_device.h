@@ -35,7 +35,7 @@ __device__ inline void track_features(
   uint32_t first = 0xFFFFFFFFu, last = 0, pair_mask = 0;
   float sum_r = 0.0f, max_r = 0.0f;
   float phi0 = 0.0f, phi1 = 0.0f, phi2 = 0.0f;
-  for (uint32_t h = 0; h < nh; ++h) {
+  const auto add_hit = [&](uint32_t h) {
     float x, y, z;
     uint32_t id;
     hits(h, &x, &y, &z, &id);
@@ -49,10 +49,28 @@ __device__ inline void track_features(
     const float r = sqrtf(dx * dx + dy * dy);
     sum_r += r;
     max_r = r > max_r ? r : max_r;
-    if (h == 0) phi0 = atan2_poly(y, x);
-    else if (h == 1) phi1 = atan2_poly(y, x);
-    else if (h == 2) phi2 = atan2_poly(y, x);
+  };
+  if (nh <= VELO_MAX_TRACK_SIZE) {
+    // unrolled: a caller's per-hit arrays keep constant indices (registers)
+#pragma clang loop unroll(full)
+    for (uint32_t h = 0; h < VELO_MAX_TRACK_SIZE; ++h) {
+      if (h >= nh) break;
+      add_hit(h);
+    }
   }
+  else {
+    for (uint32_t h = 0; h < nh; ++h) add_hit(h);
+  }
+  // phi of the seed hits, each read again (no per-hit index into locals)
+  const auto hit_phi = [&](uint32_t h) {
+    float x, y, z;
+    uint32_t id;
+    hits(h, &x, &y, &z, &id);
+    return atan2_poly(y, x);
+  };
+  if (nh > 0) phi0 = hit_phi(0);
+  if (nh > 1) phi1 = hit_phi(1);
+  if (nh > 2) phi2 = hit_phi(2);
   if (nh == 0) first = 0;
   // largest pairwise |dphi| (wrapped to [0, pi]) among the seed hits
   const auto wrapped = [](float a, float b) {

@@ -340,7 +340,7 @@ void SequenceVisitor::visit<velo_calculate_phi_and_sort_t>(
   hipStream_t& stream,
   hipEvent_t&)
 {
-  state.set_opts(dim3(host_buffers.host_number_of_selected_events[0]), dim3(512), stream);
+  state.set_opts(dim3(host_buffers.host_number_of_selected_events[0]), dim3(256), stream);
   state.set_arguments(
     arguments.offset<dev_estimated_input_size>(),
     arguments.offset<dev_module_cluster_num>(),
@@ -372,7 +372,7 @@ void SequenceVisitor::visit<velo_fill_candidates_t>(
   hipStream_t& stream,
   hipEvent_t&)
 {
-  state.set_opts(dim3(host_buffers.host_number_of_selected_events[0]), dim3(512), stream);
+  state.set_opts(dim3(host_buffers.host_number_of_selected_events[0]), dim3(256), stream);
   state.set_arguments(
     arguments.offset<dev_velo_cluster_container>(),
     arguments.offset<dev_estimated_input_size>(),

@@ -12,7 +12,7 @@
 
 namespace velo {
 
-constexpr uint32_t kSortBlock = 512;
+constexpr uint32_t kSortBlock = 256;
 constexpr uint32_t kHoleBits = 0x7FC00000u; // quiet NaN
 
 __device__ inline float module_phi(uint32_t m, float x, float y) { return (m & 1u) ? phi_odd(x, y) : phi_even(x, y); }

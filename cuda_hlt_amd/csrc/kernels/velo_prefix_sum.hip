@@ -181,13 +181,13 @@ __device__ __forceinline__ void consolidate_event(
     // the whole row (count and 26 hit slots) is read at once, so the hit
     // loops below wait for one memory round trip instead of one per hit
     uint32_t hh[VELO_MAX_TRACK_SIZE];
-#pragma unroll
+#pragma clang loop unroll(full)
     for (uint32_t h = 0; h < VELO_MAX_TRACK_SIZE; ++h) hh[h] = t[1 + h];
     const uint32_t nh = t[0] < VELO_MAX_TRACK_SIZE ? t[0] : VELO_MAX_TRACK_SIZE;
     const uint32_t ho = a.dev_velo_track_hit_number[toff + i];
     float s0 = 0.0f, sx = 0.0f, sz = 0.0f, sxz = 0.0f, sz2 = 0.0f;
     float u0 = 0.0f, uy = 0.0f, uz = 0.0f, uyz = 0.0f, uz2 = 0.0f;
-#pragma unroll
+#pragma clang loop unroll(full)
     for (uint32_t h = 0; h < VELO_MAX_TRACK_SIZE; ++h) {
       if (h >= nh) break;
       const uint32_t hi = hh[h];
@@ -232,7 +232,7 @@ __device__ __forceinline__ void consolidate_event(
     const float den31 = 1.0f / (m33 * m11 - m31 * m31);
     float ch = 0.0f;
     int ndof = -4;
-#pragma unroll
+#pragma clang loop unroll(full)
     for (uint32_t h = 0; h < VELO_MAX_TRACK_SIZE; ++h) {
       if (h >= nh) break;
       const uint32_t hi = hh[h];
@@ -276,12 +276,8 @@ __device__ __forceinline__ void consolidate_event(
   }
 }
 
-// At most 80 VGPRs: six waves per SIMD, as many as the LDS allows for
-// 512-thread workgroups.  The filtering instance needs no LDS of its own:
-// the network's weights come through scalar loads.
 template<bool Filter>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
-consolidate_kernel(velo_consolidate_velo_tracks_args a)
+__device__ __forceinline__ void consolidate_body(const velo_consolidate_velo_tracks_args& a)
 {
   extern __shared__ uint32_t s_hits[];
   const uint32_t N = a.n_events;
@@ -318,6 +314,27 @@ consolidate_kernel(velo_consolidate_velo_tracks_args a)
       consolidate_event<Filter>(g, a, e, n, toff, total_tracks, total_hits);
     }
   });
+}
+
+// At most 80 VGPRs: six waves per SIMD, as many as the LDS allows for
+// 512-thread workgroups.  The filtering instance needs no LDS of its own
+// (the network's weights come through scalar loads) and more registers for
+// the network: at most 128 VGPRs.
+template<bool Filter>
+__global__ void consolidate_kernel(velo_consolidate_velo_tracks_args a);
+
+template<>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
+consolidate_kernel<false>(velo_consolidate_velo_tracks_args a)
+{
+  consolidate_body<false>(a);
+}
+
+template<>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4)))
+consolidate_kernel<true>(velo_consolidate_velo_tracks_args a)
+{
+  consolidate_body<true>(a);
 }
 
 } // namespace velo

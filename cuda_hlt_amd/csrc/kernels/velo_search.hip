@@ -31,15 +31,17 @@ constexpr uint32_t kUsedBits = 8192;
 constexpr uint32_t kUsedWords = kUsedBits / 32;
 constexpr uint32_t kTtfTrackletFlag = 0x80000000u;
 
+// The small accessors below are forced inline: left to the inliner they
+// were called out of line from the kernel, with the set kept in scratch.
 struct UsedSet {
   uint32_t* bits;
   uint8_t* bytes;
   bool in_lds;
-  __device__ bool test(int h) const
+  __device__ __forceinline__ bool test(int h) const
   {
     return in_lds ? ((bits[(uint32_t) h >> 5] >> ((uint32_t) h & 31u)) & 1u) != 0u : bytes[h] != 0;
   }
-  __device__ void set(uint32_t h) const
+  __device__ __forceinline__ void set(uint32_t h) const
   {
     if (in_lds) atomicOr(&bits[h >> 5], 1u << (h & 31u));
     else bytes[h] = 1;
@@ -333,8 +335,8 @@ struct Ring {
   uint32_t* base; // 2 words per entry
   uint2* cur;     // LDS: entries [0, kTtfLds) of the step being forwarded
   uint2* next;    // LDS: entries [0, kTtfLds) pushed for the next step
-  __device__ uint2 entry(uint32_t begin, uint32_t idx) const { return idx < kTtfLds ? cur[idx] : get(begin + idx); }
-  __device__ void push(uint32_t begin, uint32_t slot, uint32_t w0, uint32_t w1) const
+  __device__ __forceinline__ uint2 entry(uint32_t begin, uint32_t idx) const { return idx < kTtfLds ? cur[idx] : get(begin + idx); }
+  __device__ __forceinline__ void push(uint32_t begin, uint32_t slot, uint32_t w0, uint32_t w1) const
   {
     if (slot < kTtfLds) next[slot] = make_uint2(w0, w1);
     else put(begin + slot, w0, w1);
@@ -345,12 +347,12 @@ struct Ring {
     cur = next;
     next = t;
   }
-  __device__ uint2 get(uint32_t pos) const
+  __device__ __forceinline__ uint2 get(uint32_t pos) const
   {
     const uint32_t p = pos % VELO_TTF_MODULO;
     return make_uint2(base[2 * p], base[2 * p + 1]);
   }
-  __device__ void put(uint32_t pos, uint32_t w0, uint32_t w1) const
+  __device__ __forceinline__ void put(uint32_t pos, uint32_t w0, uint32_t w1) const
   {
     const uint32_t p = pos % VELO_TTF_MODULO;
     base[2 * p] = w0;
