@@ -20,7 +20,7 @@ from pathlib import Path
 import numpy as np
 
 SPACE = {
-    "estimate_input_size": [(b, t) for b in (128, 256, 512, 1024) for t in (2048, 3072, 4096, 6144, 8192)],
+    "estimate_input_size": [(b, t) for b in (128, 256, 512, 1024) for t in (2048, 3072, 4096, 5632, 6144, 8192)],
     "masked_clustering": [(b, t) for b in (128, 256, 512, 1024) for t in (2048, 3072, 4096, 6144, 8192)],
     "calculate_phi_and_sort": [(b, 0) for b in (128, 256, 512, 1024)],
     "fill_candidates": [(b, t) for b in (64, 128, 256, 512) for t in (1024, 2048, 4096)],
@@ -32,7 +32,7 @@ SPACE = {
 # order.  The search kernel is one wave per event.
 BATCHED_BLOCKS = (64, 128, 256, 512)
 BATCHED_TILES = {
-    "estimate_input_size": (4096, 6144, 8192, 12288),
+    "estimate_input_size": (4096, 5632, 6144, 8192, 12288),
     "masked_clustering": (4096, 6144, 8192, 12288),
     "calculate_phi_and_sort": (0,),
     "fill_candidates": (1024, 2048, 4096, 8192),
@@ -45,7 +45,7 @@ BATCHED_GRIDS = (0, 1024, 2048, 4096, 8192)
 # autotuner's result on MI355X for 20 000 uneven events
 # (cuda_hlt_amd/tuning/mi355x_uneven_batched.json)
 BATCHED_DEFAULT = {
-    "estimate_input_size": (256, 6144, 0),
+    "estimate_input_size": (256, 5632, 0),
     "masked_clustering": (256, 4096, 0),
     "calculate_phi_and_sort": (256, 0, 0),
     "fill_candidates": (256, 4096, 0),

@@ -354,7 +354,7 @@ __global__ void __launch_bounds__(kOrderBlock)
 // The raw event is copied word by word into LDS by the whole workgroup when
 // it fits (tuning.lds_tile words); otherwise the same code reads it from
 // global memory.
-constexpr uint32_t kDefaultEventWords = 6144;   // estimate: 24 KB of LDS per workgroup
+constexpr uint32_t kDefaultEventWords = 5632;   // estimate: 22 KB of LDS per workgroup (six per CU)
 constexpr uint32_t kDefaultClusterWords = 6144; // clustering: 24 KB (event and candidate list)
 constexpr uint32_t kClusterEntries = kWave; // candidate entries per wave
 
@@ -424,12 +424,9 @@ __device__ inline void stage_words(uint32_t* dst, const uint32_t* src, uint32_t 
 // no search).  Flat order is bank order, which is module order, so the
 // candidate slots are one running count per wave.
 struct FlatBanks {
+  static constexpr bool kByBank = false;
   uint32_t start[VELO_N_SENSORS + 1]; // by k; start[nnz] = number of SPs
   uint32_t delta[VELO_N_SENSORS];     // by k
-  uint32_t bstart[VELO_N_SENSORS + 1]; // flat index of the first SP, by bank; [208] = number of SPs
-  uint32_t bbase[VELO_N_SENSORS];     // event word of the first SP, by bank
-  uint32_t bn[VELO_N_SENSORS];        // SP count, by bank
-  uint32_t mst[VELO_N_MODULES + 1];   // clustering: the event's module cluster starts and end
   uint32_t unsorted[(VELO_N_SENSORS + 31) / 32]; // bit b: bank b's addresses do not strictly increase
   uint32_t est[VELO_N_MODULES];       // estimate: estimates; clustering: cluster counts
   uint32_t cand[VELO_N_MODULES];
@@ -438,6 +435,16 @@ struct FlatBanks {
   uint8_t bank[VELO_N_SENSORS];       // by k
 };
 constexpr uint32_t kFlatZeroWords = (VELO_N_SENSORS + 31) / 32 + 2 * VELO_N_MODULES;
+
+// Clustering's table adds per-bank entries (estimate_input_size keeps the
+// smaller one: less LDS per workgroup, more events per CU).
+struct ClusterBanks : FlatBanks {
+  static constexpr bool kByBank = true;
+  uint32_t bstart[VELO_N_SENSORS + 1]; // flat index of the first SP, by bank; [208] = number of SPs
+  uint32_t bbase[VELO_N_SENSORS];     // event word of the first SP, by bank
+  uint32_t bn[VELO_N_SENSORS];        // SP count, by bank
+  uint32_t mst[VELO_N_MODULES + 1];   // the event's module cluster starts and end
+};
 
 // OR over the wave (the same value on every lane)
 __device__ inline uint32_t wave_or(uint32_t v)
@@ -496,8 +503,8 @@ __device__ inline uint32_t flat_bank_of(const FlatBanks& fb, uint32_t nnz, uint3
 }
 
 // Bank table of the event in flat form, by wave 0.
-template<typename P>
-__device__ inline void build_flat_banks(P ev, uint32_t nwords, FlatBanks& fb)
+template<typename P, typename B>
+__device__ inline void build_flat_banks(P ev, uint32_t nwords, B& fb)
 {
   const uint32_t lane = lane_id();
   const uint32_t n_banks = nwords ? ev[0] : 0u;
@@ -512,10 +519,12 @@ __device__ inline void build_flat_banks(P ev, uint32_t nwords, FlatBanks& fb)
     const uint32_t inc = wave_inclusive_scan(n);
     const uint32_t s = total + inc - n;
     const uint32_t k = nnz + count_below(nz);
-    if (b < VELO_N_SENSORS) {
-      fb.bstart[b] = s;
-      fb.bbase[b] = base;
-      fb.bn[b] = n;
+    if constexpr (B::kByBank) {
+      if (b < VELO_N_SENSORS) {
+        fb.bstart[b] = s;
+        fb.bbase[b] = base;
+        fb.bn[b] = n;
+      }
     }
     if (n) {
       fb.start[k] = s;
@@ -528,7 +537,7 @@ __device__ inline void build_flat_banks(P ev, uint32_t nwords, FlatBanks& fb)
   st = wave_or(st);
   if (lane == 0) {
     fb.start[nnz] = total;
-    fb.bstart[VELO_N_SENSORS] = total;
+    if constexpr (B::kByBank) fb.bstart[VELO_N_SENSORS] = total;
     fb.nnz = nnz;
     fb.total = total;
     fb.status = st;
@@ -727,7 +736,7 @@ __device__ inline uint32_t lane_above(uint32_t v)
 
 // First module of wave w's share: the module boundary nearest to w/W of the
 // event's SPs (boundaries are flat starts of banks 4m).
-__device__ inline uint32_t wave_first_module(const FlatBanks& fb, uint32_t waves, uint32_t w)
+__device__ inline uint32_t wave_first_module(const ClusterBanks& fb, uint32_t waves, uint32_t w)
 {
   if (w == 0) return 0;
   if (w >= waves) return VELO_N_MODULES;
@@ -742,7 +751,7 @@ __device__ inline uint32_t wave_first_module(const FlatBanks& fb, uint32_t waves
 
 template<typename P>
 __device__ __forceinline__ void cluster_event(
-  P ev, FlatBanks& fb, const velo_masked_clustering_args& a, uint32_t e, uint32_t* entries,
+  P ev, ClusterBanks& fb, const velo_masked_clustering_args& a, uint32_t e, uint32_t* entries,
   const uint32_t* cl, const uint32_t* cl_mod, bool check_order)
 {
   const uint32_t waves = blockDim.x / kWave;
@@ -955,7 +964,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
 masked_clustering_kernel(velo_masked_clustering_args a)
 {
   extern __shared__ uint32_t lds[];
-  __shared__ FlatBanks fb;
+  __shared__ ClusterBanks fb;
   for_each_event(a.tuning, a.n_events, [&](const uint32_t e) {
     const uint32_t ei = event_index(a.dev_event_list, e);
     const uint32_t off = a.dev_raw_input_offsets[ei];
