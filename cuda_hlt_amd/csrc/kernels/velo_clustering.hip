@@ -355,7 +355,7 @@ __global__ void __launch_bounds__(kOrderBlock)
 // it fits (tuning.lds_tile words); otherwise the same code reads it from
 // global memory.
 constexpr uint32_t kDefaultEventWords = 5632;   // estimate: 22 KB of LDS per workgroup (six per CU)
-constexpr uint32_t kDefaultClusterWords = 6144; // clustering: 24 KB (event and candidate list)
+constexpr uint32_t kDefaultClusterWords = 5632; // clustering: 22 KB (event and candidate list; six per CU)
 constexpr uint32_t kClusterEntries = kWave; // candidate entries per wave
 
 // Bank b of an event of nwords words.  A bank whose offset table, header
@@ -436,14 +436,14 @@ struct FlatBanks {
 };
 constexpr uint32_t kFlatZeroWords = (VELO_N_SENSORS + 31) / 32 + 2 * VELO_N_MODULES;
 
-// Clustering's table adds per-bank entries (estimate_input_size keeps the
-// smaller one: less LDS per workgroup, more events per CU).
+// Clustering's table adds each module's first flat SP, its cluster slots
+// and a bank -> k map (estimate_input_size keeps the smaller table: less LDS
+// per workgroup, more events per CU).
 struct ClusterBanks : FlatBanks {
   static constexpr bool kByBank = true;
-  uint32_t bstart[VELO_N_SENSORS + 1]; // flat index of the first SP, by bank; [208] = number of SPs
-  uint32_t bbase[VELO_N_SENSORS];     // event word of the first SP, by bank
-  uint32_t bn[VELO_N_SENSORS];        // SP count, by bank
+  uint32_t mflat[VELO_N_MODULES + 1]; // flat index of each module's first SP; [52] = number of SPs
   uint32_t mst[VELO_N_MODULES + 1];   // the event's module cluster starts and end
+  uint8_t kmap[VELO_N_SENSORS];       // bank -> k (0xFF: no SPs)
 };
 
 // OR over the wave (the same value on every lane)
@@ -521,9 +521,8 @@ __device__ inline void build_flat_banks(P ev, uint32_t nwords, B& fb)
     const uint32_t k = nnz + count_below(nz);
     if constexpr (B::kByBank) {
       if (b < VELO_N_SENSORS) {
-        fb.bstart[b] = s;
-        fb.bbase[b] = base;
-        fb.bn[b] = n;
+        fb.kmap[b] = n ? (uint8_t) k : (uint8_t) 0xFFu;
+        if (b % 4u == 0) fb.mflat[b / 4u] = s;
       }
     }
     if (n) {
@@ -537,7 +536,7 @@ __device__ inline void build_flat_banks(P ev, uint32_t nwords, B& fb)
   st = wave_or(st);
   if (lane == 0) {
     fb.start[nnz] = total;
-    if constexpr (B::kByBank) fb.bstart[VELO_N_SENSORS] = total;
+    if constexpr (B::kByBank) fb.mflat[VELO_N_MODULES] = total;
     fb.nnz = nnz;
     fb.total = total;
     fb.status = st;
@@ -742,8 +741,8 @@ __device__ inline uint32_t wave_first_module(const ClusterBanks& fb, uint32_t wa
   if (w >= waves) return VELO_N_MODULES;
   const uint32_t lane = lane_id();
   const uint32_t t = (uint32_t) (((uint64_t) fb.total * w) / waves);
-  const uint32_t ms = lane <= VELO_N_MODULES ? fb.bstart[4 * lane] : 0xFFFFFFFFu;
-  const uint32_t lo = (uint32_t) __popcll(ballot(ms <= t)) - 1u; // bstart[0] = 0
+  const uint32_t ms = lane <= VELO_N_MODULES ? fb.mflat[lane] : 0xFFFFFFFFu;
+  const uint32_t lo = (uint32_t) __popcll(ballot(ms <= t)) - 1u; // mflat[0] = 0
   if (lo >= VELO_N_MODULES) return VELO_N_MODULES;
   const uint32_t slo = __builtin_amdgcn_readlane(ms, lo), shi = __builtin_amdgcn_readlane(ms, lo + 1);
   return t - slo <= shi - t ? lo : lo + 1;
@@ -764,7 +763,7 @@ __device__ __forceinline__ void cluster_event(
   uint32_t* ids = a.dev_velo_cluster_container + 3 * (size_t) total;
   const uint32_t nnz = fb.nnz;
   const uint32_t m0 = wave_first_module(fb, waves, wave), m1 = wave_first_module(fb, waves, wave + 1);
-  const uint32_t fbeg = fb.bstart[4 * m0], fend = fb.bstart[4 * m1];
+  const uint32_t fbeg = fb.mflat[m0], fend = fb.mflat[m1];
   const uint32_t kfirst = fbeg < fend ? flat_bank_of(fb, nnz, fbeg) : 0u;
 
   if (check_order) {
@@ -839,12 +838,14 @@ __device__ __forceinline__ void cluster_event(
       si = ent >> 11;
       sb = (ent >> 3) & 0xFFu;
       kp = ent & 7u;
-      const uint32_t epos = ok && sb < VELO_N_SENSORS ? fb.bstart[sb] + si : 0xFFFFFFFFu;
+      const uint32_t ks = ok && sb < VELO_N_SENSORS ? fb.kmap[sb] : 0xFFu;
+      const uint32_t epos = ks != 0xFFu ? fb.start[ks] + si : 0xFFFFFFFFu;
       uint32_t limit = jb + kWave;
       if (cptr + kWave < cend) {
         const uint32_t e64 = cl[cptr + kWave];
         const uint32_t b64 = (e64 >> 3) & 0xFFu;
-        const uint32_t p64 = b64 < VELO_N_SENSORS ? fb.bstart[b64] + (e64 >> 11) : 0xFFFFFFFFu;
+        const uint32_t k64 = b64 < VELO_N_SENSORS ? fb.kmap[b64] : 0xFFu;
+        const uint32_t p64 = k64 != 0xFFu ? fb.start[k64] + (e64 >> 11) : 0xFFFFFFFFu;
         limit = p64 < limit ? p64 : limit;
       }
       limit = limit < fend ? limit : fend;
@@ -879,7 +880,11 @@ __device__ __forceinline__ void cluster_event(
     if (n_ent) {
       bool good = false;
       if (lane < n_ent) {
-        const uint32_t base = fb.bbase[sb], n = fb.bn[sb];
+        // a bank with a candidate has SPs (0xFF only for a list that does
+        // not match the banks: an empty window)
+        const uint32_t kb = sb < VELO_N_SENSORS ? fb.kmap[sb] : 0xFFu;
+        const uint32_t base = kb != 0xFFu ? fb.start[kb] + fb.delta[kb] : 0u;
+        const uint32_t n = kb != 0xFFu ? fb.start[kb + 1] - fb.start[kb] : 0u;
         // the reference decodes the candidate's SP index as uint8_t (:179)
         const uint32_t cw = ev[base + (si & 0xFFu)];
         const uint32_t caddr = (cw & 0x007FFF00u) >> 8;
