@@ -84,7 +84,7 @@ extern "C" {
  * is only the capacity. */
 typedef struct {
   uint32_t block_dim; /* threads per workgroup, multiple of 64; 0 = default */
-  uint32_t lds_tile;  /* per-wave LDS staging capacity in elements; 0 = default */
+  uint32_t lds_tile;  /* LDS staging capacity in elements (event words, hits or phis, per kernel); 0 = default */
   uint32_t grid_dim;  /* 0: one workgroup per event; >0: persistent workgroups */
   uint32_t reserved;  /* must be 0 */
   uint32_t* dev_work_counter;     /* persistent launches: 2 u32, zero at the first launch */
