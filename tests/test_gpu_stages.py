@@ -476,6 +476,44 @@ def test_null_argument_is_rejected(stages):
         nat.launch("velo_search_by_triplet_launch", nat.SearchByTripletArgs())
 
 
+def _flat_edge_events():
+    """Events for the flat bank table: one SP in every bank (a 64-SP chunk
+    spans 64 banks), one bank of 300 SPs (a bank over several chunks and
+    waves), an empty event, SP pairs in every third bank, and a mix."""
+    from tests import rawevent
+    one_each = [(s, 10 + 2 * (s % 7), 20 + 4 * (s % 5)) for s in range(208)]
+    big = [(17, 2 * c, 4 * r) for c in range(30) for r in range(10)] + [(40, 5, 5), (41, 100, 30)]
+    pairs = [(s, 2 * (s % 50) + 1, 8) for s in range(0, 208, 3)] + [(s, 2 * (s % 50) + 2, 8) for s in range(0, 208, 3)]
+    mix = one_each[::2] + big + pairs[::5]
+    return rawevent.concat([rawevent.encode(p) for p in (one_each, big, [], pairs, mix)])
+
+
+@pytest.mark.parametrize("block,tile", [(64, 0), (256, 0), (1024, 0), (256, 32)])
+def test_flat_bank_table_edges(stages, consts, geometry, block, tile):
+    """estimate_input_size and masked clustering on the flat bank table's
+    edge cases equal the oracle, from LDS (tile 0) and global memory (32)."""
+    ev, off = _flat_edge_events()
+    n = len(off) - 1
+    oest, _, ocnum, ocand, ost = po.estimate_input_size(ev, off)
+    est, num, cnum, cand, st = stages.estimate(ev, off, consts, tuning=(block, tile))
+    np.testing.assert_array_equal(est[: n * N_MODULES], oest[: n * N_MODULES])
+    assert not np.any(num[: n * N_MODULES])
+    np.testing.assert_array_equal(cnum, ocnum)
+    for e in range(n):
+        k = int(ocnum[e])
+        np.testing.assert_array_equal(cand[e * 3000: e * 3000 + k], ocand[e * 3000: e * 3000 + k])
+    np.testing.assert_array_equal(st, ost)
+    starts = po.prefix_sum(oest)
+    onums, ocont, _ = po.masked_clustering(ev, off, starts, geometry)
+    total = int(starts[n * N_MODULES])
+    mask = module_mask(n, starts, onums)
+    for cands in (None, (ocnum, ocand)):
+        nums, cont, _ = stages.clustering(ev, off, starts, consts, (block, tile), cands, status0=ost)
+        np.testing.assert_array_equal(nums[: n * N_MODULES], onums)
+        for k in range(4):
+            np.testing.assert_array_equal(cont[k * total: (k + 1) * total][mask], ocont[k * total: (k + 1) * total][mask])
+
+
 def test_huge_bank_is_skipped(stages, consts, geometry):
     """A bank of 2^22 SPs (inside its 16 MB event) is flagged BAD_BANK and
     skipped as the oracle does; the other banks decode as usual."""
