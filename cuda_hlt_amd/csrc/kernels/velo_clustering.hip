@@ -547,7 +547,10 @@ template<typename P>
 __device__ __forceinline__ void estimate_event_flat(
   P ev, FlatBanks& fb, const velo_estimate_input_size_args& a, uint32_t e, uint32_t* stash)
 {
-  // stash: as in estimate_event
+  // stash: the event's LDS copy (null when the event is read from global
+  // memory).  The estimate pass keeps each non-isolated SP's candidate byte
+  // in bits 23-30 of its word there, bits no reader of the copy looks at,
+  // so the candidate pass need not compute it again.
   const uint32_t waves = blockDim.x / kWave;
   const uint32_t wave = threadIdx.x / kWave;
   const uint32_t lane = lane_id();
