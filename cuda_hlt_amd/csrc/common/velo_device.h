@@ -245,6 +245,67 @@ __device__ inline uint32_t wave_inclusive_max(uint32_t v)
   return v;
 }
 
+// OR over the wave (the same value on every lane)
+__device__ inline uint32_t wave_or(uint32_t v)
+{
+  v |= (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x111, 0xf, 0xf, false);
+  v |= (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x112, 0xf, 0xf, false);
+  v |= (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x114, 0xf, 0xf, false);
+  v |= (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x118, 0xf, 0xf, false);
+  v |= (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x142, 0xa, 0xf, false);
+  v |= (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x143, 0xc, 0xf, false);
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
+// Runs k = 0..nk-1 of a flat list (SPs of a bank, hits of a track): run k
+// holds positions [start[k], start[k+1]), start[nk] = the list's length,
+// and the starts increase strictly (no empty run).  Lane j of a wave takes
+// position f0 + j; flat_run gives its run k and the run's range [s, e)
+// from the run starts that fall inside the 64 positions (one LDS read, a
+// wave OR of their bits, popcounts; no search).
+struct FlatRun {
+  uint32_t k, s, e;
+};
+
+// kcur (the same on every lane): the run holding position f0; *knext: the
+// run holding position f0 + 64.
+__device__ inline FlatRun flat_run(const uint32_t* start, uint32_t nk, uint32_t f0, uint32_t kcur, uint32_t* knext)
+{
+  const uint32_t lane = lane_id();
+  const uint32_t kk = kcur + lane;
+  // starts increase strictly, so lanes 1.. hold starts above f0 and the
+  // ones inside the chunk come first
+  const uint32_t sl = kk <= nk ? start[kk] : 0xFFFFFFFFu;
+  const uint32_t p = sl - f0;
+  const bool inside = lane > 0 && p < kWave;
+  const uint32_t lo = inside && p < 32u ? 1u << p : 0u;
+  const uint32_t hi = inside && p >= 32u ? 1u << (p - 32u) : 0u;
+  const uint64_t bits = ((uint64_t) wave_or(hi) << 32) | (uint64_t) wave_or(lo);
+  const uint64_t upto = lane == kWave - 1 ? ~0ull : (2ull << lane) - 1ull;
+  const uint64_t below = bits & upto, above = bits & ~upto;
+  const uint32_t inner = (uint32_t) __popcll(bits);
+  uint32_t next; // first run start past the chunk
+  if (inner + 1 < kWave) next = __builtin_amdgcn_readlane(sl, inner + 1);
+  else next = kcur + kWave <= nk ? start[kcur + kWave] : 0xFFFFFFFFu;
+  FlatRun r;
+  r.k = kcur + (uint32_t) __popcll(below);
+  r.s = below ? f0 + 63u - (uint32_t) __clzll(below) : __builtin_amdgcn_readlane(sl, 0);
+  r.e = above ? f0 + (uint32_t) __ffsll((unsigned long long) above) - 1u : next;
+  *knext = kcur + inner + (next == f0 + kWave ? 1u : 0u);
+  return r;
+}
+
+// The run holding position f (f below the list's length).
+__device__ inline uint32_t flat_run_first(const uint32_t* start, uint32_t nk, uint32_t f)
+{
+  uint32_t c = 0;
+  for (uint32_t k0 = 0; k0 < nk; k0 += kWave) {
+    const uint32_t k = k0 + lane_id();
+    c += (uint32_t) __popcll(ballot(k < nk && start[k] <= f));
+  }
+  return c - 1u;
+}
+
 // Orders this wave's LDS and global accesses (a wave-local barrier).
 __device__ inline void wave_sync()
 {

@@ -446,60 +446,19 @@ struct ClusterBanks : FlatBanks {
   uint8_t kmap[VELO_N_SENSORS];       // bank -> k (0xFF: no SPs)
 };
 
-// OR over the wave (the same value on every lane)
-__device__ inline uint32_t wave_or(uint32_t v)
-{
-  v |= (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x111, 0xf, 0xf, false);
-  v |= (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x112, 0xf, 0xf, false);
-  v |= (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x114, 0xf, 0xf, false);
-  v |= (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x118, 0xf, 0xf, false);
-  v |= (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x142, 0xa, 0xf, false);
-  v |= (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x143, 0xc, 0xf, false);
-  return __builtin_amdgcn_readlane(v, 63);
-}
+// The flat SP index of a lane: flat_run over the starts of the banks with
+// SPs (velo_device.h).
+using FlatSp = FlatRun;
 
-// The lane's SP f0 + lane: its bank k and the bank's flat range [s, e).
-struct FlatSp {
-  uint32_t k, s, e;
-};
-
-// kcur (the same on every lane): the bank holding SP f0; *knext: the bank
-// holding SP f0 + 64.
 __device__ inline FlatSp flat_sp(const FlatBanks& fb, uint32_t nnz, uint32_t f0, uint32_t kcur, uint32_t* knext)
 {
-  const uint32_t lane = lane_id();
-  const uint32_t kk = kcur + lane;
-  // starts increase strictly, so lanes 1.. hold starts above f0 and the
-  // ones inside the chunk come first
-  const uint32_t sl = kk <= nnz ? fb.start[kk] : 0xFFFFFFFFu;
-  const uint32_t p = sl - f0;
-  const bool inside = lane > 0 && p < kWave;
-  const uint32_t lo = inside && p < 32u ? 1u << p : 0u;
-  const uint32_t hi = inside && p >= 32u ? 1u << (p - 32u) : 0u;
-  const uint64_t bits = ((uint64_t) wave_or(hi) << 32) | (uint64_t) wave_or(lo);
-  const uint64_t upto = lane == kWave - 1 ? ~0ull : (2ull << lane) - 1ull;
-  const uint64_t below = bits & upto, above = bits & ~upto;
-  const uint32_t inner = (uint32_t) __popcll(bits);
-  uint32_t next; // first bank start past the chunk
-  if (inner + 1 < kWave) next = __builtin_amdgcn_readlane(sl, inner + 1);
-  else next = kcur + kWave <= nnz ? fb.start[kcur + kWave] : 0xFFFFFFFFu;
-  FlatSp r;
-  r.k = kcur + (uint32_t) __popcll(below);
-  r.s = below ? f0 + 63u - (uint32_t) __clzll(below) : __builtin_amdgcn_readlane(sl, 0);
-  r.e = above ? f0 + (uint32_t) __ffsll((unsigned long long) above) - 1u : next;
-  *knext = kcur + inner + (next == f0 + kWave ? 1u : 0u);
-  return r;
+  return flat_run(fb.start, nnz, f0, kcur, knext);
 }
 
 // The bank holding SP f (f below the number of SPs).
 __device__ inline uint32_t flat_bank_of(const FlatBanks& fb, uint32_t nnz, uint32_t f)
 {
-  uint32_t c = 0;
-  for (uint32_t k0 = 0; k0 < nnz; k0 += kWave) {
-    const uint32_t k = k0 + lane_id();
-    c += (uint32_t) __popcll(ballot(k < nnz && fb.start[k] <= f));
-  }
-  return c - 1u;
+  return flat_run_first(fb.start, nnz, f);
 }
 
 // Bank table of the event in flat form, by wave 0.

@@ -163,10 +163,12 @@ struct TrackHitsOf {
 };
 
 // Filter: score tracks with the ghost model at a.dev_ghost_model
+// write_hits: each lane stores its track's hits (else the flat pass below
+// stores them).
 template<bool Filter, typename V>
 __device__ __forceinline__ void consolidate_event(
   const V& hv, const velo_consolidate_velo_tracks_args& a, uint32_t e, uint32_t n, uint32_t toff,
-  uint32_t total_tracks, uint32_t total_hits)
+  uint32_t total_tracks, uint32_t total_hits, bool write_hits)
 {
   float* cx = reinterpret_cast<float*>(a.dev_velo_track_hits);
   float* cy = cx + total_hits;
@@ -192,10 +194,12 @@ __device__ __forceinline__ void consolidate_event(
       if (h >= nh) break;
       const uint32_t hi = hh[h];
       const float x = hv.x[hi], y = hv.y[hi], z = hv.z[hi];
-      cx[ho + h] = x;
-      cy[ho + h] = y;
-      cz[ho + h] = z;
-      cid[ho + h] = hv.id[hi];
+      if (write_hits) {
+        cx[ho + h] = x;
+        cy[ho + h] = y;
+        cz[ho + h] = z;
+        cid[ho + h] = hv.id[hi];
+      }
       const float wx = kParamW;
       const float wx_t_x = wx * x;
       const float wx_t_z = wx * z;
@@ -276,10 +280,45 @@ __device__ __forceinline__ void consolidate_event(
   }
 }
 
+// The consolidated hits of an event's tracks, one output slot per lane:
+// consecutive lanes store consecutive slots.  ends[0..n]: the tracks' first
+// slots relative to the event's first (ends[n] = its hit count); a lane
+// finds its track from the track starts inside its 64 slots (flat_run).
+template<typename V>
+__device__ inline void store_track_hits(
+  const V& hv, const uint8_t* tracks, const uint32_t* ends, uint32_t n, uint32_t ho0, float* cx, float* cy, float* cz,
+  uint32_t* cid)
+{
+  const uint32_t waves = blockDim.x / kWave, wave = threadIdx.x / kWave, lane = lane_id();
+  const uint32_t count = ends[n];
+  const uint32_t chunks = (count + kWave - 1) / kWave;
+  const uint32_t per_wave = (chunks + waves - 1) / waves;
+  const uint32_t c0 = wave * per_wave < chunks ? wave * per_wave : chunks;
+  const uint32_t c1 = c0 + per_wave < chunks ? c0 + per_wave : chunks;
+  if (c0 >= c1) return;
+  uint32_t kcur = flat_run_first(ends, n, c0 * kWave);
+  for (uint32_t c = c0; c < c1; ++c) {
+    const uint32_t f = c * kWave + lane;
+    const FlatRun r = flat_run(ends, n, c * kWave, kcur, &kcur);
+    if (f < count) {
+      const uint16_t* t = reinterpret_cast<const uint16_t*>(tracks + (size_t) r.k * VELO_TRACK_HITS_BYTES);
+      const uint32_t hi = t[1 + (f - r.s)];
+      cx[ho0 + f] = hv.x[hi];
+      cy[ho0 + f] = hv.y[hi];
+      cz[ho0 + f] = hv.z[hi];
+      cid[ho0 + f] = hv.id[hi];
+    }
+  }
+}
+
+// Events with up to this many tracks store their hits in the flat pass.
+constexpr uint32_t kFlatStoreTracks = 512;
+
 template<bool Filter>
 __device__ __forceinline__ void consolidate_body(const velo_consolidate_velo_tracks_args& a)
 {
   extern __shared__ uint32_t s_hits[];
+  __shared__ uint32_t s_ends[kFlatStoreTracks + 1];
   const uint32_t N = a.n_events;
   for_each_event(a.tuning, a.n_events, [&](const uint32_t e) {
     const uint32_t total_tracks = (uint32_t) a.dev_atomics_velo[2 * N];
@@ -295,23 +334,46 @@ __device__ __forceinline__ void consolidate_body(const velo_consolidate_velo_tra
     g.id = a.dev_velo_cluster_container + 2 * (size_t) total + off;
     g.x = reinterpret_cast<const float*>(a.dev_velo_cluster_container + 5 * (size_t) total + off);
     const uint32_t cap = a.tuning.lds_tile;
-    if (n > 0 && ev_hits <= cap) {
+    if (n == 0) return;
+    // the tracks' first output slots (flat store pass), when every track
+    // has a hit and there are not too many tracks
+    const uint32_t ho0 = a.dev_velo_track_hit_number[toff];
+    bool empty_track = false;
+    if (n <= kFlatStoreTracks) {
+      for (uint32_t i = threadIdx.x; i <= n; i += blockDim.x) {
+        const uint32_t hn = a.dev_velo_track_hit_number[toff + i];
+        s_ends[i] = hn - ho0;
+        if (i < n) empty_track |= a.dev_velo_track_hit_number[toff + i + 1] <= hn;
+      }
+    }
+    const bool staged = ev_hits <= cap;
+    if (staged) {
       for (uint32_t i = threadIdx.x; i < ev_hits; i += blockDim.x) {
         s_hits[i] = __float_as_uint(g.x[i]);
         s_hits[cap + i] = __float_as_uint(g.y[i]);
         s_hits[2 * cap + i] = __float_as_uint(g.z[i]);
         s_hits[3 * cap + i] = g.id[i];
       }
-      __syncthreads();
+    }
+    const bool flat = n <= kFlatStoreTracks && !__syncthreads_or(empty_track);
+    float* cx = reinterpret_cast<float*>(a.dev_velo_track_hits);
+    float* cy = cx + total_hits;
+    float* cz = cy + total_hits;
+    uint32_t* cid = reinterpret_cast<uint32_t*>(cz + total_hits);
+    const uint8_t* tracks =
+      static_cast<const uint8_t*>(a.dev_tracks) + (size_t) e * VELO_MAX_TRACKS * VELO_TRACK_HITS_BYTES;
+    if (staged) {
       SlotView l;
       l.x = reinterpret_cast<const float*>(s_hits);
       l.y = reinterpret_cast<const float*>(s_hits + cap);
       l.z = reinterpret_cast<const float*>(s_hits + 2 * cap);
       l.id = s_hits + 3 * cap;
-      consolidate_event<Filter>(l, a, e, n, toff, total_tracks, total_hits);
+      consolidate_event<Filter>(l, a, e, n, toff, total_tracks, total_hits, !flat);
+      if (flat) store_track_hits(l, tracks, s_ends, n, ho0, cx, cy, cz, cid);
     }
     else {
-      consolidate_event<Filter>(g, a, e, n, toff, total_tracks, total_hits);
+      consolidate_event<Filter>(g, a, e, n, toff, total_tracks, total_hits, !flat);
+      if (flat) store_track_hits(g, tracks, s_ends, n, ho0, cx, cy, cz, cid);
     }
   });
 }
