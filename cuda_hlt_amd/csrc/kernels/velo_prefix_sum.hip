@@ -336,10 +336,12 @@ __device__ __forceinline__ void consolidate_body(const velo_consolidate_velo_tra
     const uint32_t cap = a.tuning.lds_tile;
     if (n == 0) return;
     // the tracks' first output slots (flat store pass), when every track
-    // has a hit and there are not too many tracks
+    // has a hit and there are not too many tracks.  The filtering instance
+    // stores each track's hits in its own loop: its lanes spend most of the
+    // time on the network, and a pass after it would only lengthen the event.
     const uint32_t ho0 = a.dev_velo_track_hit_number[toff];
     bool empty_track = false;
-    if (n <= kFlatStoreTracks) {
+    if (!Filter && n <= kFlatStoreTracks) {
       for (uint32_t i = threadIdx.x; i <= n; i += blockDim.x) {
         const uint32_t hn = a.dev_velo_track_hit_number[toff + i];
         s_ends[i] = hn - ho0;
@@ -355,7 +357,9 @@ __device__ __forceinline__ void consolidate_body(const velo_consolidate_velo_tra
         s_hits[3 * cap + i] = g.id[i];
       }
     }
-    const bool flat = n <= kFlatStoreTracks && !__syncthreads_or(empty_track);
+    // the workgroup barrier after the staging, for every instance
+    const bool any_empty = __syncthreads_or(empty_track) != 0;
+    const bool flat = !Filter && n <= kFlatStoreTracks && !any_empty;
     float* cx = reinterpret_cast<float*>(a.dev_velo_track_hits);
     float* cy = cx + total_hits;
     float* cz = cy + total_hits;
