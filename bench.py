@@ -444,7 +444,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(events shard across GPUs with no exchange; see DESIGN.md).", file=sys.stderr)
+              "(events would shard across GPUs with no exchange: each GPU runs this same sequence "
+              "on its own share of the events).", file=sys.stderr)
         sys.exit(2)
 
     import torch
